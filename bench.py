@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s per RK3 step of the 256×256×128 FP32 free-convection LES
+(WENO5 + AnisotropicMinimumDissipation + BuoyancyTracer + FPlane + surface buoyancy flux,
+FFT pressure solver, Δt = 1 s) on one MI355X, through the C ABI.
+
+Protocol (benchmarking/src/utils.jl:84-101): `--warmup` untimed steps, device sync, `--steps`
+timed steps, device sync.  Inputs are resident in HBM before the timed region.  Per-phase device
+times are HIP events recorded on the model's stream inside the timed region; the roofline entry
+is the dominant kernel's.  The CPU baseline is the FP64 oracle (a CPU restatement of the
+reference algorithm) timed on a bounded sample of the same workload on this host.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "oceananigans.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+N_TRACERS = 1
+
+
+def algorithmic_bytes_per_cell_step(n_prognostic, word):
+    """Compulsory traffic per cell per RK3 step, W = 17n + 54 words (tendency 2n, RK update +
+    divergence 4n+1 (3n+1 on stage 1), 5-pass FFT Poisson 10, correction 7 per substage)."""
+    return (17 * n_prognostic + 54) * word
+
+
+def run_gpu(args):
+    import oceananigans_mi355x as om
+    from oceananigans_mi355x import workloads
+
+    N = tuple(int(x) for x in args.size.split(","))
+    ft = np.float32 if args.precision == "fp32" else np.float64
+    word = 4 if ft == np.float32 else 8
+    grid, kw, init = workloads.free_convection(N=N, float_type=ft)
+    model = om.NonhydrostaticModel(grid, **kw)
+    om.set_(model, **init)
+    dt = 1.0
+    if args.warmup:
+        model.time_step(dt, args.warmup)
+    model.set_phase_timing(True)
+    t0 = time.perf_counter()
+    model.time_step(dt, args.steps)          # synchronous: returns after the device has finished
+    t1 = time.perf_counter()
+    phases = model.phase_times()
+    ev_ms = model.last_step_ms()
+    wall = t1 - t0
+    ncell = N[0] * N[1] * N[2]
+    n = 3 + N_TRACERS
+    bytes_step = algorithmic_bytes_per_cell_step(n, word) * ncell
+    # dominant kernel: the phase with the largest device time
+    dom = max(phases, key=lambda p: phases[p][0])
+    dom_ms, dom_launches = phases[dom]
+    per_cell = {"tendencies": 2 * n * word,                       # read n fields, write n tendencies
+                "poisson": 10 * word, "correction": 7 * word,
+                "rk_update": (4 * n + 1) * word, "closure": (3 + 2 * N_TRACERS) * word}.get(dom, 2 * n * word)
+    dom_bytes = per_cell * ncell
+    dom_avg_s = dom_ms / max(dom_launches, 1) / 1e3
+    achieved = dom_bytes / dom_avg_s / 1e9 if dom_avg_s > 0 else 0.0
+    div = float(np.abs(model.divergence()).max())
+    finite = all(np.isfinite(model.velocities[v].interior()).all() for v in "uvw")
+    model.close()
+    return dict(N=N, ncell=ncell, wall=wall, ev_ms=ev_ms, phases=phases, dom=dom, dom_avg_ms=dom_avg_s * 1e3,
+                dom_bytes=dom_bytes, achieved=achieved, bytes_step=bytes_step, word=word, div=div, finite=finite)
+
+
+def run_cpu_baseline(args):
+    """FP64 oracle on a bounded sample of the same workload (test infrastructure, not the product)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cases import oracle_for
+    from oceananigans_mi355x import workloads
+
+    N = tuple(int(x) for x in args.cpu_sample.split(","))
+    grid, kw, init = workloads.free_convection(N=N, float_type=np.float64)
+    orc = oracle_for(grid, kw, init)
+    orc.time_step(1.0, 1)                     # warm-up step
+    t0 = time.perf_counter()
+    orc.time_step(1.0, args.cpu_steps)
+    t1 = time.perf_counter()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ncell = N[0] * N[1] * N[2]
+    return {"value": ncell * args.cpu_steps / (t1 - t0), "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (CPU restatement of the reference algorithm), same physics on {N[0]}x{N[1]}x{N[2]}, "
+                      f"{args.cpu_steps} RK3 steps after 1 warm-up step, {t1 - t0:.1f} s, OpenMP threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", default="256,256,128")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--cpu-sample", default="128,128,128")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    r = run_gpu(args)
+    ms_per_step = r["wall"] * 1e3 / args.steps
+    value = r["ncell"] * args.steps / r["wall"]
+    cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
+    step_gbs = r["bytes_step"] * args.steps / r["wall"] / 1e9
+    out = {
+        "metric": "cell-updates/s per RK3 step (256³-class FP32 LES: 256x256x128 WENO5+AMD free convection)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if r["word"] == 4 else "f64",
+        "data": "synthetic (seeded SplitMix64 initial buoyancy noise; the reference's configuration)",
+        "config": {"workload": "free_convection_LES", "grid": "x".join(map(str, r["N"])),
+                   "advection": "WENO5", "closure": "AnisotropicMinimumDissipation", "buoyancy": "BuoyancyTracer",
+                   "coriolis": "FPlane(f=1e-4)", "pressure_solver": "FFT", "dt": 1.0, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None, "kernel": r["dom"],
+                     "kernel_avg_ms": r["dom_avg_ms"], "algorithmic_bytes_per_launch": r["dom_bytes"]},
+        "step_roofline": {"algorithmic_bytes_per_cell_step": algorithmic_bytes_per_cell_step(3 + N_TRACERS, r["word"]),
+                          "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+        "phases_ms_per_step": {k: v[0] / args.steps for k, v in r["phases"].items()},
+        "device_ms_per_step": r["ev_ms"] / args.steps,
+        "max_abs_divergence": r["div"],
+        "finite": r["finite"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+/*
+ * oceananigans_mi355x.h — C ABI of the MI355X-native NonhydrostaticModel time step.
+ *
+ * One opaque handle per model.  All device memory is owned by the library;
+ * host buffers passed in are borrowed for the duration of the call only.
+ * Every entry point returns 0 on success and a negative OM_E* code on error;
+ * om_last_error(handle) gives the message.  No C++ exception crosses the ABI.
+ * One host thread per handle; all work is ordered on one HIP stream owned by
+ * the handle and every call is synchronous on return.
+ *
+ * Reference interfaces replaced (paths relative to the Oceananigans.jl tree):
+ *   om_create            NonhydrostaticModel(grid; advection, closure, buoyancy, coriolis,
+ *                        tracers, boundary_conditions)
+ *                        src/Models/NonhydrostaticModels/nonhydrostatic_model.jl:124-313
+ *   om_destroy           finalizer of the model (Julia GC); no reference counterpart
+ *   om_field_info        Field data / OffsetArray parent layout
+ *                        src/Grids/new_data.jl:11-17, src/Grids/grid_utils.jl:74-80
+ *   om_set_field         set!(field, array) src/Fields/set!.jl:127-146
+ *   om_get_field         interior(field) / parent(field) copy to host
+ *   om_fill_halos        fill_halo_regions!(fields; fill_normal_flow_bcs)
+ *                        src/BoundaryConditions/fill_halo_regions.jl:20-38
+ *   om_update_state      update_state!(model)
+ *                        src/Models/NonhydrostaticModels/update_nonhydrostatic_model_state.jl:22-62
+ *   om_compute_tendencies compute_tendencies!(model)
+ *                        src/Models/NonhydrostaticModels/compute_nonhydrostatic_tendencies.jl:12-40
+ *   om_rk3_substep       rk3_substep!(model, Δt, γ, ζ) → pressure_correction_rk3_substep!
+ *                        src/Models/NonhydrostaticModels/nonhydrostatic_rk3_substep.jl:31-63
+ *   om_cache_previous_tendencies cache_previous_tendencies!(model)
+ *                        src/Models/NonhydrostaticModels/cache_nonhydrostatic_tendencies.jl:21-31
+ *   om_pressure_correction compute_pressure_correction!(model, Δt) + make_pressure_correction!(model, Δt)
+ *                        src/Models/NonhydrostaticModels/pressure_correction.jl:6-20,82-106
+ *   om_time_step         time_step!(model::AbstractModel{<:RungeKutta3TimeStepper}, Δt)
+ *                        src/TimeSteppers/runge_kutta_3.jl:103-168
+ *   om_get_clock/om_set_clock  model.clock (time, iteration, stage, last_Δt)
+ *                        src/TimeSteppers/clock.jl:147-173
+ *   om_poisson_solve     solve!(ϕ, ::FFTBasedPoissonSolver / ::FourierTridiagonalPoissonSolver, b)
+ *                        src/Solvers/fft_based_poisson_solver.jl:94-124,
+ *                        src/Solvers/fourier_tridiagonal_poisson_solver.jl:231-260
+ *   om_divergence        divᶜᶜᶜ(i, j, k, grid, u, v, w) src/Operators/divergence_operators.jl:16-19
+ */
+#ifndef OCEANANIGANS_MI355X_H
+#define OCEANANIGANS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OM_ABI_VERSION 1
+
+/* status codes */
+#define OM_OK            0
+#define OM_EINVAL       -1   /* invalid argument or unsupported configuration */
+#define OM_EHIP         -2   /* HIP runtime failure; the handle is unusable afterwards */
+#define OM_ENOMEM       -3
+#define OM_ESTATE       -4   /* call not valid in the current state */
+#define OM_ENOTCONV     -5   /* iterative solver did not converge */
+
+/* topology per dimension (src/Grids/topology ... Periodic, Bounded, Flat) */
+#define OM_PERIODIC 0
+#define OM_BOUNDED  1
+#define OM_FLAT     2
+
+/* advection scheme (flux form) */
+#define OM_ADV_NONE       0
+#define OM_ADV_CENTERED2  1   /* Centered(order=2) */
+#define OM_ADV_WENO5      2   /* WENO(order=5): buffer WENO(order=3) → Centered(order=2), advecting velocity Centered(order=4) */
+
+/* turbulence closure */
+#define OM_CLOSURE_NONE                 0
+#define OM_CLOSURE_AMD                  1  /* AnisotropicMinimumDissipation(C=Cν=Cκ), Cb = nothing   */
+#define OM_CLOSURE_SMAGORINSKY_LILLY    2  /* SmagorinskyLilly(C, Cb, Pr)                             */
+#define OM_CLOSURE_SCALAR_DIFFUSIVITY   3  /* ScalarDiffusivity(ν, κ) (isotropic, constant)            */
+
+/* buoyancy */
+#define OM_BUOYANCY_NONE        0
+#define OM_BUOYANCY_TRACER      1  /* BuoyancyTracer, tracer index buoyancy_tracer              */
+#define OM_BUOYANCY_LINEAR_EOS  2  /* SeawaterBuoyancy(LinearEquationOfState(α, β)), tracers T,S */
+
+/* boundary condition kinds on the bottom/top of a Bounded z */
+#define OM_BC_DEFAULT   0  /* NoFlux for Center fields, Impenetrable for w */
+#define OM_BC_FLUX      1
+#define OM_BC_VALUE     2
+#define OM_BC_GRADIENT  3
+
+/* pressure solver */
+#define OM_SOLVER_FFT                  0
+#define OM_SOLVER_FOURIER_TRIDIAGONAL  1
+
+#define OM_MAX_TRACERS 8
+#define OM_MAX_FIELDS  (3 + OM_MAX_TRACERS)
+
+/* field identifiers for om_field_info / om_set_field / om_get_field */
+#define OM_FIELD_U        0
+#define OM_FIELD_V        1
+#define OM_FIELD_W        2
+#define OM_FIELD_TRACER0  3      /* tracer n is OM_FIELD_TRACER0 + n */
+#define OM_FIELD_GN       100    /* Gⁿ of prognostic field f is OM_FIELD_GN + f */
+#define OM_FIELD_GM       200    /* G⁻ of prognostic field f is OM_FIELD_GM + f */
+#define OM_FIELD_PNHS     300
+#define OM_FIELD_PHY      301    /* hydrostatic pressure anomaly pHY′ */
+#define OM_FIELD_NU       302    /* eddy viscosity νₑ */
+#define OM_FIELD_KAPPA0   303    /* eddy diffusivity κₑ of tracer n is OM_FIELD_KAPPA0 + n (AMD) */
+#define OM_FIELD_RHS      400    /* Poisson right-hand side / solution scratch (Center, interior only) */
+
+typedef struct {
+    int32_t precision;            /* 4 = FP32, 8 = FP64 (fixed at create) */
+    int32_t N[3];                 /* Nx, Ny, Nz */
+    int32_t H[3];                 /* halo widths (0 on a Flat dimension) */
+    int32_t topology[3];          /* OM_PERIODIC / OM_BOUNDED / OM_FLAT */
+    double  origin[3];            /* position of the first interior face in x, y, z */
+    double  extent[3];            /* Lx, Ly, Lz */
+    const double *z_faces;        /* NULL: uniform z; else Nz+1 increasing interior z faces */
+    int32_t advection;            /* OM_ADV_* */
+    int32_t closure;              /* OM_CLOSURE_* */
+    double  closure_params[4];    /* AMD: {Cν, Cκ}; SmagorinskyLilly: {C, Cb, Pr}; ScalarDiffusivity: {ν, κ} */
+    int32_t buoyancy;             /* OM_BUOYANCY_* */
+    int32_t buoyancy_tracer;      /* tracer index of b (BuoyancyTracer) */
+    int32_t T_tracer, S_tracer;   /* tracer indices of T and S (linear EOS) */
+    double  gravitational_acceleration;
+    double  thermal_expansion, haline_contraction;
+    int32_t coriolis;             /* 0 none, 1 FPlane */
+    double  f;                    /* Coriolis parameter of the FPlane */
+    int32_t n_tracers;
+    int32_t bc_bottom[OM_MAX_FIELDS];      /* OM_BC_* per prognostic field (u, v, w, tracers...) */
+    int32_t bc_top[OM_MAX_FIELDS];
+    double  bc_bottom_value[OM_MAX_FIELDS];
+    double  bc_top_value[OM_MAX_FIELDS];
+    int32_t pressure_solver;      /* OM_SOLVER_*; stretched z requires FOURIER_TRIDIAGONAL */
+    int32_t hydrostatic_pressure; /* -1: reference default rule; 0 off; 1 on */
+    int32_t reserved[8];
+} om_config_t;
+
+typedef struct {
+    void    *data;          /* device pointer to parent(field)[1,1,1] (first halo point) */
+    int32_t  element_size;  /* 4 or 8 */
+    int32_t  parent_size[3];/* Nx+2Hx(+1 on a Bounded Face axis), ... */
+    int32_t  offset[3];     /* index of the first parent element in reference indices (1 - H) */
+    int64_t  stride[3];     /* element strides (x fastest: 1, nx, nx*ny) */
+} om_field_info_t;
+
+typedef struct om_model om_model_t;
+
+/* lifecycle */
+int         om_create(const om_config_t *cfg, om_model_t **out);
+int         om_destroy(om_model_t *m);
+const char *om_last_error(const om_model_t *m);   /* per handle; om_last_error(NULL) gives the create error */
+int         om_abi_version(void);
+
+/* data movement (host buffers are FP64, x fastest) */
+int om_field_info(om_model_t *m, int32_t field, om_field_info_t *info);
+int om_set_field(om_model_t *m, int32_t field, const double *host, int32_t with_halos);
+int om_get_field(om_model_t *m, int32_t field, double *host, int32_t with_halos);
+
+/* time stepping (reference semantics) */
+int om_fill_halos(om_model_t *m, uint32_t field_mask, int32_t fill_normal_flow);
+int om_update_state(om_model_t *m);
+int om_compute_tendencies(om_model_t *m);
+int om_rk3_substep(om_model_t *m, double dt, double gamma, double zeta, int32_t first_stage);
+int om_cache_previous_tendencies(om_model_t *m);
+int om_pressure_correction(om_model_t *m, double dt);
+int om_time_step(om_model_t *m, double dt, int32_t nsteps);
+
+/* clock */
+int om_get_clock(const om_model_t *m, double *time, int64_t *iteration, int32_t *stage,
+                 double *last_dt, double *last_stage_dt);
+int om_set_clock(om_model_t *m, double time, int64_t iteration, int32_t stage,
+                 double last_dt, double last_stage_dt);
+
+/* solver building blocks */
+int om_poisson_solve(om_model_t *m, const double *rhs_interior, double *phi_interior);
+int om_divergence(om_model_t *m, double *div_interior);
+
+/* timing of the last om_time_step call (HIP events on the handle's stream).
+ * ms_total: the whole call.  With phase timing enabled, ms_per_phase[p] and launches_per_phase[p]
+ * (p < nphases ≤ OM_NPHASES) give the summed device time and the number of timed launches of
+ * each phase below.  No reference counterpart (the reference times whole steps only,
+ * benchmarking/src/utils.jl:84-101). */
+#define OM_PHASE_RK_UPDATE    0   /* RK3 update + flux-BC tendencies                        */
+#define OM_PHASE_POISSON      1   /* source term + transforms + eigenvalue / tridiagonal solve */
+#define OM_PHASE_CORRECTION   2   /* U -= ∇p, p /= Δt                                        */
+#define OM_PHASE_HALOS        3   /* halo fills                                              */
+#define OM_PHASE_CLOSURE      4   /* eddy viscosity / diffusivity                            */
+#define OM_PHASE_HYDROSTATIC  5   /* pHY′ column integral                                    */
+#define OM_PHASE_TENDENCIES   6   /* fused tendency kernel                                   */
+#define OM_NPHASES            7
+int om_set_phase_timing(om_model_t *m, int32_t enable);
+int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_phase, int32_t *launches_per_phase,
+                        int32_t nphases);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCEANANIGANS_MI355X_H */

@@ -1,0 +1,712 @@
+// om_api.cpp — C ABI of the MI355X NonhydrostaticModel time step (see include/oceananigans_mi355x.h).
+//
+// Owns the device state of one model: prognostic fields, tendencies Gⁿ/G⁻, pressures, closure
+// fields, grid metrics and the Poisson solver plan; sequences the kernels of one RK3 substage
+// on the handle's stream.  Reference sequencing:
+//   time_step!               src/TimeSteppers/runge_kutta_3.jl:103-168
+//   rk3_substep!             src/Models/NonhydrostaticModels/nonhydrostatic_rk3_substep.jl:31-63
+//   update_state!            src/Models/NonhydrostaticModels/update_nonhydrostatic_model_state.jl:22-62
+//   pressure correction      src/Models/NonhydrostaticModels/pressure_correction.jl:6-106
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "om_internal.h"
+#include "om_kernels.h"
+
+using namespace om;
+
+namespace {
+
+std::string g_create_error;
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPCHECK(expr)                                                      \
+    do {                                                                    \
+        hipError_t _e = (expr);                                             \
+        if (_e != hipSuccess) throw HipError{_e, #expr};                    \
+    } while (0)
+
+struct ArgError {
+    std::string msg;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Grid metrics in FT (src/Grids/grid_generation.jl:34-95 stretched, :117-156 uniform)
+// ---------------------------------------------------------------------------------------------
+template <class T> struct HostGrid {
+    int N[3], H[3], topo[3];
+    T dx, dy;
+    std::vector<T> dzc, dzf;   // index k + Hz (k from -Hz .. Nz + Hz), 0-based device k
+    T Lx, Ly, Lz;
+    bool stretched = false;
+};
+
+template <class T> static T spacing(double L, int N) { return (T)((long double)L / (long double)N); }
+
+template <class T> static void make_grid(HostGrid<T> &g, const om_config_t &c) {
+    for (int d = 0; d < 3; ++d) { g.N[d] = c.N[d]; g.topo[d] = c.topology[d]; g.H[d] = c.topology[d] == OM_FLAT ? 0 : c.H[d]; }
+    g.dx = g.topo[0] == OM_FLAT ? T(1) : spacing<T>(c.extent[0], c.N[0]);
+    g.dy = g.topo[1] == OM_FLAT ? T(1) : spacing<T>(c.extent[1], c.N[1]);
+    g.Lx = g.topo[0] == OM_FLAT ? T(1) : (T)c.extent[0];
+    g.Ly = g.topo[1] == OM_FLAT ? T(1) : (T)c.extent[1];
+    const int Nz = g.N[2], Hz = g.H[2];
+    const int n = Nz + 2 * Hz + 1;
+    g.dzc.assign(n, T(1));
+    g.dzf.assign(n, T(1));
+    if (g.topo[2] == OM_FLAT) { g.Lz = T(1); return; }
+    if (!c.z_faces) {
+        const T dz = spacing<T>(c.extent[2], Nz);
+        for (auto &x : g.dzc) x = dz;
+        for (auto &x : g.dzf) x = dz;
+        g.Lz = (T)c.extent[2];
+        return;
+    }
+    g.stretched = true;
+    // faces F[m] for reference face index m - Hz + 1, m = 0 .. Nz + 2Hz
+    std::vector<T> Fi(Nz + 1);
+    for (int q = 0; q <= Nz; ++q) Fi[q] = (T)c.z_faces[q];
+    std::vector<T> lo(Hz), hi(Hz);
+    for (int q = 0; q < Hz; ++q) {
+        if (g.topo[2] == OM_BOUNDED) { lo[q] = Fi[1] - Fi[0]; hi[q] = Fi[Nz] - Fi[Nz - 1]; }
+        else { lo[q] = Fi[Nz - Hz + q + 1] - Fi[Nz - Hz + q]; hi[q] = Fi[q + 1] - Fi[q]; }
+    }
+    std::vector<T> hir(hi.rbegin(), hi.rend());
+    std::vector<T> F;
+    for (int q = 0; q < Hz; ++q) { T s = 0; for (int r = q; r < Hz; ++r) s += lo[r]; F.push_back(Fi[0] - s); }
+    for (int q = 0; q <= Nz; ++q) F.push_back(Fi[q]);
+    std::vector<T> top;
+    for (int q = 0; q < Hz; ++q) { T s = 0; for (int r = q; r < Hz; ++r) s += hir[r]; top.push_back(Fi[Nz] + s); }
+    for (int q = Hz - 1; q >= 0; --q) F.push_back(top[q]);
+    const int TC = Nz + 2 * Hz;
+    std::vector<T> C(TC);
+    for (int q = 0; q < TC; ++q) C[q] = (F[q + 1] + F[q]) / T(2);
+    // device index kd = reference k - 1; array slot kd + Hz
+    for (int kd = -Hz; kd < Nz + Hz; ++kd) g.dzc[kd + Hz] = F[kd + Hz + 1] - F[kd + Hz];
+    for (int kd = -Hz + 1; kd < Nz + Hz; ++kd) g.dzf[kd + Hz] = C[kd + Hz] - C[kd + Hz - 1];
+    g.dzf[0] = g.dzf[1];
+    g.dzc[n - 1] = g.dzc[n - 2];
+    g.dzf[n - 1] = g.dzf[n - 2];
+    g.Lz = Fi[Nz] - Fi[0];
+}
+
+// poisson_eigenvalues.jl:8-32 (evaluated in double with L/N rounded in FT, then rounded to FT)
+template <class T> static std::vector<T> eigen(int N, T L, int topo, int extension) {
+    std::vector<T> lam(N * extension, T(0));
+    const double h = (double)(T)(L / T(N));
+    for (int m = 0; m < N * extension; ++m) {
+        double v = 0;
+        if (topo == OM_PERIODIC) v = std::pow(2 * std::sin(m * M_PI / N) / h, 2);
+        else if (topo == OM_BOUNDED) {
+            int mm = m < N ? m : 2 * N - m;   // even extension: λ(2N - m) = λ(m)
+            v = std::pow(2 * std::sin(mm * M_PI / (2.0 * N)) / h, 2);
+        }
+        lam[m] = (T)v;
+    }
+    return lam;
+}
+
+template <class T> static std::vector<typename Vec2<T>::type> twiddles(int n) {
+    std::vector<typename Vec2<T>::type> tw(n > 0 ? n : 1);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)n;
+        tw[m].x = (T)std::cos(a);
+        tw[m].y = (T)std::sin(a);
+    }
+    return tw;
+}
+
+struct ModelBase {
+    virtual ~ModelBase() {}
+};
+
+template <class T> struct ModelT : ModelBase {
+    om_config_t cfg;
+    HostGrid<T> hg;
+    DevGrid<T> g;
+    DevPhysics<T> ph;
+    StateView<T> s;
+    PoissonPlan<T> P;
+    hipStream_t st = nullptr;
+    int nt = 0, nf = 0;
+    bool has_pHY = false;
+    int pnx = 0, pny = 0, pnz = 0;
+    size_t fsize = 0;                  // elements per field
+    std::vector<void *> allocs;
+    T *fields_block = nullptr;
+    T *rhs_dense = nullptr, *phi_dense = nullptr;
+    // clock
+    double time = 0, last_dt = INFINITY, last_stage_dt = INFINITY;
+    int64_t iteration = 0;
+    int stage = 1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    // optional per-phase timing: a pool of events recorded around each phase on the stream
+    bool phase_timing = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    std::vector<int> evphase;
+    double phase_ms[OM_NPHASES] = {0};
+    int phase_n[OM_NPHASES] = {0};
+
+    struct PhaseScope {
+        ModelT *M;
+        int phase;
+        size_t slot;
+        PhaseScope(ModelT *m, int p) : M(m), phase(p), slot(0) {
+            if (!M->phase_timing) return;
+            if (M->evused + 2 > M->evpool.size()) {
+                for (int q = 0; q < 256; ++q) { hipEvent_t e; HIPCHECK(hipEventCreate(&e)); M->evpool.push_back(e); }
+                M->evphase.resize(M->evpool.size());
+            }
+            slot = M->evused;
+            M->evused += 2;
+            M->evphase[slot] = p;
+            HIPCHECK(hipEventRecord(M->evpool[slot], M->st));
+        }
+        ~PhaseScope() noexcept(false) {
+            if (!M->phase_timing) return;
+            HIPCHECK(hipEventRecord(M->evpool[slot + 1], M->st));
+        }
+    };
+    void collect_phase_times() {
+        for (int p = 0; p < OM_NPHASES; ++p) { phase_ms[p] = 0; phase_n[p] = 0; }
+        for (size_t q = 0; q + 1 < evused; q += 2) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, evpool[q], evpool[q + 1]));
+            phase_ms[evphase[q]] += ms;
+            phase_n[evphase[q]] += 1;
+        }
+        evused = 0;
+    }
+
+    ~ModelT() override {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        for (auto e : evpool) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    template <class U> U *dalloc(size_t n) {
+        void *p = nullptr;
+        HIPCHECK(hipMalloc(&p, n * sizeof(U) + 256));
+        HIPCHECK(hipMemset(p, 0, n * sizeof(U) + 256));
+        allocs.push_back(p);
+        return reinterpret_cast<U *>(p);
+    }
+    template <class U> const U *upload(const std::vector<U> &h) {
+        U *d = dalloc<U>(h.size());
+        HIPCHECK(hipMemcpy(d, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
+        return d;
+    }
+
+    T *field_origin(T *base) const { return base + (size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny; }
+    T *field_base(const T *origin) const { return const_cast<T *>(origin) - ((size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny); }
+
+    void init(const om_config_t &c) {
+        cfg = c;
+        nt = c.n_tracers;
+        nf = 3 + nt;
+        make_grid(hg, c);
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&ev0));
+        HIPCHECK(hipEventCreate(&ev1));
+        // device grid
+        g.Nx = hg.N[0]; g.Ny = hg.N[1]; g.Nz = hg.N[2];
+        g.Hx = hg.H[0]; g.Hy = hg.H[1]; g.Hz = hg.H[2];
+        g.tx = hg.topo[0]; g.ty = hg.topo[1]; g.tz = hg.topo[2];
+        pnx = g.Nx + 2 * g.Hx; pny = g.Ny + 2 * g.Hy; pnz = g.Nz + 2 * g.Hz + 1;
+        g.sy = pnx; g.sz = (int64_t)pnx * pny;
+        g.dx = hg.dx; g.dy = hg.dy; g.idx = T(1) / hg.dx; g.idy = T(1) / hg.dy;
+        g.Az = hg.dx * hg.dy; g.iAz = T(1) / g.Az;
+        std::vector<T> idzc(hg.dzc.size()), idzf(hg.dzf.size());
+        for (size_t q = 0; q < idzc.size(); ++q) { idzc[q] = T(1) / hg.dzc[q]; idzf[q] = T(1) / hg.dzf[q]; }
+        g.dzc = upload(hg.dzc) + g.Hz;
+        g.dzf = upload(hg.dzf) + g.Hz;
+        g.idzc = upload(idzc) + g.Hz;
+        g.idzf = upload(idzf) + g.Hz;
+        g.stretched = hg.stretched;
+        // physics
+        ph.advection = c.advection; ph.closure = c.closure; ph.buoyancy = c.buoyancy; ph.coriolis = c.coriolis;
+        ph.n_tracers = nt; ph.b_tracer = c.buoyancy_tracer; ph.T_tracer = c.T_tracer; ph.S_tracer = c.S_tracer;
+        ph.f = (T)c.f; ph.g = (T)c.gravitational_acceleration; ph.alpha = (T)c.thermal_expansion; ph.beta = (T)c.haline_contraction;
+        ph.c0 = (T)c.closure_params[0]; ph.c1 = (T)c.closure_params[1]; ph.c2 = (T)c.closure_params[2];
+        for (int f = 0; f < kMaxFields; ++f) {
+            ph.bc_bottom[f] = c.bc_bottom[f]; ph.bc_top[f] = c.bc_top[f];
+            ph.bc_bottom_value[f] = (T)c.bc_bottom_value[f]; ph.bc_top_value[f] = (T)c.bc_top_value[f];
+        }
+        if (c.hydrostatic_pressure < 0)
+            has_pHY = c.topology[2] == OM_BOUNDED && c.buoyancy != OM_BUOYANCY_NONE;
+        else has_pHY = c.hydrostatic_pressure == 1;
+        ph.hydrostatic = has_pHY ? 1 : 0;
+        // fields: one block; order U[nf], Gn[nf], Gm[nf], pNHS, pHY, nu, kap[nt]
+        fsize = (size_t)pnx * pny * pnz;
+        fsize = (fsize + 63) & ~(size_t)63;   // 256-byte aligned field starts
+        const int nblocks = 3 * nf + 3 + nt;
+        fields_block = dalloc<T>(fsize * nblocks);
+        memset(&s, 0, sizeof(s));
+        int q = 0;
+        for (int f = 0; f < nf; ++f) s.U[f] = field_origin(fields_block + fsize * (q++));
+        for (int f = 0; f < nf; ++f) s.Gn[f] = field_origin(fields_block + fsize * (q++));
+        for (int f = 0; f < nf; ++f) s.Gm[f] = field_origin(fields_block + fsize * (q++));
+        s.pNHS = field_origin(fields_block + fsize * (q++));
+        s.pHY = field_origin(fields_block + fsize * (q++));
+        s.nu = field_origin(fields_block + fsize * (q++));
+        for (int n = 0; n < nt; ++n) s.kap[n] = field_origin(fields_block + fsize * (q++));
+        // Poisson plan
+        const size_t ncell = (size_t)g.Nx * g.Ny * g.Nz;
+        rhs_dense = dalloc<T>(ncell);
+        phi_dense = dalloc<T>(ncell);
+        const int Mx = g.Nx / 2 + 1;
+        P.S = dalloc<typename Vec2<T>::type>((size_t)Mx * g.Ny * g.Nz);
+        const int xline = (g.Nx % 2 == 0) ? g.Nx / 2 : g.Nx;
+        P.tw_xh = upload(twiddles<T>(xline));
+        {
+            std::vector<typename Vec2<T>::type> post(Mx);
+            for (int k = 0; k < Mx; ++k) {
+                const double a = -2.0 * M_PI * (double)k / (double)g.Nx;
+                post[k].x = (T)std::cos(a);
+                post[k].y = (T)std::sin(a);
+            }
+            P.post = upload(post);
+        }
+        P.tw_y = upload(twiddles<T>(g.Ny));
+        P.tridiagonal = c.pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL ? 1 : 0;
+        P.Lz_line = g.tz == OM_BOUNDED ? 2 * g.Nz : g.Nz;
+        P.tw_z = upload(twiddles<T>(P.Lz_line));
+        P.lx = upload(eigen<T>(g.Nx, hg.Lx, g.tx, 1));
+        P.ly = upload(eigen<T>(g.Ny, hg.Ly, g.ty, 1));
+        P.lz = upload(eigen<T>(g.Nz, hg.Lz, g.tz, g.tz == OM_BOUNDED ? 2 : 1));
+        P.eps10 = T(10) * std::numeric_limits<T>::epsilon();
+        HIPCHECK(hipDeviceSynchronize());
+    }
+
+    Ctx<T> ctx() const {
+        Ctx<T> C;
+        C.g = g; C.s = s; C.ph = ph;
+        return C;
+    }
+
+    // ---- halo fills
+    void halos_prognostic(uint32_t mask, bool fill_normal) {
+        HaloJob<T> J;
+        memset(&J, 0, sizeof(J));
+        for (int f = 0; f < nf; ++f) {
+            if (!(mask & (1u << f))) continue;
+            const int n = J.nf++;
+            J.f[n] = s.U[f];
+            if (f == 2) { J.kind_bottom[n] = fill_normal ? -1 : -2; J.kind_top[n] = J.kind_bottom[n]; }
+            else {
+                J.kind_bottom[n] = cfg.bc_bottom[f]; J.kind_top[n] = cfg.bc_top[f];
+                J.val_bottom[n] = (T)cfg.bc_bottom_value[f]; J.val_top[n] = (T)cfg.bc_top_value[f];
+            }
+        }
+        PhaseScope ps(this, OM_PHASE_HALOS);
+        launch_halos(g, J, st);
+    }
+    void halos_aux(std::initializer_list<T *> fs) {
+        HaloJob<T> J;
+        memset(&J, 0, sizeof(J));
+        for (T *p : fs) {
+            const int n = J.nf++;
+            J.f[n] = p;
+            J.kind_bottom[n] = OM_BC_DEFAULT; J.kind_top[n] = OM_BC_DEFAULT;
+        }
+        PhaseScope ps(this, OM_PHASE_HALOS);
+        launch_halos(g, J, st);
+    }
+    void halos_kappa() {
+        if (nt == 0) return;
+        HaloJob<T> J;
+        memset(&J, 0, sizeof(J));
+        for (int n = 0; n < nt; ++n) { J.f[J.nf] = s.kap[n]; J.kind_bottom[J.nf] = OM_BC_DEFAULT; J.kind_top[J.nf] = OM_BC_DEFAULT; J.nf++; }
+        PhaseScope ps(this, OM_PHASE_HALOS);
+        launch_halos(g, J, st);
+    }
+    uint32_t all_mask() const { return (1u << nf) - 1u; }
+
+    // ---- phases
+    void compute_tendencies() { PhaseScope ps(this, OM_PHASE_TENDENCIES); launch_tendencies(ctx(), st); }
+    void update_state() {
+        halos_prognostic(all_mask(), false);
+        if (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
+            { PhaseScope ps(this, OM_PHASE_CLOSURE); launch_closure(ctx(), st); }
+            halos_aux({s.nu});
+            if (cfg.closure == OM_CLOSURE_AMD) halos_kappa();
+        }
+        if (has_pHY) {
+            { PhaseScope ps(this, OM_PHASE_HYDROSTATIC); launch_hydrostatic(ctx(), st); }
+            halos_aux({s.pHY});
+        }
+        compute_tendencies();
+    }
+    void pressure_correction(T dtau) {
+        halos_prognostic(0x7u, true);
+        const T eps = std::numeric_limits<T>::epsilon();
+        const T dtp = dtau > eps ? dtau : eps;
+        { PhaseScope ps(this, OM_PHASE_POISSON); launch_poisson(P, ctx(), (const T *)nullptr, s.pNHS, 0, dtp, st); }
+        halos_aux({s.pNHS});
+        { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, st); }
+    }
+    void rk3_substep(double dt, T gamma, T zeta, bool first) {
+        const T kdt = (T)dt;
+        const T dtau = first ? (T)(dt * (double)gamma) : (T)(dt * (double)(gamma + zeta));
+        { PhaseScope ps(this, OM_PHASE_RK_UPDATE); launch_rk3(ctx(), kdt, gamma, zeta, first ? 1 : 0, nf, st); }
+        pressure_correction(dtau);
+    }
+    void cache_previous_tendencies() {
+        // G⁻ ← Gⁿ by exchanging the buffers: the next tendency evaluation overwrites every
+        // interior point of Gⁿ, so the copy is not needed.
+        for (int f = 0; f < nf; ++f) { T *t = s.Gm[f]; s.Gm[f] = s.Gn[f]; s.Gn[f] = t; }
+    }
+    void time_step(double dt) {
+        const T g1 = T(8.0 / 15), g2 = T(5.0 / 12), g3 = T(3.0 / 4), z2 = T(-17.0 / 60), z3 = T(-5.0 / 12);
+        if (iteration == 0) {
+            last_dt = dt;
+            last_stage_dt = dt * (double)g1;
+            update_state();
+        }
+        const double tn1 = time + dt;
+        rk3_substep(dt, g1, T(0), true);
+        cache_previous_tendencies();
+        time += dt * (double)g1; stage = 2; last_stage_dt = dt * (double)g1;
+        update_state();
+        rk3_substep(dt, g2, z2, false);
+        cache_previous_tendencies();
+        time += dt * (double)(g2 + z2); stage = 3; last_stage_dt = dt * (double)(g2 + z2);
+        update_state();
+        rk3_substep(dt, g3, z3, false);
+        cache_previous_tendencies();
+        const double corrected = tn1 - time;
+        time += corrected; iteration += 1; stage = 1; last_dt = dt; last_stage_dt = corrected;
+        update_state();
+    }
+
+    T *field_ptr(int id, bool &is_w) {
+        is_w = false;
+        if (id >= 0 && id < nf) { is_w = id == 2; return s.U[id]; }
+        if (id >= OM_FIELD_GN && id < OM_FIELD_GN + nf) { is_w = id - OM_FIELD_GN == 2; return s.Gn[id - OM_FIELD_GN]; }
+        if (id >= OM_FIELD_GM && id < OM_FIELD_GM + nf) { is_w = id - OM_FIELD_GM == 2; return s.Gm[id - OM_FIELD_GM]; }
+        if (id == OM_FIELD_PNHS) return s.pNHS;
+        if (id == OM_FIELD_PHY) return s.pHY;
+        if (id == OM_FIELD_NU) return s.nu;
+        if (id >= OM_FIELD_KAPPA0 && id < OM_FIELD_KAPPA0 + nt) return s.kap[id - OM_FIELD_KAPPA0];
+        return nullptr;
+    }
+    // interior extent along z of a field (w on a Bounded z has Nz + 1 faces)
+    int nz_interior(bool is_w) const { return (is_w && g.tz == OM_BOUNDED) ? g.Nz + 1 : g.Nz; }
+    int nz_parent(bool is_w) const { return (is_w && g.tz == OM_BOUNDED) ? g.Nz + 2 * g.Hz + 1 : g.Nz + 2 * g.Hz; }
+
+    void set_field(int id, const double *h, bool with_halos) {
+        bool is_w;
+        T *o = field_ptr(id, is_w);
+        if (!o) throw ArgError{"unknown field id"};
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<T> par(fsize);
+        T *base = field_base(o);
+        HIPCHECK(hipMemcpy(par.data(), base, fsize * sizeof(T), hipMemcpyDeviceToHost));
+        if (with_halos) {
+            const size_t n = (size_t)pnx * pny * nz_parent(is_w);
+            for (size_t q = 0; q < n; ++q) par[q] = (T)h[q];
+        } else {
+            size_t t = 0;
+            const int nzi = nz_interior(is_w);
+            for (int k = 0; k < nzi; ++k) for (int j = 0; j < g.Ny; ++j) for (int i = 0; i < g.Nx; ++i)
+                par[(size_t)(i + g.Hx) + (size_t)pnx * ((size_t)(j + g.Hy) + (size_t)pny * (size_t)(k + g.Hz))] = (T)h[t++];
+        }
+        HIPCHECK(hipMemcpy(base, par.data(), fsize * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void get_field(int id, double *h, bool with_halos) {
+        bool is_w;
+        T *o = field_ptr(id, is_w);
+        if (!o) throw ArgError{"unknown field id"};
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<T> par(fsize);
+        HIPCHECK(hipMemcpy(par.data(), field_base(o), fsize * sizeof(T), hipMemcpyDeviceToHost));
+        if (with_halos) {
+            const size_t n = (size_t)pnx * pny * nz_parent(is_w);
+            for (size_t q = 0; q < n; ++q) h[q] = (double)par[q];
+        } else {
+            size_t t = 0;
+            const int nzi = nz_interior(is_w);
+            for (int k = 0; k < nzi; ++k) for (int j = 0; j < g.Ny; ++j) for (int i = 0; i < g.Nx; ++i)
+                h[t++] = (double)par[(size_t)(i + g.Hx) + (size_t)pnx * ((size_t)(j + g.Hy) + (size_t)pny * (size_t)(k + g.Hz))];
+        }
+    }
+};
+
+}  // namespace
+
+struct om_model {
+    int precision = 8;
+    ModelBase *impl = nullptr;
+    std::string err;
+    bool broken = false;
+};
+
+template <class F> static int guarded(om_model *m, F &&fn) {
+    if (!m) return OM_EINVAL;
+    if (m->broken) { m->err = "handle unusable after an earlier HIP failure: " + m->err; return OM_EHIP; }
+    try {
+        fn();
+        return OM_OK;
+    } catch (const HipError &e) {
+        m->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        m->broken = true;
+        return OM_EHIP;
+    } catch (const ArgError &e) {
+        m->err = e.msg;
+        return OM_EINVAL;
+    } catch (const std::bad_alloc &) {
+        m->err = "host allocation failed";
+        return OM_ENOMEM;
+    } catch (...) {
+        m->err = "unexpected C++ exception";
+        return OM_EINVAL;
+    }
+}
+
+#define WITH_MODEL(m, body)                                                          \
+    guarded(m, [&]() {                                                              \
+        if ((m)->precision == 8) { auto *M = static_cast<ModelT<double> *>((m)->impl); body; } \
+        else { auto *M = static_cast<ModelT<float> *>((m)->impl); body; }           \
+    })
+
+static std::string validate(const om_config_t *c) {
+    if (!c) return "null config";
+    if (c->precision != 4 && c->precision != 8) return "precision must be 4 (FP32) or 8 (FP64)";
+    for (int d = 0; d < 3; ++d) {
+        if (c->N[d] < 1) return "grid size must be positive";
+        if (c->topology[d] < 0 || c->topology[d] > 2) return "unknown topology";
+        if (c->topology[d] == OM_FLAT && c->N[d] != 1) return "a Flat dimension must have size 1";
+    }
+    if (c->topology[0] != OM_PERIODIC || c->topology[1] != OM_PERIODIC)
+        return "x and y must be Periodic on the MI355X path";
+    if (c->N[0] < 2) return "Nx must be at least 2";
+    if (c->n_tracers < 0 || c->n_tracers > OM_MAX_TRACERS) return "n_tracers out of range";
+    int need = c->advection == OM_ADV_WENO5 ? 3 : 1;
+    if (c->closure != OM_CLOSURE_NONE && need < 2) need = 2;
+    for (int d = 0; d < 3; ++d) {
+        if (c->topology[d] == OM_FLAT) continue;
+        if (c->H[d] < need) return "halo too small for the advection scheme / closure (WENO5 needs 3)";
+        if (c->topology[d] != OM_FLAT && c->advection == OM_ADV_WENO5 && c->N[d] < 3)
+            return "WENO5 needs at least 3 cells per non-Flat dimension";
+    }
+    if (c->closure != OM_CLOSURE_NONE && c->closure != OM_CLOSURE_SCALAR_DIFFUSIVITY && c->topology[2] == OM_FLAT)
+        return "AMD / Smagorinsky closures need a three-dimensional grid";
+    if (c->z_faces && c->topology[2] != OM_BOUNDED) return "stretched z requires a Bounded z";
+    if (c->z_faces && c->pressure_solver != OM_SOLVER_FOURIER_TRIDIAGONAL)
+        return "a stretched z grid needs the Fourier-tridiagonal pressure solver";
+    if (c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL && c->topology[2] != OM_BOUNDED)
+        return "the Fourier-tridiagonal solver needs a Bounded z";
+    if (c->buoyancy == OM_BUOYANCY_TRACER && (c->buoyancy_tracer < 0 || c->buoyancy_tracer >= c->n_tracers))
+        return "buoyancy tracer index out of range";
+    if (c->buoyancy == OM_BUOYANCY_LINEAR_EOS &&
+        (c->T_tracer < 0 || c->T_tracer >= c->n_tracers || c->S_tracer < 0 || c->S_tracer >= c->n_tracers))
+        return "T/S tracer index out of range";
+    return "";
+}
+
+extern "C" {
+
+int om_abi_version(void) { return OM_ABI_VERSION; }
+
+int om_create(const om_config_t *cfg, om_model_t **out) {
+    if (!out) return OM_EINVAL;
+    *out = nullptr;
+    std::string v = validate(cfg);
+    if (!v.empty()) { g_create_error = v; return OM_EINVAL; }
+    auto *m = new (std::nothrow) om_model();
+    if (!m) return OM_ENOMEM;
+    m->precision = cfg->precision;
+    try {
+        if (m->precision == 8) { auto *M = new ModelT<double>(); m->impl = M; M->init(*cfg); }
+        else { auto *M = new ModelT<float>(); m->impl = M; M->init(*cfg); }
+    } catch (const HipError &e) {
+        g_create_error = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        delete m->impl;
+        delete m;
+        return OM_EHIP;
+    } catch (const ArgError &e) {
+        g_create_error = e.msg;
+        delete m->impl;
+        delete m;
+        return OM_EINVAL;
+    }
+    *out = m;
+    return OM_OK;
+}
+
+int om_destroy(om_model_t *m) {
+    if (!m) return OM_EINVAL;
+    delete m->impl;
+    delete m;
+    return OM_OK;
+}
+
+const char *om_last_error(const om_model_t *m) {
+    if (!m) return g_create_error.c_str();
+    return m->err.c_str();
+}
+
+int om_field_info(om_model_t *m, int32_t field, om_field_info_t *info) {
+    if (!info) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        bool is_w;
+        auto *o = M->field_ptr(field, is_w);
+        if (!o) throw ArgError{"unknown field id"};
+        info->data = M->field_base(o);
+        info->element_size = m->precision;
+        info->parent_size[0] = M->pnx;
+        info->parent_size[1] = M->pny;
+        info->parent_size[2] = M->nz_parent(is_w);
+        info->offset[0] = 1 - M->g.Hx;
+        info->offset[1] = 1 - M->g.Hy;
+        info->offset[2] = 1 - M->g.Hz;
+        info->stride[0] = 1;
+        info->stride[1] = M->g.sy;
+        info->stride[2] = M->g.sz;
+    });
+}
+
+int om_set_field(om_model_t *m, int32_t field, const double *host, int32_t with_halos) {
+    if (!host) return OM_EINVAL;
+    return WITH_MODEL(m, M->set_field(field, host, with_halos != 0));
+}
+
+int om_get_field(om_model_t *m, int32_t field, double *host, int32_t with_halos) {
+    if (!host) return OM_EINVAL;
+    return WITH_MODEL(m, M->get_field(field, host, with_halos != 0));
+}
+
+int om_fill_halos(om_model_t *m, uint32_t mask, int32_t fill_normal_flow) {
+    return WITH_MODEL(m, {
+        M->halos_prognostic(mask & M->all_mask(), fill_normal_flow != 0);
+        if (mask & (1u << 16)) M->halos_aux({M->s.pNHS});
+        HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_update_state(om_model_t *m) {
+    return WITH_MODEL(m, { M->update_state(); HIPCHECK(hipStreamSynchronize(M->st)); HIPCHECK(hipGetLastError()); });
+}
+
+int om_compute_tendencies(om_model_t *m) {
+    return WITH_MODEL(m, { M->compute_tendencies(); HIPCHECK(hipStreamSynchronize(M->st)); HIPCHECK(hipGetLastError()); });
+}
+
+int om_rk3_substep(om_model_t *m, double dt, double gamma, double zeta, int32_t first_stage) {
+    return WITH_MODEL(m, {
+        using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        M->rk3_substep(dt, (TT)gamma, (TT)zeta, first_stage != 0);
+        HIPCHECK(hipStreamSynchronize(M->st));
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+int om_cache_previous_tendencies(om_model_t *m) { return WITH_MODEL(m, M->cache_previous_tendencies()); }
+
+int om_pressure_correction(om_model_t *m, double dt) {
+    return WITH_MODEL(m, {
+        using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        M->pressure_correction((TT)dt);
+        HIPCHECK(hipStreamSynchronize(M->st));
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+int om_time_step(om_model_t *m, double dt, int32_t nsteps) {
+    if (nsteps < 0) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        M->evused = 0;
+        HIPCHECK(hipEventRecord(M->ev0, M->st));
+        for (int q = 0; q < nsteps; ++q) M->time_step(dt);
+        HIPCHECK(hipEventRecord(M->ev1, M->st));
+        HIPCHECK(hipEventSynchronize(M->ev1));
+        HIPCHECK(hipGetLastError());
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, M->ev0, M->ev1));
+        M->last_ms = ms;
+        if (M->phase_timing) M->collect_phase_times();
+    });
+}
+
+int om_get_clock(const om_model_t *mc, double *time, int64_t *iteration, int32_t *stage, double *last_dt,
+                 double *last_stage_dt) {
+    om_model_t *m = const_cast<om_model_t *>(mc);
+    return WITH_MODEL(m, {
+        if (time) *time = M->time;
+        if (iteration) *iteration = M->iteration;
+        if (stage) *stage = M->stage;
+        if (last_dt) *last_dt = M->last_dt;
+        if (last_stage_dt) *last_stage_dt = M->last_stage_dt;
+    });
+}
+
+int om_set_clock(om_model_t *m, double time, int64_t iteration, int32_t stage, double last_dt, double last_stage_dt) {
+    return WITH_MODEL(m, {
+        M->time = time; M->iteration = iteration; M->stage = stage; M->last_dt = last_dt; M->last_stage_dt = last_stage_dt;
+    });
+}
+
+int om_poisson_solve(om_model_t *m, const double *rhs, double *phi) {
+    if (!rhs || !phi) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        const size_t n = (size_t)M->g.Nx * M->g.Ny * M->g.Nz;
+        std::vector<TT> h(n);
+        for (size_t q = 0; q < n; ++q) h[q] = (TT)rhs[q];
+        HIPCHECK(hipMemcpy(M->rhs_dense, h.data(), n * sizeof(TT), hipMemcpyHostToDevice));
+        launch_poisson(M->P, M->ctx(), M->rhs_dense, M->phi_dense, 1, (TT)1, M->st);
+        HIPCHECK(hipStreamSynchronize(M->st));
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpy(h.data(), M->phi_dense, n * sizeof(TT), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) phi[q] = (double)h[q];
+    });
+}
+
+int om_divergence(om_model_t *m, double *out) {
+    if (!out) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        const size_t n = (size_t)M->g.Nx * M->g.Ny * M->g.Nz;
+        launch_divergence(M->ctx(), M->rhs_dense, M->st);
+        HIPCHECK(hipStreamSynchronize(M->st));
+        HIPCHECK(hipGetLastError());
+        std::vector<TT> h(n);
+        HIPCHECK(hipMemcpy(h.data(), M->rhs_dense, n * sizeof(TT), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) out[q] = (double)h[q];
+    });
+}
+
+int om_set_phase_timing(om_model_t *m, int32_t enable) {
+    return WITH_MODEL(m, { M->phase_timing = enable != 0; M->evused = 0; });
+}
+
+int om_last_step_timing(const om_model_t *mc, double *ms_total, double *ms_per_phase, int32_t *launches_per_phase,
+                        int32_t nphases) {
+    om_model_t *m = const_cast<om_model_t *>(mc);
+    if (nphases < 0 || nphases > OM_NPHASES) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (ms_total) *ms_total = M->last_ms;
+        for (int p = 0; p < nphases; ++p) {
+            if (ms_per_phase) ms_per_phase[p] = M->phase_ms[p];
+            if (launches_per_phase) launches_per_phase[p] = M->phase_n[p];
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// om_internal.h — device-side views shared by the HIP kernels and the C-ABI host code.
+//
+// Memory layout (HBM): every 3D field is one dense block, x fastest, with the parent shape
+// (Nx+2Hx) × (Ny+2Hy) × (Nz+2Hz+1).  All fields share the same strides, so one index serves
+// Center and Face fields (a Center field simply never touches its last z plane).  The view
+// pointer `p` addresses the first interior point; device indices are 0-based (i = 0 is the
+// reference's i = 1), so halo points have negative indices or indices ≥ N.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/oceananigans_mi355x.h"
+
+namespace om {
+
+constexpr int kMaxTracers = OM_MAX_TRACERS;
+constexpr int kMaxFields = OM_MAX_FIELDS;
+
+template <class T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+// Grid description passed by value to every kernel.
+template <class T> struct DevGrid {
+    int Nx, Ny, Nz;
+    int Hx, Hy, Hz;
+    int tx, ty, tz;          // topology per axis
+    int64_t sy, sz;          // strides (elements) of the y and z axes
+    T dx, dy, idx, idy;      // uniform horizontal spacings and their reciprocals
+    T Az, iAz;               // Δx·Δy
+    const T *dzc, *dzf;      // Δzᶜ[k], Δzᶠ[k] for k = -Hz .. Nz+Hz (pointer offset so [k] is valid)
+    const T *idzc, *idzf;    // reciprocals
+    int stretched;
+};
+
+template <class T> struct DevField {
+    T *p;  // address of interior point (0,0,0)
+};
+
+// Physics parameters of the tendency / closure kernels.
+template <class T> struct DevPhysics {
+    int advection, closure, buoyancy, coriolis, hydrostatic;
+    int n_tracers, b_tracer, T_tracer, S_tracer;
+    T f, g, alpha, beta;
+    T c0, c1, c2;            // closure parameters (AMD: Cν, Cκ; Smag: C, Cb, Pr; constant: ν, κ)
+    int bc_bottom[kMaxFields], bc_top[kMaxFields];
+    T bc_bottom_value[kMaxFields], bc_top_value[kMaxFields];
+};
+
+template <class T> struct StateView {
+    T *U[kMaxFields];        // u, v, w, tracers (interior origin)
+    T *Gn[kMaxFields];
+    T *Gm[kMaxFields];
+    T *nu;                   // νₑ
+    T *kap[kMaxTracers];     // κₑ per tracer (AMD)
+    T *pHY;                  // hydrostatic pressure anomaly
+    T *pNHS;
+};
+
+__host__ __device__ inline int64_t idx3(int i, int j, int k, int64_t sy, int64_t sz) {
+    return (int64_t)i + (int64_t)j * sy + (int64_t)k * sz;
+}
+
+}  // namespace om

@@ -1,0 +1,376 @@
+// om_kernels.h — device operators shared by the kernel sources, and the launcher interface
+// used by the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "om_internal.h"
+
+namespace om {
+
+// ----------------------------------------------------------------------------------------
+// arithmetic helpers
+// ----------------------------------------------------------------------------------------
+// Approximate division for the WENO weights (the reference's BackendOptimizedDivision,
+// src/Utils/newton_div.jl:57).  FP32: hardware reciprocal (≤1 ulp) times the numerator;
+// FP64: IEEE division, so FP64 parity is not limited by it.
+template <class T> __device__ __forceinline__ T wdiv(T a, T b);
+template <> __device__ __forceinline__ float wdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> __device__ __forceinline__ double wdiv<double>(double a, double b) { return a / b; }
+
+// Reconstruction coefficients rounded to FT; the last entry of each tuple is 1 - Σ(others)
+// evaluated in FT, as the reference builds them (reconstruction_coefficients.jl:74-76).
+template <class T> struct RC {
+    static constexpr T a0 = T(1.0 / 3), a1 = T(5.0 / 6), a2 = T(1) - (a0 + a1);      // WENO5 stencil 0
+    static constexpr T b0 = T(-1.0 / 6), b1 = T(5.0 / 6), b2 = T(1) - (b0 + b1);     // stencil 1
+    static constexpr T c0 = T(1.0 / 3), c1 = T(-7.0 / 6), c2 = T(1) - (c0 + c1);     // stencil 2
+    static constexpr T d0 = T(0.5), d1 = T(1) - d0;                                   // WENO3 stencil 0
+    static constexpr T e0 = T(-0.5), e1 = T(1) - e0;                                  // WENO3 stencil 1
+    static constexpr T m1 = T(-1.0 / 12), s7 = T(7.0 / 12);
+    static constexpr T q_p1 = m1, q_0 = s7, q_m1 = s7, q_m2 = T(1) - ((m1 + s7) + s7); // Centered(4)
+    static constexpr T h_0 = T(0.5), h_m1 = T(1) - T(0.5);                            // Centered(2)
+    static constexpr T C50 = T(3.0 / 10), C51 = T(3.0 / 5), C52 = T(1.0 / 10);
+    static constexpr T C30 = T(2.0 / 3), C31 = T(1.0 / 3);
+    static constexpr T eps = T(1e-8f);
+};
+
+// Smoothness quadratic form.  FP64 evaluates it on raw stencil values, where it cancels
+// heavily when the values carry a large offset; it is therefore evaluated without FMA
+// contraction and in the reference's left-to-right order, so that identical inputs give
+// identically rounded indicators.
+template <class T> __device__ __forceinline__ T beta_q(T a, T b, T c, T k0, T k1, T k2, T k3, T k4, T k5) {
+#pragma clang fp contract(off)
+    return a * (k0 * a + k1 * b + k2 * c) + b * (k3 * b + k4 * c) + c * (k5 * c);
+}
+
+// WENO(order=5) value at face i from s0..s5 = ψ[i-3..i+2] (weno_interpolants.jl:348-354, 520-531)
+template <class T> __device__ __forceinline__ T weno5(T s0, T s1, T s2, T s3, T s4, T s5, bool left) {
+    typedef RC<T> K;
+    const T a0 = left ? s2 : s3, a1 = left ? s3 : s2, a2 = left ? s4 : s1;
+    const T b0 = left ? s1 : s4, b1 = left ? s2 : s3, b2 = left ? s3 : s2;
+    const T c0 = left ? s0 : s5, c1 = left ? s1 : s4, c2 = left ? s2 : s3;
+    T ba, bb, bc;
+    if (sizeof(T) == 4) {
+        // FP32: the central value of each stencil is subtracted first (weno_interpolants.jl:276-281)
+        ba = beta_q<T>(a0 - a1, T(0), a2 - a1, T(10), T(-31), T(11), T(25), T(-19), T(4));
+        bb = beta_q<T>(b0 - b1, T(0), b2 - b1, T(4), T(-13), T(5), T(13), T(-13), T(4));
+        bc = beta_q<T>(c0 - c1, T(0), c2 - c1, T(4), T(-19), T(11), T(25), T(-31), T(10));
+    } else {
+        ba = beta_q<T>(a0, a1, a2, T(10), T(-31), T(11), T(25), T(-19), T(4));
+        bb = beta_q<T>(b0, b1, b2, T(4), T(-13), T(5), T(13), T(-13), T(4));
+        bc = beta_q<T>(c0, c1, c2, T(4), T(-19), T(11), T(25), T(-31), T(10));
+    }
+    const T tau = fabs(ba - bc);
+    const T qa = wdiv<T>(tau, ba + K::eps), qb = wdiv<T>(tau, bb + K::eps), qc = wdiv<T>(tau, bc + K::eps);
+    const T al0 = K::C50 * (T(1) + qa * qa), al1 = K::C51 * (T(1) + qb * qb), al2 = K::C52 * (T(1) + qc * qc);
+    const T sinv = T(1) / ((al0 + al1) + al2);
+    const T pa = K::a0 * a0 + K::a1 * a1 + K::a2 * a2;
+    const T pb = K::b0 * b0 + K::b1 * b1 + K::b2 * b2;
+    const T pc = K::c0 * c0 + K::c1 * c1 + K::c2 * c2;
+    return (al0 * sinv) * pa + (al1 * sinv) * pb + (al2 * sinv) * pc;
+}
+
+template <class T> __device__ __forceinline__ T beta2_q(T a, T b) {
+#pragma clang fp contract(off)
+    return a * (T(1) * a + T(-2) * b) + b * (T(1) * b);
+}
+
+// WENO(order=3) value at face i from s0..s3 = ψ[i-2..i+1]
+template <class T> __device__ __forceinline__ T weno3(T s0, T s1, T s2, T s3, bool left) {
+    typedef RC<T> K;
+    const T a0 = left ? s1 : s2, a1 = left ? s2 : s1;
+    const T b0 = left ? s0 : s3, b1 = left ? s1 : s2;
+    T ba, bb;
+    if (sizeof(T) == 4) {
+        const T da = a0 - a1, db = b0 - b1;
+        ba = da * (T(1) * da + T(-2) * T(0)) + T(0);
+        bb = db * (T(1) * db + T(-2) * T(0)) + T(0);
+    } else {
+        ba = beta2_q<T>(a0, a1);
+        bb = beta2_q<T>(b0, b1);
+    }
+    const T tau = fabs(ba - bb);
+    const T qa = wdiv<T>(tau, ba + K::eps), qb = wdiv<T>(tau, bb + K::eps);
+    const T al0 = K::C30 * (T(1) + qa * qa), al1 = K::C31 * (T(1) + qb * qb);
+    const T sinv = T(1) / (al0 + al1);
+    return (al0 * sinv) * (K::d0 * a0 + K::d1 * a1) + (al1 * sinv) * (K::e0 * b0 + K::e1 * b1);
+}
+
+template <class T> __device__ __forceinline__ T cen4(T m2, T m1, T z0, T p1) {
+    typedef RC<T> K;
+    return K::q_m2 * m2 + K::q_m1 * m1 + K::q_0 * z0 + K::q_p1 * p1;
+}
+template <class T> __device__ __forceinline__ T cen2(T m1, T z0) {
+    typedef RC<T> K;
+    return K::h_m1 * m1 + K::h_0 * z0;
+}
+
+// ----------------------------------------------------------------------------------------
+// Context: grid + state + physics with the reference's operators as member functions
+// ----------------------------------------------------------------------------------------
+template <class T> struct Ctx {
+    DevGrid<T> g;
+    StateView<T> s;
+    DevPhysics<T> ph;
+
+    __device__ __forceinline__ int64_t at(int i, int j, int k) const { return idx3(i, j, k, g.sy, g.sz); }
+    __device__ __forceinline__ T ld(const T *f, int i, int j, int k) const { return f[at(i, j, k)]; }
+    __device__ __forceinline__ bool flx() const { return g.tx == OM_FLAT; }
+    __device__ __forceinline__ bool fly() const { return g.ty == OM_FLAT; }
+    __device__ __forceinline__ bool flz() const { return g.tz == OM_FLAT; }
+    __device__ __forceinline__ bool bz() const { return g.tz == OM_BOUNDED; }
+
+    // metric operators (area at z-Center / z-Face, volume)
+    __device__ __forceinline__ T Axc(int k) const { return g.dy * g.dzc[k]; }
+    __device__ __forceinline__ T Axf(int k) const { return g.dy * g.dzf[k]; }
+    __device__ __forceinline__ T Ayc(int k) const { return g.dx * g.dzc[k]; }
+    __device__ __forceinline__ T Ayf(int k) const { return g.dx * g.dzf[k]; }
+    __device__ __forceinline__ T iVc(int k) const { return T(1) / (g.Az * g.dzc[k]); }
+    __device__ __forceinline__ T iVf(int k) const { return T(1) / (g.Az * g.dzf[k]); }
+
+    // z boundary switching: 0 = 5th-order (or Centered4), 1 = 3rd-order, 2 = Centered2
+    __device__ __forceinline__ int zb_face(int k) const {
+        if (!bz()) return 0;
+        const int kr = k + 1, N = g.Nz;
+        return (kr >= 4 && kr <= N - 2) ? 0 : ((kr >= 3 && kr <= N - 1) ? 1 : 2);
+    }
+    __device__ __forceinline__ int zb_center(int k) const {
+        if (!bz()) return 0;
+        const int kr = k + 1, N = g.Nz;
+        return (kr >= 3 && kr <= N - 2) ? 0 : ((kr >= 2 && kr <= N - 1) ? 1 : 2);
+    }
+    __device__ __forceinline__ bool zs4_face(int k) const { const int kr = k + 1; return !bz() || (kr >= 4 && kr <= g.Nz - 2); }
+    __device__ __forceinline__ bool zs4_center(int k) const { const int kr = k + 1; return !bz() || (kr >= 3 && kr <= g.Nz - 2); }
+
+    // ---- biased reconstructions of field f at faces; x and y are never switched
+    __device__ __forceinline__ T bx(const T *f, int i, int j, int k, bool left) const {
+        return weno5<T>(ld(f, i - 3, j, k), ld(f, i - 2, j, k), ld(f, i - 1, j, k), ld(f, i, j, k), ld(f, i + 1, j, k), ld(f, i + 2, j, k), left);
+    }
+    __device__ __forceinline__ T by(const T *f, int i, int j, int k, bool left) const {
+        return weno5<T>(ld(f, i, j - 3, k), ld(f, i, j - 2, k), ld(f, i, j - 1, k), ld(f, i, j, k), ld(f, i, j + 1, k), ld(f, i, j + 2, k), left);
+    }
+    // z face k with scheme s (stencil centred on face k)
+    __device__ __forceinline__ T bz_s(const T *f, int i, int j, int k, bool left, int sch) const {
+        if (sch == 0) return weno5<T>(ld(f, i, j, k - 3), ld(f, i, j, k - 2), ld(f, i, j, k - 1), ld(f, i, j, k), ld(f, i, j, k + 1), ld(f, i, j, k + 2), left);
+        if (sch == 1) return weno3<T>(ld(f, i, j, k - 2), ld(f, i, j, k - 1), ld(f, i, j, k), ld(f, i, j, k + 1), left);
+        return cen2<T>(ld(f, i, j, k - 1), ld(f, i, j, k));
+    }
+
+    // ================= momentum advective fluxes (upwind_biased_advective_fluxes.jl:23-93)
+    __device__ T F_Uu(int i, int j, int k) const {  // ccc
+        const T *u = s.U[0];
+        if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(u, i, j, k) + ld(u, i + 1, j, k)); return Axc(k) * a * a; }
+        const T A = Axc(k);
+        const T ut = cen4<T>(A * ld(u, i - 1, j, k), A * ld(u, i, j, k), A * ld(u, i + 1, j, k), A * ld(u, i + 2, j, k));
+        return ut * bx(u, i + 1, j, k, ut > T(0));
+    }
+    __device__ T F_Vu(int i, int j, int k) const {  // ffc
+        if (fly()) return T(0);
+        const T *u = s.U[0], *v = s.U[1];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T vi = flx() ? ld(v, i, j, k) : T(0.5) * (ld(v, i - 1, j, k) + ld(v, i, j, k));
+            return Ayc(k) * vi * (T(0.5) * (ld(u, i, j - 1, k) + ld(u, i, j, k)));
+        }
+        const T A = Ayc(k);
+        const T vt = flx() ? A * ld(v, i, j, k) : cen4<T>(A * ld(v, i - 2, j, k), A * ld(v, i - 1, j, k), A * ld(v, i, j, k), A * ld(v, i + 1, j, k));
+        return vt * by(u, i, j, k, vt > T(0));
+    }
+    __device__ T F_Wu(int i, int j, int k) const {  // fcf
+        if (flz()) return T(0);
+        const T *u = s.U[0], *w = s.U[2];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T wi = flx() ? ld(w, i, j, k) : T(0.5) * (ld(w, i - 1, j, k) + ld(w, i, j, k));
+            return g.Az * wi * (T(0.5) * (ld(u, i, j, k - 1) + ld(u, i, j, k)));
+        }
+        const T A = g.Az;
+        const T wt = flx() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i - 2, j, k), A * ld(w, i - 1, j, k), A * ld(w, i, j, k), A * ld(w, i + 1, j, k));
+        return wt * bz_s(u, i, j, k, wt > T(0), zb_face(k));
+    }
+    __device__ T F_Uv(int i, int j, int k) const {  // ffc
+        if (flx()) return T(0);
+        const T *u = s.U[0], *v = s.U[1];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T uj = fly() ? ld(u, i, j, k) : T(0.5) * (ld(u, i, j - 1, k) + ld(u, i, j, k));
+            return Axc(k) * uj * (T(0.5) * (ld(v, i - 1, j, k) + ld(v, i, j, k)));
+        }
+        const T A = Axc(k);
+        const T ut = fly() ? A * ld(u, i, j, k) : cen4<T>(A * ld(u, i, j - 2, k), A * ld(u, i, j - 1, k), A * ld(u, i, j, k), A * ld(u, i, j + 1, k));
+        return ut * bx(v, i, j, k, ut > T(0));
+    }
+    __device__ T F_Vv(int i, int j, int k) const {  // ccc
+        const T *v = s.U[1];
+        if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(v, i, j, k) + ld(v, i, j + 1, k)); return Ayc(k) * a * a; }
+        const T A = Ayc(k);
+        const T vt = cen4<T>(A * ld(v, i, j - 1, k), A * ld(v, i, j, k), A * ld(v, i, j + 1, k), A * ld(v, i, j + 2, k));
+        return vt * by(v, i, j + 1, k, vt > T(0));
+    }
+    __device__ T F_Wv(int i, int j, int k) const {  // cff
+        if (flz()) return T(0);
+        const T *v = s.U[1], *w = s.U[2];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T wj = fly() ? ld(w, i, j, k) : T(0.5) * (ld(w, i, j - 1, k) + ld(w, i, j, k));
+            return g.Az * wj * (T(0.5) * (ld(v, i, j, k - 1) + ld(v, i, j, k)));
+        }
+        const T A = g.Az;
+        const T wt = fly() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i, j - 2, k), A * ld(w, i, j - 1, k), A * ld(w, i, j, k), A * ld(w, i, j + 1, k));
+        return wt * bz_s(v, i, j, k, wt > T(0), zb_face(k));
+    }
+    __device__ T F_Uw(int i, int j, int k) const {  // fcf
+        if (flx()) return T(0);
+        const T *u = s.U[0], *w = s.U[2];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T uk = flz() ? ld(u, i, j, k) : T(0.5) * (ld(u, i, j, k - 1) + ld(u, i, j, k));
+            return Axf(k) * uk * (T(0.5) * (ld(w, i - 1, j, k) + ld(w, i, j, k)));
+        }
+        T ut;
+        if (flz()) ut = Axc(k) * ld(u, i, j, k);
+        else if (zs4_face(k))
+            ut = cen4<T>(Axc(k - 2) * ld(u, i, j, k - 2), Axc(k - 1) * ld(u, i, j, k - 1), Axc(k) * ld(u, i, j, k), Axc(k + 1) * ld(u, i, j, k + 1));
+        else ut = cen2<T>(Axc(k - 1) * ld(u, i, j, k - 1), Axc(k) * ld(u, i, j, k));
+        return ut * bx(w, i, j, k, ut > T(0));
+    }
+    __device__ T F_Vw(int i, int j, int k) const {  // cff
+        if (fly()) return T(0);
+        const T *v = s.U[1], *w = s.U[2];
+        if (ph.advection == OM_ADV_CENTERED2) {
+            T vk = flz() ? ld(v, i, j, k) : T(0.5) * (ld(v, i, j, k - 1) + ld(v, i, j, k));
+            return Ayf(k) * vk * (T(0.5) * (ld(w, i, j - 1, k) + ld(w, i, j, k)));
+        }
+        T vt;
+        if (flz()) vt = Ayc(k) * ld(v, i, j, k);
+        else if (zs4_face(k))
+            vt = cen4<T>(Ayc(k - 2) * ld(v, i, j, k - 2), Ayc(k - 1) * ld(v, i, j, k - 1), Ayc(k) * ld(v, i, j, k), Ayc(k + 1) * ld(v, i, j, k + 1));
+        else vt = cen2<T>(Ayc(k - 1) * ld(v, i, j, k - 1), Ayc(k) * ld(v, i, j, k));
+        return vt * by(w, i, j, k, vt > T(0));
+    }
+    __device__ T F_Ww(int i, int j, int k) const {  // ccc (centre k uses the face k+1 stencils)
+        if (flz()) return T(0);
+        const T *w = s.U[2];
+        if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(w, i, j, k) + ld(w, i, j, k + 1)); return g.Az * a * a; }
+        const T A = g.Az;
+        const T wt = zs4_center(k) ? cen4<T>(A * ld(w, i, j, k - 1), A * ld(w, i, j, k), A * ld(w, i, j, k + 1), A * ld(w, i, j, k + 2))
+                                   : cen2<T>(A * ld(w, i, j, k), A * ld(w, i, j, k + 1));
+        return wt * bz_s(w, i, j, k + 1, wt > T(0), zb_center(k));
+    }
+    // ================= tracer advective fluxes (upwind_biased_advective_fluxes.jl:99-121)
+    __device__ T F_cx(const T *c, int i, int j, int k) const {
+        const T ui = ld(s.U[0], i, j, k);
+        if (ph.advection == OM_ADV_CENTERED2) return (Axc(k) * ui) * cen2<T>(ld(c, i - 1, j, k), ld(c, i, j, k));
+        return Axc(k) * ui * bx(c, i, j, k, ui > T(0));
+    }
+    __device__ T F_cy(const T *c, int i, int j, int k) const {
+        const T vj = ld(s.U[1], i, j, k);
+        if (ph.advection == OM_ADV_CENTERED2) return (Ayc(k) * vj) * cen2<T>(ld(c, i, j - 1, k), ld(c, i, j, k));
+        return Ayc(k) * vj * by(c, i, j, k, vj > T(0));
+    }
+    __device__ T F_cz(const T *c, int i, int j, int k) const {
+        const T wk = ld(s.U[2], i, j, k);
+        if (ph.advection == OM_ADV_CENTERED2) return (g.Az * wk) * cen2<T>(ld(c, i, j, k - 1), ld(c, i, j, k));
+        return g.Az * wk * bz_s(c, i, j, k, wk > T(0), zb_face(k));
+    }
+
+    // ================= gradients (velocity_tracer_gradients.jl:6-18)
+    __device__ __forceinline__ T dx_u(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[0], i + 1, j, k) - ld(s.U[0], i, j, k)) * g.idx; }
+    __device__ __forceinline__ T dy_v(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[1], i, j + 1, k) - ld(s.U[1], i, j, k)) * g.idy; }
+    __device__ __forceinline__ T dz_w(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[2], i, j, k + 1) - ld(s.U[2], i, j, k)) * g.idzc[k]; }
+    __device__ __forceinline__ T dx_v(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dx_w(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dy_u(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dy_w(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dz_u(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j, k - 1)) * g.idzf[k]; }
+    __device__ __forceinline__ T dz_v(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i, j, k - 1)) * g.idzf[k]; }
+    __device__ __forceinline__ T dxc(const T *c, int i, int j, int k) const { return flx() ? T(0) : (ld(c, i, j, k) - ld(c, i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dyc(const T *c, int i, int j, int k) const { return fly() ? T(0) : (ld(c, i, j, k) - ld(c, i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dzc(const T *c, int i, int j, int k) const { return flz() ? T(0) : (ld(c, i, j, k) - ld(c, i, j, k - 1)) * g.idzf[k]; }
+    __device__ __forceinline__ T S12(int i, int j, int k) const { return T(0.5) * (dy_u(i, j, k) + dx_v(i, j, k)); }
+    __device__ __forceinline__ T S13(int i, int j, int k) const { return T(0.5) * (dz_u(i, j, k) + dx_w(i, j, k)); }
+    __device__ __forceinline__ T S23(int i, int j, int k) const { return T(0.5) * (dz_v(i, j, k) + dy_w(i, j, k)); }
+
+    // ================= eddy viscosity at the stress locations (abstract_scalar_diffusivity_closure.jl:343-353)
+    __device__ __forceinline__ T nu_ccc(int i, int j, int k) const {
+        if (ph.closure == OM_CLOSURE_SCALAR_DIFFUSIVITY) return ph.c0;
+        return ld(s.nu, i, j, k);
+    }
+    __device__ __forceinline__ T nu_ffc(int i, int j, int k) const {
+        if (ph.closure == OM_CLOSURE_SCALAR_DIFFUSIVITY) return ph.c0;
+        const T *n = s.nu;
+        T a = flx() ? ld(n, i, j - 1, k) : T(0.5) * (ld(n, i - 1, j - 1, k) + ld(n, i, j - 1, k));
+        T b = flx() ? ld(n, i, j, k) : T(0.5) * (ld(n, i - 1, j, k) + ld(n, i, j, k));
+        if (fly()) return b;
+        return T(0.5) * (a + b);
+    }
+    __device__ __forceinline__ T nu_fcf(int i, int j, int k) const {
+        if (ph.closure == OM_CLOSURE_SCALAR_DIFFUSIVITY) return ph.c0;
+        const T *n = s.nu;
+        T a = flx() ? ld(n, i, j, k - 1) : T(0.5) * (ld(n, i - 1, j, k - 1) + ld(n, i, j, k - 1));
+        T b = flx() ? ld(n, i, j, k) : T(0.5) * (ld(n, i - 1, j, k) + ld(n, i, j, k));
+        if (flz()) return b;
+        return T(0.5) * (a + b);
+    }
+    __device__ __forceinline__ T nu_cff(int i, int j, int k) const {
+        if (ph.closure == OM_CLOSURE_SCALAR_DIFFUSIVITY) return ph.c0;
+        const T *n = s.nu;
+        T a = fly() ? ld(n, i, j, k - 1) : T(0.5) * (ld(n, i, j - 1, k - 1) + ld(n, i, j, k - 1));
+        T b = fly() ? ld(n, i, j, k) : T(0.5) * (ld(n, i, j - 1, k) + ld(n, i, j, k));
+        if (flz()) return b;
+        return T(0.5) * (a + b);
+    }
+    // tracer diffusivity at fcc / cfc / ccf
+    __device__ __forceinline__ T kap_f(int n, int i, int j, int k, int di, int dj, int dk) const {
+        if (ph.closure == OM_CLOSURE_SCALAR_DIFFUSIVITY) return ph.c1;
+        if (ph.closure == OM_CLOSURE_AMD) return T(0.5) * (ld(s.kap[n], i - di, j - dj, k - dk) + ld(s.kap[n], i, j, k));
+        return (T(0.5) * (ld(s.nu, i - di, j - dj, k - dk) + ld(s.nu, i, j, k))) / ph.c2;
+    }
+
+    __device__ T buoy(int i, int j, int k) const {
+        if (ph.buoyancy == OM_BUOYANCY_TRACER) return ld(s.U[3 + ph.b_tracer], i, j, k);
+        if (ph.buoyancy == OM_BUOYANCY_LINEAR_EOS)
+            return ph.g * (ph.alpha * ld(s.U[3 + ph.T_tracer], i, j, k) - ph.beta * ld(s.U[3 + ph.S_tracer], i, j, k));
+        return T(0);
+    }
+    __device__ T dzb(int i, int j, int k) const {  // ∂z_b at ccf
+        if (ph.buoyancy == OM_BUOYANCY_TRACER) return dzc(s.U[3 + ph.b_tracer], i, j, k);
+        if (ph.buoyancy == OM_BUOYANCY_LINEAR_EOS)
+            return ph.g * (ph.alpha * dzc(s.U[3 + ph.T_tracer], i, j, k) - ph.beta * dzc(s.U[3 + ph.S_tracer], i, j, k));
+        return T(0);
+    }
+};
+
+template <class T> struct HaloJob {
+    T *f[kMaxFields + 4];
+    int kind_bottom[kMaxFields + 4], kind_top[kMaxFields + 4];   // OM_BC_* ; -1 = w normal flow; -2 = skip
+    T val_bottom[kMaxFields + 4], val_top[kMaxFields + 4];
+    int nf;
+};
+
+template <class T> struct RKArgs {
+    T dt, gamma, zeta;
+    int first;
+    int nfields;
+};
+
+// Poisson solver plan: spectral storage and per-axis tables (device pointers).
+template <class T> struct PoissonPlan {
+    typename Vec2<T>::type *S;          // complex spectral array, Mx · Ny · Nz
+    const typename Vec2<T>::type *tw_xh; // twiddles for the x line (length Nx/2 when Nx is even, Nx otherwise)
+    const typename Vec2<T>::type *post;  // exp(-2πi k / Nx), k = 0 .. Nx/2
+    const typename Vec2<T>::type *tw_y;  // length Ny
+    const typename Vec2<T>::type *tw_z;  // length Lz_line
+    const T *lx, *ly, *lz;               // eigenvalues (lz has Lz_line entries)
+    int Lz_line;                         // 2·Nz (Bounded), Nz (Periodic)
+    int tridiagonal;                     // Fourier–tridiagonal z solve (stretched z)
+    T eps10;                             // 10·eps(FT): the Thomas pivot guard
+};
+
+template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st);
+template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st);
+template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
+template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, hipStream_t st);
+template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st);
+template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st);
+// Solve ∇²ϕ = source; writes ϕ / divisor into `out` (the nonhydrostatic pressure p/Δt when
+// called from the pressure correction, divisor = 1 otherwise).
+template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
+                                       T divisor, hipStream_t st);
+
+}  // namespace om

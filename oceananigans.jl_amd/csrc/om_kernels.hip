@@ -1,0 +1,453 @@
+// om_kernels.hip — HIP kernels (gfx950) for one NonhydrostaticModel RK3 substage:
+// halo fills, eddy viscosity/diffusivity, hydrostatic pressure, the fused tendency kernel,
+// the RK3 update, and the pressure correction.
+//
+// Indices are 0-based device indices (i = 0 is the reference's interior index 1).  Boundary
+// switching in z compares the reference index kr = k + 1 against the reference's bounds.
+#include <hip/hip_runtime.h>
+
+#include "om_internal.h"
+#include "om_kernels.h"
+
+namespace om {
+
+// ----------------------------------------------------------------------------------------
+// Tendency kernel: one thread per cell computes Gu, Gv, Gw and every tracer tendency.
+// nonhydrostatic_tendency_kernel_functions.jl:71-302
+// ----------------------------------------------------------------------------------------
+template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const bool adv = C.ph.advection != OM_ADV_NONE;
+    const bool clo = C.ph.closure != OM_CLOSURE_NONE;
+    const int64_t o = C.at(i, j, k);
+
+    // ---------------- u (fcc)
+    {
+        T G = T(0);
+        if (adv) {
+            T sx = C.flx() ? T(0) : C.F_Uu(i, j, k) - C.F_Uu(i - 1, j, k);
+            T sy = C.F_Vu(i, j + 1, k) - C.F_Vu(i, j, k);
+            T sz = C.F_Wu(i, j, k + 1) - C.F_Wu(i, j, k);
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        if (C.ph.coriolis) {
+            const T *v = C.s.U[1];
+            const T A = C.Ayc(k);
+            T a = C.flx() ? A * C.ld(v, i, j, k) : T(0.5) * (A * C.ld(v, i - 1, j, k) + A * C.ld(v, i, j, k));
+            T b = C.flx() ? A * C.ld(v, i, j + 1, k) : T(0.5) * (A * C.ld(v, i - 1, j + 1, k) + A * C.ld(v, i, j + 1, k));
+            T I = C.fly() ? a : T(0.5) * (a + b);
+            G += (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
+        }
+        if (C.ph.hydrostatic) G -= (C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k)) * C.g.idx;
+        if (clo) {
+            const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
+            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_ccc(i, j, k) * C.dx_u(i, j, k))) - Ax * (T(-2) * (C.nu_ccc(i - 1, j, k) * C.dx_u(i - 1, j, k)));
+            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_ffc(i, j + 1, k) * C.S12(i, j + 1, k))) - Ay * (T(-2) * (C.nu_ffc(i, j, k) * C.S12(i, j, k)));
+            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_fcf(i, j, k + 1) * C.S13(i, j, k + 1))) - Az * (T(-2) * (C.nu_fcf(i, j, k) * C.S13(i, j, k)));
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        C.s.Gn[0][o] = G;
+    }
+    // ---------------- v (cfc)
+    {
+        T G = T(0);
+        if (adv) {
+            T sx = C.F_Uv(i + 1, j, k) - C.F_Uv(i, j, k);
+            T sy = C.fly() ? T(0) : C.F_Vv(i, j, k) - C.F_Vv(i, j - 1, k);
+            T sz = C.F_Wv(i, j, k + 1) - C.F_Wv(i, j, k);
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        if (C.ph.coriolis) {
+            const T *u = C.s.U[0];
+            const T A = C.Axc(k);
+            T a = C.flx() ? A * C.ld(u, i, j - 1, k) : T(0.5) * (A * C.ld(u, i, j - 1, k) + A * C.ld(u, i + 1, j - 1, k));
+            T b = C.flx() ? A * C.ld(u, i, j, k) : T(0.5) * (A * C.ld(u, i, j, k) + A * C.ld(u, i + 1, j, k));
+            T I = C.fly() ? b : T(0.5) * (a + b);
+            G -= (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
+        }
+        if (C.ph.hydrostatic) G -= (C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k)) * C.g.idy;
+        if (clo) {
+            const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
+            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_ffc(i + 1, j, k) * C.S12(i + 1, j, k))) - Ax * (T(-2) * (C.nu_ffc(i, j, k) * C.S12(i, j, k)));
+            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_ccc(i, j, k) * C.dy_v(i, j, k))) - Ay * (T(-2) * (C.nu_ccc(i, j - 1, k) * C.dy_v(i, j - 1, k)));
+            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_cff(i, j, k + 1) * C.S23(i, j, k + 1))) - Az * (T(-2) * (C.nu_cff(i, j, k) * C.S23(i, j, k)));
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        C.s.Gn[1][o] = G;
+    }
+    // ---------------- w (ccf)
+    {
+        T G = T(0);
+        if (adv) {
+            T sx = C.F_Uw(i + 1, j, k) - C.F_Uw(i, j, k);
+            T sy = C.F_Vw(i, j + 1, k) - C.F_Vw(i, j, k);
+            T sz = C.flz() ? T(0) : C.F_Ww(i, j, k) - C.F_Ww(i, j, k - 1);
+            G -= C.iVf(k) * ((sx + sy) + sz);
+        }
+        if (!C.ph.hydrostatic && C.ph.buoyancy != OM_BUOYANCY_NONE && !C.flz())
+            G += T(0.5) * (C.buoy(i, j, k - 1) + C.buoy(i, j, k));
+        if (clo) {
+            const T Ax = C.Axf(k), Ay = C.Ayf(k), Az = C.g.Az;
+            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_fcf(i + 1, j, k) * C.S13(i + 1, j, k))) - Ax * (T(-2) * (C.nu_fcf(i, j, k) * C.S13(i, j, k)));
+            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_cff(i, j + 1, k) * C.S23(i, j + 1, k))) - Ay * (T(-2) * (C.nu_cff(i, j, k) * C.S23(i, j, k)));
+            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_ccc(i, j, k) * C.dz_w(i, j, k))) - Az * (T(-2) * (C.nu_ccc(i, j, k - 1) * C.dz_w(i, j, k - 1)));
+            G -= C.iVf(k) * ((sx + sy) + sz);
+        }
+        C.s.Gn[2][o] = G;
+    }
+    // ---------------- tracers (ccc)
+    for (int n = 0; n < C.ph.n_tracers; ++n) {
+        const T *c = C.s.U[3 + n];
+        T G = T(0);
+        if (adv) {
+            T sx = C.flx() ? T(0) : C.F_cx(c, i + 1, j, k) - C.F_cx(c, i, j, k);
+            T sy = C.fly() ? T(0) : C.F_cy(c, i, j + 1, k) - C.F_cy(c, i, j, k);
+            T sz = C.flz() ? T(0) : C.F_cz(c, i, j, k + 1) - C.F_cz(c, i, j, k);
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        if (clo) {
+            const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
+            T sx = C.flx() ? T(0) : Ax * (-C.kap_f(n, i + 1, j, k, 1, 0, 0) * C.dxc(c, i + 1, j, k)) - Ax * (-C.kap_f(n, i, j, k, 1, 0, 0) * C.dxc(c, i, j, k));
+            T sy = C.fly() ? T(0) : Ay * (-C.kap_f(n, i, j + 1, k, 0, 1, 0) * C.dyc(c, i, j + 1, k)) - Ay * (-C.kap_f(n, i, j, k, 0, 1, 0) * C.dyc(c, i, j, k));
+            T sz = C.flz() ? T(0) : Az * (-C.kap_f(n, i, j, k + 1, 0, 0, 1) * C.dzc(c, i, j, k + 1)) - Az * (-C.kap_f(n, i, j, k, 0, 0, 1) * C.dzc(c, i, j, k));
+            G -= C.iVc(k) * ((sx + sy) + sz);
+        }
+        C.s.Gn[3 + n][o] = G;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Eddy viscosity / diffusivity (AMD: anisotropic_minimum_dissipation.jl:161-358;
+// Smagorinsky-Lilly: smagorinsky.jl:90-104, lilly_coefficient.jl:129-142)
+// ----------------------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T avg4(T a, T b, T c, T d) { return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d)); }
+
+template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const int64_t o = C.at(i, j, k);
+    const T dxu = C.dx_u(i, j, k), dyv = C.dy_v(i, j, k), dzw = C.dz_w(i, j, k);
+
+    if (C.ph.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
+        T s12[4] = {C.S12(i, j, k), C.S12(i + 1, j, k), C.S12(i, j + 1, k), C.S12(i + 1, j + 1, k)};
+        T s13[4] = {C.S13(i, j, k), C.S13(i + 1, j, k), C.S13(i, j, k + 1), C.S13(i + 1, j, k + 1)};
+        T s23[4] = {C.S23(i, j, k), C.S23(i, j + 1, k), C.S23(i, j, k + 1), C.S23(i, j + 1, k + 1)};
+        T tr = dxu * dxu + dyv * dyv + dzw * dzw;
+        T S2 = tr + T(2) * avg4(s12[0] * s12[0], s12[1] * s12[1], s12[2] * s12[2], s12[3] * s12[3])
+                  + T(2) * avg4(s13[0] * s13[0], s13[1] * s13[1], s13[2] * s13[2], s13[3] * s13[3])
+                  + T(2) * avg4(s23[0] * s23[0], s23[1] * s23[1], s23[2] * s23[2], s23[3] * s23[3]);
+        T D3 = (C.g.dx * C.g.dy) * C.g.dzc[k];
+        T Df = cbrt(D3);
+        T N2 = T(0.5) * (C.dzb(i, j, k) + C.dzb(i, j, k + 1));
+        T c0 = C.ph.c0, cb = C.ph.c1;
+        T N2p = fmax(T(0), N2);
+        T s2 = T(1) - fmin(T(1), cb * N2p / S2);
+        T sig = (S2 == T(0)) ? T(0) : sqrt(s2);
+        C.s.nu[o] = (sig * (c0 * c0)) * (Df * Df) * sqrt(T(2) * S2);
+        return;
+    }
+
+    // ---- AMD: normalised gradients at the edge points around the cell
+    const T Dfx = T(2) * C.g.dx, Dfy = T(2) * C.g.dy;
+    auto Dfz = [&](int kk) { return T(2) * C.g.dzc[kk]; };
+    // ffc points (i,j),(i+1,j),(i,j+1),(i+1,j+1)
+    T ndxv[4], ndyu[4];
+    {
+        const int ii[4] = {i, i + 1, i, i + 1}, jj[4] = {j, j, j + 1, j + 1};
+        for (int q = 0; q < 4; ++q) {
+            ndxv[q] = Dfx / Dfy * C.dx_v(ii[q], jj[q], k);
+            ndyu[q] = Dfy / Dfx * C.dy_u(ii[q], jj[q], k);
+        }
+    }
+    // fcf points (i,k),(i+1,k),(i,k+1),(i+1,k+1); also ∂y_w at those points (see cy_uy below)
+    T ndxw[4], ndzu[4], ndyw_x[4];
+    {
+        const int ii[4] = {i, i + 1, i, i + 1}, kk[4] = {k, k, k + 1, k + 1};
+        for (int q = 0; q < 4; ++q) {
+            ndxw[q] = Dfx / Dfz(kk[q]) * C.dx_w(ii[q], j, kk[q]);
+            ndzu[q] = Dfz(kk[q]) / Dfx * C.dz_u(ii[q], j, kk[q]);
+            ndyw_x[q] = Dfy / Dfz(kk[q]) * C.dy_w(ii[q], j, kk[q]);
+        }
+    }
+    // cff points (j,k),(j+1,k),(j,k+1),(j+1,k+1)
+    T ndyw[4], ndzv[4];
+    {
+        const int jj[4] = {j, j + 1, j, j + 1}, kk[4] = {k, k, k + 1, k + 1};
+        for (int q = 0; q < 4; ++q) {
+            ndyw[q] = Dfy / Dfz(kk[q]) * C.dy_w(i, jj[q], kk[q]);
+            ndzv[q] = Dfz(kk[q]) / Dfy * C.dz_v(i, jj[q], kk[q]);
+        }
+    }
+    T nS12[4], nS13[4], nS23[4];
+    for (int q = 0; q < 4; ++q) {
+        nS12[q] = T(0.5) * (ndyu[q] + ndxv[q]);
+        nS13[q] = T(0.5) * (ndzu[q] + ndxw[q]);
+        nS23[q] = T(0.5) * (ndzv[q] + ndyw[q]);
+    }
+#define A4(expr) avg4<T>([&](int q) { return expr; }(0), [&](int q) { return expr; }(1), [&](int q) { return expr; }(2), [&](int q) { return expr; }(3))
+    const T q2 = dxu * dxu + dyv * dyv + dzw * dzw
+               + A4(ndxv[q] * ndxv[q]) + A4(ndyu[q] * ndyu[q])
+               + A4(ndxw[q] * ndxw[q]) + A4(ndzu[q] * ndzu[q])
+               + A4(ndyw[q] * ndyw[q]) + A4(ndzv[q] * ndzv[q]);
+    const T delta2 = T(3) / ((T(1) / (Dfx * Dfx) + T(1) / (Dfy * Dfy)) + T(1) / (Dfz(k) * Dfz(k)));
+    T nu = T(0);
+    if (q2 != T(0)) {
+        const T S11 = dxu, S22 = dyv, S33 = dzw;
+        const T t1 = S11 * (dxu * dxu) + S22 * A4(ndxv[q] * ndxv[q]) + S33 * A4(ndxw[q] * ndxw[q])
+                   + T(2) * dxu * A4(ndxv[q] * nS12[q]) + T(2) * dxu * A4(ndxw[q] * nS13[q])
+                   + T(2) * A4(ndxv[q]) * A4(ndxw[q]) * A4(nS23[q]);
+        const T t2 = S11 * A4(ndyu[q] * ndyu[q]) + S22 * (dyv * dyv) + S33 * A4(ndyw[q] * ndyw[q])
+                   + T(2) * dyv * A4(ndyu[q] * nS12[q]) + T(2) * A4(ndyu[q]) * A4(ndyw[q]) * A4(nS13[q])
+                   + T(2) * dyv * A4(ndyw[q] * nS23[q]);
+        const T t3 = S11 * A4(ndzu[q] * ndzu[q]) + S22 * A4(ndzv[q] * ndzv[q]) + S33 * (dzw * dzw)
+                   + T(2) * A4(ndzu[q]) * A4(ndzv[q]) * A4(nS12[q])
+                   + T(2) * dzw * A4(ndzu[q] * nS13[q]) + T(2) * dzw * A4(ndzv[q] * nS23[q]);
+        const T r = (t1 + t2) + t3;
+        nu = -C.ph.c0 * delta2 * (r - T(0)) / q2;
+    }
+    C.s.nu[o] = fmax(T(0), nu);
+
+    for (int n = 0; n < C.ph.n_tracers; ++n) {
+        const T *c = C.s.U[3 + n];
+        const T gx0 = Dfx * C.dxc(c, i, j, k), gx1 = Dfx * C.dxc(c, i + 1, j, k);
+        const T gy0 = Dfy * C.dyc(c, i, j, k), gy1 = Dfy * C.dyc(c, i, j + 1, k);
+        const T gz0 = Dfz(k) * C.dzc(c, i, j, k), gz1 = Dfz(k + 1) * C.dzc(c, i, j, k + 1);
+        const T Ix2 = T(0.5) * (gx0 * gx0 + gx1 * gx1), Iy2 = T(0.5) * (gy0 * gy0 + gy1 * gy1), Iz2 = T(0.5) * (gz0 * gz0 + gz1 * gz1);
+        const T sig = (Ix2 + Iy2) + Iz2;
+        T kap = T(0);
+        if (sig != T(0)) {
+            const T Ix = T(0.5) * (gx0 + gx1), Iy = T(0.5) * (gy0 + gy1), Iz = T(0.5) * (gz0 + gz1);
+            const T cx_ux = dxu * Ix2 + A4(ndxv[q]) * Ix * Iy + A4(ndxw[q]) * Ix * Iz;
+            // the reference interpolates ∂y_w with ℑxzᶜᵃᶜ in this term (anisotropic_minimum_dissipation.jl:343)
+            const T cy_uy = A4(ndyu[q]) * Iy * Ix + dyv * Iy2 + A4(ndyw_x[q]) * Iy * Iz;
+            const T cz_uz = A4(ndzu[q]) * Iz * Ix + A4(ndzv[q]) * Iz * Iy + dzw * Iz2;
+            const T theta = (cx_ux + cy_uy) + cz_uz;
+            kap = -C.ph.c1 * delta2 * theta / sig;
+        }
+        C.s.kap[n][o] = fmax(T(0), kap);
+    }
+#undef A4
+}
+
+// ----------------------------------------------------------------------------------------
+// Hydrostatic pressure anomaly: downward column integral (update_hydrostatic_pressure.jl:11-21)
+// ----------------------------------------------------------------------------------------
+template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C) {
+    // The column sum cancels between neighbouring columns when ∂x pHY′ is formed; it is
+    // accumulated without FMA contraction, in the reference's order.
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const int Nz = C.g.Nz;
+    T bk = C.buoy(i, j, Nz);                      // halo value b[Nz+1] (0-based index Nz)
+    T bkm = C.buoy(i, j, Nz - 1);
+    T p = -(T(0.5) * (bkm + bk)) * C.g.dzf[Nz];
+    C.s.pHY[C.at(i, j, Nz - 1)] = p;
+    for (int k = Nz - 2; k >= 0; --k) {
+        bk = bkm;
+        bkm = C.buoy(i, j, k);
+        p -= (T(0.5) * (bkm + bk)) * C.g.dzf[k + 1];
+        C.s.pHY[C.at(i, j, k)] = p;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Halo fills (fill_halo_regions*.jl).  One launch per stage for up to kMaxFields fields.
+// Stage z: Flux/Value/Gradient one-layer fills of Center fields, impenetrable w faces.
+// Stage x, y: periodic copies over the whole parent extent of the other axes.
+// ----------------------------------------------------------------------------------------
+
+template <class T> __global__ void k_halo_z(DevGrid<T> g, HaloJob<T> J) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n = blockIdx.z;
+    if (i >= g.Nx || j >= g.Ny || n >= J.nf) return;
+    T *c = J.f[n];
+    const int Nz = g.Nz;
+    const int kb = J.kind_bottom[n], kt = J.kind_top[n];
+    if (kb == -2) return;
+    if (kb == -1) {  // impenetrable w: boundary faces set to zero
+        c[idx3(i, j, 0, g.sy, g.sz)] = T(0);
+        c[idx3(i, j, Nz, g.sy, g.sz)] = T(0);
+        return;
+    }
+    {
+        const T c1 = c[idx3(i, j, 0, g.sy, g.sz)];
+        T h = c1;
+        if (kb == OM_BC_VALUE || kb == OM_BC_GRADIENT) {
+            const T D = g.dzf[0];
+            const T grad = kb == OM_BC_GRADIENT ? J.val_bottom[n] : (c1 - J.val_bottom[n]) / (D / T(2));
+            h = c1 + grad * (-D);
+        }
+        c[idx3(i, j, -1, g.sy, g.sz)] = h;
+    }
+    {
+        const T cN = c[idx3(i, j, Nz - 1, g.sy, g.sz)];
+        T h = cN;
+        if (kt == OM_BC_VALUE || kt == OM_BC_GRADIENT) {
+            const T D = g.dzf[Nz];
+            const T grad = kt == OM_BC_GRADIENT ? J.val_top[n] : (J.val_top[n] - cN) / (D / T(2));
+            h = cN + grad * D;
+        }
+        c[idx3(i, j, Nz, g.sy, g.sz)] = h;
+    }
+}
+
+// periodic fill along axis `ax` for all fields in J; thread grid spans the parent extent of the
+// two other axes (including their halos) so corners are copied.
+template <class T> __global__ void k_halo_periodic(DevGrid<T> g, HaloJob<T> J, int ax) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;   // first other axis (parent index)
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;   // second other axis
+    const int n = blockIdx.z;
+    if (n >= J.nf) return;
+    T *c = J.f[n];
+    const int pnx = g.Nx + 2 * g.Hx, pny = g.Ny + 2 * g.Hy, pnz = g.Nz + 2 * g.Hz + 1;
+    if (ax == 0) {
+        if (a >= pny || b >= pnz) return;
+        const int j = a - g.Hy, k = b - g.Hz;
+        for (int h = 1; h <= g.Hx; ++h) {
+            c[idx3(-h, j, k, g.sy, g.sz)] = c[idx3(g.Nx - h, j, k, g.sy, g.sz)];
+            c[idx3(g.Nx - 1 + h, j, k, g.sy, g.sz)] = c[idx3(h - 1, j, k, g.sy, g.sz)];
+        }
+    } else if (ax == 1) {
+        if (a >= pnx || b >= pnz) return;
+        const int i = a - g.Hx, k = b - g.Hz;
+        for (int h = 1; h <= g.Hy; ++h) {
+            c[idx3(i, -h, k, g.sy, g.sz)] = c[idx3(i, g.Ny - h, k, g.sy, g.sz)];
+            c[idx3(i, g.Ny - 1 + h, k, g.sy, g.sz)] = c[idx3(i, h - 1, k, g.sy, g.sz)];
+        }
+    } else {
+        if (a >= pnx || b >= pny) return;
+        const int i = a - g.Hx, j = b - g.Hy;
+        for (int h = 1; h <= g.Hz; ++h) {
+            c[idx3(i, j, -h, g.sy, g.sz)] = c[idx3(i, j, g.Nz - h, g.sy, g.sz)];
+            c[idx3(i, j, g.Nz - 1 + h, g.sy, g.sz)] = c[idx3(i, j, h - 1, g.sy, g.sz)];
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// RK3 update of all prognostic fields, with the flux-BC tendency contributions added to Gⁿ
+// first (compute_flux_bcs.jl:113-162, runge_kutta_3.jl:196-204, nonhydrostatic_rk3_substep.jl:37-46)
+// ----------------------------------------------------------------------------------------
+
+template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs<T> a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const int64_t o = C.at(i, j, k);
+    const bool bz = C.bz();
+    const T Az = C.g.Az;
+    for (int f = 0; f < a.nfields; ++f) {
+        T G = C.s.Gn[f][o];
+        if (bz && f != 2) {
+            bool changed = false;
+            if (k == 0 && C.ph.bc_bottom[f] == OM_BC_FLUX) { G += C.ph.bc_bottom_value[f] * Az / (Az * C.g.dzc[0]); changed = true; }
+            if (k == C.g.Nz - 1 && C.ph.bc_top[f] == OM_BC_FLUX) { G -= C.ph.bc_top_value[f] * Az / (Az * C.g.dzc[C.g.Nz - 1]); changed = true; }
+            if (changed) C.s.Gn[f][o] = G;
+        }
+        if (f == 2 && bz && k == 0 && C.g.Nz > 1) continue;   // exclude_periphery for w
+        T *U = C.s.U[f];
+        if (a.first) U[o] += a.dt * a.gamma * G;
+        else U[o] += a.dt * (a.gamma * G + a.zeta * C.s.Gm[f][o]);
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Pressure correction U -= ∇(p Δt) (pressure_correction.jl:67-73).  The solver has already stored
+// pNHS = (p Δt) / max(ϵ, Δt) (pressure_correction.jl:103), so the kernel only reads the pressure
+// and never writes it: no thread overwrites a neighbour's value while it is being read.
+// ----------------------------------------------------------------------------------------
+template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T dtp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const int64_t o = C.at(i, j, k);
+    const T *p = C.s.pNHS;
+    const T pc = p[o];
+    if (!C.flx()) C.s.U[0][o] -= (dtp * (pc - p[C.at(i - 1, j, k)])) * C.g.idx;
+    if (!C.fly()) C.s.U[1][o] -= (dtp * (pc - p[C.at(i, j - 1, k)])) * C.g.idy;
+    if (!C.flz()) C.s.U[2][o] -= (dtp * (pc - p[C.at(i, j, k - 1)])) * C.g.idzf[k];
+}
+
+// ∇·U at cell centres (divergence_operators.jl:16-19)
+template <class T> __global__ void __launch_bounds__(256) k_divergence(Ctx<T> C, T *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const T *u = C.s.U[0], *v = C.s.U[1], *w = C.s.U[2];
+    T sx = C.flx() ? T(0) : C.Axc(k) * C.ld(u, i + 1, j, k) - C.Axc(k) * C.ld(u, i, j, k);
+    T sy = C.fly() ? T(0) : C.Ayc(k) * C.ld(v, i, j + 1, k) - C.Ayc(k) * C.ld(v, i, j, k);
+    T sz = C.flz() ? T(0) : C.g.Az * C.ld(w, i, j, k + 1) - C.g.Az * C.ld(w, i, j, k);
+    out[(int64_t)i + (int64_t)C.g.Nx * ((int64_t)j + (int64_t)C.g.Ny * k)] = C.iVc(k) * ((sx + sy) + sz);
+}
+
+// ----------------------------------------------------------------------------------------
+// Host-side launchers
+// ----------------------------------------------------------------------------------------
+static inline dim3 cell_grid(int Nx, int Ny, int Nz, dim3 blk) {
+    return dim3((Nx + blk.x - 1) / blk.x, (Ny + blk.y - 1) / blk.y, Nz);
+}
+
+template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_tendencies<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C);
+}
+template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C);
+}
+template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_hydrostatic<T>, cell_grid(C.g.Nx, C.g.Ny, 1, blk), blk, 0, st, C);
+}
+template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    RKArgs<T> a{dt, gamma, zeta, first, nfields};
+    hipLaunchKernelGGL(k_rk3<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, a);
+}
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, dtp);
+}
+template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st) {
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_divergence<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, out);
+}
+template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st) {
+    if (J.nf == 0) return;
+    if (g.tz == OM_BOUNDED) {
+        dim3 blk(64, 4, 1);
+        hipLaunchKernelGGL(k_halo_z<T>, dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4, J.nf), blk, 0, st, g, J);
+    }
+    const int pnx = g.Nx + 2 * g.Hx, pny = g.Ny + 2 * g.Hy, pnz = g.Nz + 2 * g.Hz + 1;
+    dim3 blk(64, 4, 1);
+    if (g.tz == OM_PERIODIC && g.Hz > 0)
+        hipLaunchKernelGGL(k_halo_periodic<T>, dim3((pnx + 63) / 64, (pny + 3) / 4, J.nf), blk, 0, st, g, J, 2);
+    if (g.tx == OM_PERIODIC && g.Hx > 0)
+        hipLaunchKernelGGL(k_halo_periodic<T>, dim3((pny + 63) / 64, (pnz + 3) / 4, J.nf), blk, 0, st, g, J, 0);
+    if (g.ty == OM_PERIODIC && g.Hy > 0)
+        hipLaunchKernelGGL(k_halo_periodic<T>, dim3((pnx + 63) / 64, (pnz + 3) / 4, J.nf), blk, 0, st, g, J, 1);
+}
+
+#define OM_INSTANTIATE(T)                                                                           \
+    template void launch_tendencies<T>(const Ctx<T> &, hipStream_t);                               \
+    template void launch_closure<T>(const Ctx<T> &, hipStream_t);                                  \
+    template void launch_hydrostatic<T>(const Ctx<T> &, hipStream_t);                              \
+    template void launch_rk3<T>(const Ctx<T> &, T, T, T, int, int, hipStream_t);                    \
+    template void launch_correct<T>(const Ctx<T> &, T, hipStream_t);                               \
+    template void launch_divergence<T>(const Ctx<T> &, T *, hipStream_t);                          \
+    template void launch_halos<T>(const DevGrid<T> &, const HaloJob<T> &, hipStream_t);
+OM_INSTANTIATE(float)
+OM_INSTANTIATE(double)
+
+}  // namespace om

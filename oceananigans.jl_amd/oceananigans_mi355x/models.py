@@ -1,0 +1,358 @@
+"""NonhydrostaticModel on the MI355X (src/Models/NonhydrostaticModels/nonhydrostatic_model.jl:124-313).
+
+``build_config`` turns the reference-style description into the C ABI's ``om_config_t``;
+``NonhydrostaticModel`` owns the library handle.  ``set_`` follows ``set!(model; kwargs...)``
+(set_nonhydrostatic_model.jl:39-74) and ``time_step_`` follows ``time_step!`` (runge_kutta_3.jl:103-168).
+"""
+import ctypes
+import inspect
+
+import numpy as np
+
+from . import _abi
+from .grids import Bounded, Flat, Periodic, RectilinearGrid
+from .physics import (AnisotropicMinimumDissipation, BuoyancyTracer, Centered, FieldBoundaryConditions, FPlane,
+                      ScalarDiffusivity, SeawaterBuoyancy, SmagorinskyLilly, WENO)
+
+VELOCITY_NAMES = ("u", "v", "w")
+
+
+def build_config(grid, advection=None, closure=None, buoyancy=None, coriolis=None, tracers=(),
+                 boundary_conditions=None, pressure_solver=None, hydrostatic_pressure_anomaly="default"):
+    """Return (om_config_t, keepalive) for a model description.  keepalive holds arrays the
+    struct points to (z faces)."""
+    if not isinstance(grid, RectilinearGrid):
+        raise TypeError("grid must be a RectilinearGrid")
+    if isinstance(tracers, str):
+        tracers = (tracers,)
+    tracers = tuple(tracers)
+    if len(tracers) > _abi.OM_MAX_TRACERS:
+        raise ValueError("too many tracers")
+    c = _abi.OmConfig()
+    c.precision = 4 if grid.float_type == np.float32 else 8
+    for d in range(3):
+        c.N[d] = grid.N[d]
+        c.H[d] = grid.H[d]
+        c.topology[d] = grid.topology[d].code
+        c.origin[d] = grid.origin[d]
+        c.extent[d] = grid.extent[d]
+    keep = []
+    if grid.z_faces is not None:
+        zf = np.ascontiguousarray(grid.z_faces, dtype=np.float64)
+        keep.append(zf)
+        c.z_faces = _abi.dptr(zf)
+    # advection (reference default: Centered(order=2))
+    if advection is None:
+        advection = Centered()
+    if advection is False:
+        c.advection = _abi.OM_ADV_NONE
+    elif isinstance(advection, WENO):
+        c.advection = _abi.OM_ADV_WENO5
+    elif isinstance(advection, Centered):
+        c.advection = _abi.OM_ADV_CENTERED2
+    else:
+        raise ValueError(f"unsupported advection {advection!r}")
+    # closure
+    if closure is None:
+        c.closure = _abi.OM_CLOSURE_NONE
+    elif isinstance(closure, AnisotropicMinimumDissipation):
+        c.closure = _abi.OM_CLOSURE_AMD
+        c.closure_params[0] = closure.Cnu
+        c.closure_params[1] = closure.Ckappa
+    elif isinstance(closure, SmagorinskyLilly):
+        c.closure = _abi.OM_CLOSURE_SMAGORINSKY_LILLY
+        c.closure_params[0] = closure.C
+        c.closure_params[1] = closure.Cb
+        c.closure_params[2] = closure.Pr
+    elif isinstance(closure, ScalarDiffusivity):
+        c.closure = _abi.OM_CLOSURE_SCALAR_DIFFUSIVITY
+        c.closure_params[0] = closure.nu
+        c.closure_params[1] = closure.kappa
+    else:
+        raise ValueError(f"unsupported closure {closure!r}")
+    # buoyancy
+    c.gravitational_acceleration = 9.80665
+    if buoyancy is None:
+        c.buoyancy = _abi.OM_BUOYANCY_NONE
+    elif isinstance(buoyancy, BuoyancyTracer):
+        if "b" not in tracers:
+            raise ValueError("BuoyancyTracer requires a tracer named b")
+        c.buoyancy = _abi.OM_BUOYANCY_TRACER
+        c.buoyancy_tracer = tracers.index("b")
+    elif isinstance(buoyancy, SeawaterBuoyancy):
+        if "T" not in tracers or "S" not in tracers:
+            raise ValueError("SeawaterBuoyancy requires tracers T and S")
+        c.buoyancy = _abi.OM_BUOYANCY_LINEAR_EOS
+        c.T_tracer = tracers.index("T")
+        c.S_tracer = tracers.index("S")
+        c.gravitational_acceleration = buoyancy.gravitational_acceleration
+        c.thermal_expansion = buoyancy.equation_of_state.thermal_expansion
+        c.haline_contraction = buoyancy.equation_of_state.haline_contraction
+    else:
+        raise ValueError(f"unsupported buoyancy {buoyancy!r}")
+    # Coriolis
+    if coriolis is None:
+        c.coriolis = 0
+    elif isinstance(coriolis, FPlane):
+        c.coriolis = 1
+        c.f = coriolis.f
+    else:
+        raise ValueError(f"unsupported coriolis {coriolis!r}")
+    c.n_tracers = len(tracers)
+    # boundary conditions (z only)
+    bcs = boundary_conditions or {}
+    names = VELOCITY_NAMES + tracers
+    for name, fbc in bcs.items():
+        if name not in names:
+            raise ValueError(f"boundary conditions given for unknown field {name}")
+        if not isinstance(fbc, FieldBoundaryConditions):
+            raise TypeError("boundary conditions must be FieldBoundaryConditions")
+        f = names.index(name)
+        if name == "w" and (fbc.top is not None or fbc.bottom is not None):
+            raise ValueError("w keeps its impenetrable top and bottom")
+        if grid.topology[2] is not Bounded and (fbc.top is not None or fbc.bottom is not None):
+            raise ValueError("top/bottom boundary conditions need a Bounded z")
+        if fbc.bottom is not None:
+            c.bc_bottom[f] = fbc.bottom.kind
+            c.bc_bottom_value[f] = fbc.bottom.condition
+        if fbc.top is not None:
+            c.bc_top[f] = fbc.top.kind
+            c.bc_top_value[f] = fbc.top.condition
+    # pressure solver (NonhydrostaticModels.jl:40-56: FFT for uniform grids, Fourier–tridiagonal for stretched z)
+    if pressure_solver is None:
+        c.pressure_solver = _abi.OM_SOLVER_FOURIER_TRIDIAGONAL if grid.stretched else _abi.OM_SOLVER_FFT
+    elif pressure_solver in ("FFTBasedPoissonSolver", _abi.OM_SOLVER_FFT):
+        c.pressure_solver = _abi.OM_SOLVER_FFT
+    elif pressure_solver in ("FourierTridiagonalPoissonSolver", _abi.OM_SOLVER_FOURIER_TRIDIAGONAL):
+        c.pressure_solver = _abi.OM_SOLVER_FOURIER_TRIDIAGONAL
+    else:
+        raise ValueError(f"unsupported pressure solver {pressure_solver!r}")
+    c.hydrostatic_pressure = {"default": -1, None: 0, False: 0, True: 1}[hydrostatic_pressure_anomaly]
+    return c, keep
+
+
+def interior_shape(grid, field_id):
+    nz = grid.Nz
+    if field_id % 100 == 2 and field_id < 300 and grid.topology[2] is Bounded:
+        nz += 1
+    return (grid.Nx, grid.Ny, nz)
+
+
+def field_nodes(grid, name):
+    """Node coordinates (x, y, z) of a prognostic field (u: fcc, v: cfc, w: ccf, tracers: ccc)."""
+    x = grid.xnodes("f" if name == "u" else "c")
+    y = grid.ynodes("f" if name == "v" else "c")
+    z = grid.znodes("f" if name == "w" else "c")
+    return x, y, z
+
+
+class Clock:
+    def __init__(self, model):
+        self._m = model
+
+    def _get(self):
+        t, it, st, ldt, lsdt = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        _abi.check(self._m._lib.om_get_clock(self._m._h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(st),
+                                             ctypes.byref(ldt), ctypes.byref(lsdt)), self._m._h)
+        return t.value, it.value, st.value, ldt.value, lsdt.value
+
+    time = property(lambda s: s._get()[0])
+    iteration = property(lambda s: s._get()[1])
+    stage = property(lambda s: s._get()[2])
+    last_Δt = property(lambda s: s._get()[3])
+
+
+class Field:
+    """A device field of the model; ``interior()`` copies it to the host (FP64)."""
+
+    def __init__(self, model, field_id, name):
+        self.model, self.id, self.name = model, field_id, name
+
+    def interior(self):
+        shape = interior_shape(self.model.grid, self.id)
+        out = np.empty(int(np.prod(shape)), dtype=np.float64)
+        _abi.check(self.model._lib.om_get_field(self.model._h, self.id, _abi.dptr(out), 0), self.model._h)
+        return out.reshape(shape, order="F")
+
+    def parent(self):
+        info = self.info()
+        shape = tuple(info.parent_size)
+        out = np.empty(int(np.prod(shape)), dtype=np.float64)
+        _abi.check(self.model._lib.om_get_field(self.model._h, self.id, _abi.dptr(out), 1), self.model._h)
+        return out.reshape(shape, order="F")
+
+    def info(self):
+        info = _abi.OmFieldInfo()
+        _abi.check(self.model._lib.om_field_info(self.model._h, self.id, ctypes.byref(info)), self.model._h)
+        return info
+
+    def __repr__(self):
+        return f"Field {self.name} on {self.model.grid!r}"
+
+
+class _NamedFields(dict):
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+
+class NonhydrostaticModel:
+    def __init__(self, grid, *, advection=None, closure=None, buoyancy=None, coriolis=None, tracers=(),
+                 boundary_conditions=None, pressure_solver=None, hydrostatic_pressure_anomaly="default",
+                 forcing=None, stokes_drift=None, background_fields=None, particles=None, biogeochemistry=None,
+                 timestepper="RungeKutta3"):
+        for name, val in (("forcing", forcing), ("stokes_drift", stokes_drift), ("background_fields", background_fields),
+                          ("particles", particles), ("biogeochemistry", biogeochemistry)):
+            if val:
+                raise ValueError(f"{name} is not supported on the MI355X path")
+        if timestepper != "RungeKutta3":
+            raise ValueError("the MI355X path implements timestepper = :RungeKutta3")
+        if isinstance(tracers, str):
+            tracers = (tracers,)
+        self.grid = grid
+        self.tracer_names = tuple(tracers)
+        self.advection, self.closure, self.buoyancy, self.coriolis = advection, closure, buoyancy, coriolis
+        self.config, self._keep = build_config(grid, advection, closure, buoyancy, coriolis, self.tracer_names,
+                                               boundary_conditions, pressure_solver, hydrostatic_pressure_anomaly)
+        self._lib = _abi.load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.om_create(ctypes.byref(self.config), ctypes.byref(h))
+        if rc != 0:
+            raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: {self._lib.om_last_error(None).decode()}")
+        self._h = h
+        names = VELOCITY_NAMES + self.tracer_names
+        self.velocities = _NamedFields((n, Field(self, i, n)) for i, n in enumerate(VELOCITY_NAMES))
+        self.tracers = _NamedFields((n, Field(self, 3 + i, n)) for i, n in enumerate(self.tracer_names))
+        self.pressures = _NamedFields(pNHS=Field(self, _abi.OM_FIELD_PNHS, "pNHS"),
+                                      pHY=Field(self, _abi.OM_FIELD_PHY, "pHY′"))
+        self.closure_fields = _NamedFields(nu=Field(self, _abi.OM_FIELD_NU, "νₑ"))
+        self.Gn = _NamedFields((n, Field(self, _abi.OM_FIELD_GN + i, "G" + n)) for i, n in enumerate(names))
+        self.Gm = _NamedFields((n, Field(self, _abi.OM_FIELD_GM + i, "G⁻" + n)) for i, n in enumerate(names))
+        self.clock = Clock(self)
+        # the constructor ends with update_state!(model) (nonhydrostatic_model.jl:309)
+        self.update_state()
+
+    # ---------------------------------------------------------------- lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.om_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- reference operations
+    def field_id(self, name):
+        names = VELOCITY_NAMES + self.tracer_names
+        if name not in names:
+            raise ValueError(f"name {name} not found in model.velocities or model.tracers.")
+        return names.index(name)
+
+    def set_interior(self, name, values):
+        fid = self.field_id(name) if isinstance(name, str) else name
+        shape = interior_shape(self.grid, fid)
+        a = np.asarray(values, dtype=np.float64)
+        if a.shape != shape:
+            if fid == 2 and a.shape == (shape[0], shape[1], shape[2] - 1):
+                a = np.concatenate([a, np.zeros((shape[0], shape[1], 1))], axis=2)
+            else:
+                a = np.broadcast_to(a, shape)
+        buf = _abi.as_fortran_f64(a)
+        _abi.check(self._lib.om_set_field(self._h, fid, _abi.dptr(buf), 0), self._h)
+
+    def fill_halo_regions(self, names=None, fill_normal_flow_bcs=True):
+        names = names or (VELOCITY_NAMES + self.tracer_names)
+        mask = 0
+        for n in names:
+            mask |= 1 << self.field_id(n)
+        _abi.check(self._lib.om_fill_halos(self._h, mask, 1 if fill_normal_flow_bcs else 0), self._h)
+
+    def update_state(self):
+        _abi.check(self._lib.om_update_state(self._h), self._h)
+
+    def compute_pressure_correction(self, dt):
+        _abi.check(self._lib.om_pressure_correction(self._h, float(dt)), self._h)
+
+    def time_step(self, dt, nsteps=1):
+        if dt == 0:
+            import warnings
+            warnings.warn("Δt == 0 may cause model blowup!")
+        _abi.check(self._lib.om_time_step(self._h, float(dt), int(nsteps)), self._h)
+
+    def set_phase_timing(self, enable=True):
+        _abi.check(self._lib.om_set_phase_timing(self._h, 1 if enable else 0), self._h)
+
+    def last_step_ms(self):
+        t = ctypes.c_double()
+        _abi.check(self._lib.om_last_step_timing(self._h, ctypes.byref(t), None, None, 0), self._h)
+        return t.value
+
+    def phase_times(self):
+        """{phase: (total ms, launches)} of the last time_step call with phase timing enabled."""
+        n = _abi.OM_NPHASES
+        t = ctypes.c_double()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int32 * n)()
+        _abi.check(self._lib.om_last_step_timing(self._h, ctypes.byref(t), ms, cnt, n), self._h)
+        return {name: (ms[p], cnt[p]) for p, name in enumerate(_abi.PHASES)}
+
+    def divergence(self):
+        n = self.grid.Nx * self.grid.Ny * self.grid.Nz
+        out = np.empty(n)
+        _abi.check(self._lib.om_divergence(self._h, _abi.dptr(out)), self._h)
+        return out.reshape((self.grid.Nx, self.grid.Ny, self.grid.Nz), order="F")
+
+    def poisson_solve(self, rhs):
+        rhs = _abi.as_fortran_f64(rhs)
+        out = np.empty_like(rhs)
+        _abi.check(self._lib.om_poisson_solve(self._h, _abi.dptr(rhs), _abi.dptr(out)), self._h)
+        return out.reshape((self.grid.Nx, self.grid.Ny, self.grid.Nz), order="F")
+
+    def __repr__(self):
+        return (f"NonhydrostaticModel{{MI355X, RectilinearGrid}}(time = {self.clock.time}, iteration = {self.clock.iteration})\n"
+                f"├── grid: {self.grid!r}\n├── timestepper: RungeKutta3TimeStepper\n"
+                f"├── advection scheme: {self.advection!r}\n├── tracers: {self.tracer_names}\n"
+                f"├── closure: {self.closure!r}\n├── buoyancy: {type(self.buoyancy).__name__ if self.buoyancy else 'Nothing'}\n"
+                f"└── coriolis: {type(self.coriolis).__name__ if self.coriolis else 'Nothing'}")
+
+
+def _evaluate(grid, name, value):
+    """Field values for set!: arrays are copied; functions are evaluated at the field's nodes
+    (src/Fields/set!.jl:81-125), f(x, y, z) in 3D and f(x, y) when z is Flat."""
+    if not callable(value):
+        return value
+    x, y, z = field_nodes(grid, name)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    if grid.topology[2] is Flat:
+        nargs = len(inspect.signature(value).parameters)
+        out = value(X, Y) if nargs == 2 else value(X, Y, Z)
+    else:
+        out = value(X, Y, Z)
+    return np.broadcast_to(np.asarray(out, dtype=np.float64), X.shape)
+
+
+def set_(model, enforce_incompressibility=True, **kwargs):
+    """set!(model; kwargs...) — set_nonhydrostatic_model.jl:39-74."""
+    for name, value in kwargs.items():
+        model.set_interior(name, _evaluate(model.grid, name, value))
+        model.fill_halo_regions([name], fill_normal_flow_bcs=True)
+    model.update_state()
+    if enforce_incompressibility:
+        model.compute_pressure_correction(1.0)
+        model.update_state()
+
+
+def time_step_(model, dt):
+    """time_step!(model, Δt) — one RK3 step (runge_kutta_3.jl:103-168)."""
+    model.time_step(dt, 1)

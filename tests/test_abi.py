@@ -1,0 +1,127 @@
+"""CPU checks of the C ABI: the HIP library loads, exports every declared entry point, its config
+struct layout matches the header, and configuration errors are reported without touching a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oceananigans_mi355x as om
+from oceananigans_mi355x import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "oceananigans_mi355x.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\s*\*)\s*(om_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_expected_entry_points():
+    names = declared_functions()
+    assert {"om_create", "om_destroy", "om_time_step", "om_rk3_substep", "om_update_state"} <= set(names)
+    assert set(names) == set(_abi.SIGNATURES), "ctypes signatures and header declarations differ"
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _abi.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.om_abi_version() == 1
+
+
+def test_config_struct_layout_matches_c():
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "oceananigans_mi355x.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(om_config_t), offsetof(om_config_t, z_faces),
+         offsetof(om_config_t, bc_top_value), offsetof(om_config_t, pressure_solver),
+         sizeof(om_field_info_t), offsetof(om_field_info_t, stride));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        vals = [int(x) for x in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    C = _abi.OmConfig
+    F = _abi.OmFieldInfo
+    assert vals == [ctypes.sizeof(C), C.z_faces.offset, C.bc_top_value.offset, C.pressure_solver.offset,
+                    ctypes.sizeof(F), F.stride.offset]
+
+
+def _config(**over):
+    grid = om.RectilinearGrid(size=(8, 8, 8), extent=(8, 8, 8))
+    cfg, keep = om.build_config(grid, advection=om.WENO())
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    return cfg, keep
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("precision", 2, "precision"),
+    ("n_tracers", 99, "n_tracers"),
+])
+def test_invalid_configuration_is_rejected_before_any_device_work(field, value, msg):
+    lib = _abi.load_library()
+    cfg, keep = _config(**{field: value})
+    h = ctypes.c_void_p()
+    rc = lib.om_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert msg in lib.om_last_error(None).decode()
+
+
+def test_bounded_x_and_small_halo_are_rejected():
+    lib = _abi.load_library()
+    cfg, keep = _config()
+    cfg.topology[0] = _abi.OM_BOUNDED
+    h = ctypes.c_void_p()
+    assert lib.om_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    cfg, keep = _config()
+    cfg.H[2] = 2
+    assert lib.om_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert "halo" in lib.om_last_error(None).decode()
+
+
+def test_null_handle_calls_fail_cleanly():
+    lib = _abi.load_library()
+    assert lib.om_time_step(None, 1.0, 1) == -1
+    assert lib.om_destroy(None) == -1
+
+
+def test_host_api_validation():
+    grid = om.RectilinearGrid(size=(8, 8, 8), extent=(8, 8, 8))
+    with pytest.raises(ValueError):
+        om.WENO(order=7)
+    with pytest.raises(ValueError):
+        om.build_config(grid, buoyancy=om.BuoyancyTracer(), tracers=("c",))
+    with pytest.raises(ValueError):
+        om.build_config(grid, boundary_conditions={"w": om.FieldBoundaryConditions(top=om.FluxBoundaryCondition(1.0))})
+    with pytest.raises(ValueError):
+        om.FPlane()
+    assert abs(om.FPlane(latitude=45).f - 2 * 7.292115e-5 * np.sin(np.pi / 4)) < 1e-18
+
+
+def test_grid_nodes_and_stretched_faces():
+    g = om.RectilinearGrid(size=(4, 4, 4), x=(0, 4), y=(0, 8), z=[-10, -6, -3, -1, 0])
+    assert g.stretched and g.Lz == 10
+    np.testing.assert_allclose(g.znodes("c"), [-8, -4.5, -2, -0.5])
+    np.testing.assert_allclose(g.xnodes("f"), [0, 1, 2, 3])
+    g2 = om.RectilinearGrid(size=(16, 16), x=(0, 1), y=(0, 1), topology=(om.Periodic, om.Periodic, om.Flat))
+    assert g2.N == (16, 16, 1) and g2.H == (3, 3, 0)
+
+
+def test_synthetic_stream_is_deterministic():
+    from oceananigans_mi355x.synthetic import splitmix64, uniform
+    assert int(splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF   # SplitMix64 published first output for seed 0
+    a = uniform(1, (4, 3, 2), -1, 1)
+    b = uniform(1, (4, 3, 2), -1, 1)
+    assert np.array_equal(a, b) and a.min() >= -1 and a.max() < 1
