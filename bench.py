@@ -103,7 +103,20 @@ def main():
     ap.add_argument("--cpu-sample", default="128,128,128")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_free_convection_256x256x128_fp32.json"),
+                    help="measured HBM bytes per launch of the dominant kernel (from the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is design-only)", file=sys.stderr)
@@ -114,6 +127,11 @@ def main():
     value = r["ncell"] * args.steps / r["wall"]
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
     step_gbs = r["bytes_step"] * args.steps / r["wall"] / 1e9
+    traffic = None
+    if os.path.exists(args.traffic):
+        t = json.load(open(args.traffic))
+        if t.get("kernel") == r["dom"] and t.get("grid") == "x".join(map(str, r["N"])):
+            traffic = t["hbm_bytes_per_launch"]
     out = {
         "metric": "cell-updates/s per RK3 step (256³-class FP32 LES: 256x256x128 WENO5+AMD free convection)",
         "value": value,
@@ -131,7 +149,7 @@ def main():
                    "advection": "WENO5", "closure": "AnisotropicMinimumDissipation", "buoyancy": "BuoyancyTracer",
                    "coriolis": "FPlane(f=1e-4)", "pressure_solver": "FFT", "dt": 1.0, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None, "kernel": r["dom"],
+                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": r["dom"],
                      "kernel_avg_ms": r["dom_avg_ms"], "algorithmic_bytes_per_launch": r["dom_bytes"]},
         "step_roofline": {"algorithmic_bytes_per_cell_step": algorithmic_bytes_per_cell_step(3 + N_TRACERS, r["word"]),
                           "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},

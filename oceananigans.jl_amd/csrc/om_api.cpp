@@ -297,6 +297,10 @@ template <class T> struct ModelT : ModelBase {
     Ctx<T> ctx() const {
         Ctx<T> C;
         C.g = g; C.s = s; C.ph = ph;
+        const int nt = ph.n_tracers;
+        C.s.btr = (ph.b_tracer >= 0 && ph.b_tracer < nt) ? s.U[3 + ph.b_tracer] : nullptr;
+        C.s.Ttr = (ph.T_tracer >= 0 && ph.T_tracer < nt) ? s.U[3 + ph.T_tracer] : nullptr;
+        C.s.Str = (ph.S_tracer >= 0 && ph.S_tracer < nt) ? s.U[3 + ph.S_tracer] : nullptr;
         return C;
     }
 

@@ -55,6 +55,7 @@ template <class T> struct StateView {
     T *kap[kMaxTracers];     // κₑ per tracer (AMD)
     T *pHY;                  // hydrostatic pressure anomaly
     T *pNHS;
+    const T *btr, *Ttr, *Str;  // buoyancy tracer / temperature / salinity (resolved on the host)
 };
 
 __host__ __device__ inline int64_t idx3(int i, int j, int k, int64_t sy, int64_t sz) {

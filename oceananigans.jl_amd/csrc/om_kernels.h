@@ -156,14 +156,14 @@ template <class T> struct Ctx {
     }
 
     // ================= momentum advective fluxes (upwind_biased_advective_fluxes.jl:23-93)
-    __device__ T F_Uu(int i, int j, int k) const {  // ccc
+    __device__ __forceinline__ T F_Uu(int i, int j, int k) const {  // ccc
         const T *u = s.U[0];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(u, i, j, k) + ld(u, i + 1, j, k)); return Axc(k) * a * a; }
         const T A = Axc(k);
         const T ut = cen4<T>(A * ld(u, i - 1, j, k), A * ld(u, i, j, k), A * ld(u, i + 1, j, k), A * ld(u, i + 2, j, k));
         return ut * bx(u, i + 1, j, k, ut > T(0));
     }
-    __device__ T F_Vu(int i, int j, int k) const {  // ffc
+    __device__ __forceinline__ T F_Vu(int i, int j, int k) const {  // ffc
         if (fly()) return T(0);
         const T *u = s.U[0], *v = s.U[1];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -174,7 +174,7 @@ template <class T> struct Ctx {
         const T vt = flx() ? A * ld(v, i, j, k) : cen4<T>(A * ld(v, i - 2, j, k), A * ld(v, i - 1, j, k), A * ld(v, i, j, k), A * ld(v, i + 1, j, k));
         return vt * by(u, i, j, k, vt > T(0));
     }
-    __device__ T F_Wu(int i, int j, int k) const {  // fcf
+    __device__ __forceinline__ T F_Wu(int i, int j, int k) const {  // fcf
         if (flz()) return T(0);
         const T *u = s.U[0], *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -185,7 +185,7 @@ template <class T> struct Ctx {
         const T wt = flx() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i - 2, j, k), A * ld(w, i - 1, j, k), A * ld(w, i, j, k), A * ld(w, i + 1, j, k));
         return wt * bz_s(u, i, j, k, wt > T(0), zb_face(k));
     }
-    __device__ T F_Uv(int i, int j, int k) const {  // ffc
+    __device__ __forceinline__ T F_Uv(int i, int j, int k) const {  // ffc
         if (flx()) return T(0);
         const T *u = s.U[0], *v = s.U[1];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -196,14 +196,14 @@ template <class T> struct Ctx {
         const T ut = fly() ? A * ld(u, i, j, k) : cen4<T>(A * ld(u, i, j - 2, k), A * ld(u, i, j - 1, k), A * ld(u, i, j, k), A * ld(u, i, j + 1, k));
         return ut * bx(v, i, j, k, ut > T(0));
     }
-    __device__ T F_Vv(int i, int j, int k) const {  // ccc
+    __device__ __forceinline__ T F_Vv(int i, int j, int k) const {  // ccc
         const T *v = s.U[1];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(v, i, j, k) + ld(v, i, j + 1, k)); return Ayc(k) * a * a; }
         const T A = Ayc(k);
         const T vt = cen4<T>(A * ld(v, i, j - 1, k), A * ld(v, i, j, k), A * ld(v, i, j + 1, k), A * ld(v, i, j + 2, k));
         return vt * by(v, i, j + 1, k, vt > T(0));
     }
-    __device__ T F_Wv(int i, int j, int k) const {  // cff
+    __device__ __forceinline__ T F_Wv(int i, int j, int k) const {  // cff
         if (flz()) return T(0);
         const T *v = s.U[1], *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -214,7 +214,7 @@ template <class T> struct Ctx {
         const T wt = fly() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i, j - 2, k), A * ld(w, i, j - 1, k), A * ld(w, i, j, k), A * ld(w, i, j + 1, k));
         return wt * bz_s(v, i, j, k, wt > T(0), zb_face(k));
     }
-    __device__ T F_Uw(int i, int j, int k) const {  // fcf
+    __device__ __forceinline__ T F_Uw(int i, int j, int k) const {  // fcf
         if (flx()) return T(0);
         const T *u = s.U[0], *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -228,7 +228,7 @@ template <class T> struct Ctx {
         else ut = cen2<T>(Axc(k - 1) * ld(u, i, j, k - 1), Axc(k) * ld(u, i, j, k));
         return ut * bx(w, i, j, k, ut > T(0));
     }
-    __device__ T F_Vw(int i, int j, int k) const {  // cff
+    __device__ __forceinline__ T F_Vw(int i, int j, int k) const {  // cff
         if (fly()) return T(0);
         const T *v = s.U[1], *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) {
@@ -242,7 +242,7 @@ template <class T> struct Ctx {
         else vt = cen2<T>(Ayc(k - 1) * ld(v, i, j, k - 1), Ayc(k) * ld(v, i, j, k));
         return vt * by(w, i, j, k, vt > T(0));
     }
-    __device__ T F_Ww(int i, int j, int k) const {  // ccc (centre k uses the face k+1 stencils)
+    __device__ __forceinline__ T F_Ww(int i, int j, int k) const {  // ccc (centre k uses the face k+1 stencils)
         if (flz()) return T(0);
         const T *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(w, i, j, k) + ld(w, i, j, k + 1)); return g.Az * a * a; }
@@ -252,17 +252,17 @@ template <class T> struct Ctx {
         return wt * bz_s(w, i, j, k + 1, wt > T(0), zb_center(k));
     }
     // ================= tracer advective fluxes (upwind_biased_advective_fluxes.jl:99-121)
-    __device__ T F_cx(const T *c, int i, int j, int k) const {
+    __device__ __forceinline__ T F_cx(const T *c, int i, int j, int k) const {
         const T ui = ld(s.U[0], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (Axc(k) * ui) * cen2<T>(ld(c, i - 1, j, k), ld(c, i, j, k));
         return Axc(k) * ui * bx(c, i, j, k, ui > T(0));
     }
-    __device__ T F_cy(const T *c, int i, int j, int k) const {
+    __device__ __forceinline__ T F_cy(const T *c, int i, int j, int k) const {
         const T vj = ld(s.U[1], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (Ayc(k) * vj) * cen2<T>(ld(c, i, j - 1, k), ld(c, i, j, k));
         return Ayc(k) * vj * by(c, i, j, k, vj > T(0));
     }
-    __device__ T F_cz(const T *c, int i, int j, int k) const {
+    __device__ __forceinline__ T F_cz(const T *c, int i, int j, int k) const {
         const T wk = ld(s.U[2], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (g.Az * wk) * cen2<T>(ld(c, i, j, k - 1), ld(c, i, j, k));
         return g.Az * wk * bz_s(c, i, j, k, wk > T(0), zb_face(k));
@@ -321,16 +321,16 @@ template <class T> struct Ctx {
         return (T(0.5) * (ld(s.nu, i - di, j - dj, k - dk) + ld(s.nu, i, j, k))) / ph.c2;
     }
 
-    __device__ T buoy(int i, int j, int k) const {
-        if (ph.buoyancy == OM_BUOYANCY_TRACER) return ld(s.U[3 + ph.b_tracer], i, j, k);
+    __device__ __forceinline__ T buoy(int i, int j, int k) const {
+        if (ph.buoyancy == OM_BUOYANCY_TRACER) return ld(s.btr, i, j, k);
         if (ph.buoyancy == OM_BUOYANCY_LINEAR_EOS)
-            return ph.g * (ph.alpha * ld(s.U[3 + ph.T_tracer], i, j, k) - ph.beta * ld(s.U[3 + ph.S_tracer], i, j, k));
+            return ph.g * (ph.alpha * ld(s.Ttr, i, j, k) - ph.beta * ld(s.Str, i, j, k));
         return T(0);
     }
-    __device__ T dzb(int i, int j, int k) const {  // ∂z_b at ccf
-        if (ph.buoyancy == OM_BUOYANCY_TRACER) return dzc(s.U[3 + ph.b_tracer], i, j, k);
+    __device__ __forceinline__ T dzb(int i, int j, int k) const {  // ∂z_b at ccf
+        if (ph.buoyancy == OM_BUOYANCY_TRACER) return dzc(s.btr, i, j, k);
         if (ph.buoyancy == OM_BUOYANCY_LINEAR_EOS)
-            return ph.g * (ph.alpha * dzc(s.U[3 + ph.T_tracer], i, j, k) - ph.beta * dzc(s.U[3 + ph.S_tracer], i, j, k));
+            return ph.g * (ph.alpha * dzc(s.Ttr, i, j, k) - ph.beta * dzc(s.Str, i, j, k));
         return T(0);
     }
 };

@@ -99,7 +99,11 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
         C.s.Gn[2][o] = G;
     }
     // ---------------- tracers (ccc)
-    for (int n = 0; n < C.ph.n_tracers; ++n) {
+    // compile-time tracer indices: a runtime index into the by-value pointer tables would
+    // place the whole context in per-lane scratch memory
+#pragma unroll
+    for (int n = 0; n < kMaxTracers; ++n) {
+        if (n >= C.ph.n_tracers) break;
         const T *c = C.s.U[3 + n];
         T G = T(0);
         if (adv) {
