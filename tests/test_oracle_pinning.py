@@ -387,3 +387,103 @@ def test_tracer_conservation_with_noflux_walls():
     s0 = orc.get(4).sum()
     orc.time_step(1.0, 3)
     assert abs(orc.get(4).sum() - s0) <= 1e-13 * abs(s0)
+
+
+# ---------------------------------------------------------------------------------------------
+# advection orientation and order, Coriolis, equation of state
+# ---------------------------------------------------------------------------------------------
+def _tendency_of(grid, kw, fields, target):
+    """Tendency Gⁿ of prognostic `target` for the given interior fields (halos filled by the oracle)."""
+    cfg, keep = om.build_config(grid, **kw)
+    orc = O.OracleModel(cfg, grid)
+    names = prognostic_names(kw)
+    for name, arr in fields.items():
+        orc.set(names.index(name), arr)
+    orc.fill_halos(fill_normal=False)
+    orc.compute_tendencies()
+    G = orc.get(om._abi.OM_FIELD_GN + names.index(target))
+    orc.close()
+    return G
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_weno_tracer_flux_is_upwind_at_a_step(axis):
+    """A tracer step advected by a uniform positive velocity: the Z-WENO5 face value at the jump is
+    the upwind value (weno_interpolants.jl:348-354; upwind_biased_advective_fluxes.jl:99-121), so the
+    last cell before the jump keeps G = 0 and the first cell after it gains u/Δ.  A downwind bias or a
+    shifted stencil fails this."""
+    N = 16
+    grid = om.RectilinearGrid(size=(N, N, N), extent=(N, N, N), topology=(om.Periodic,) * 3)
+    kw = dict(advection=om.WENO(), tracers=("c",))
+    shape = (N, N, N)
+    idx = np.indices(shape)[axis]
+    c = np.where(idx < N // 2, 1.0, 0.0)
+    c[idx >= N - 2] = 1.0                         # second jump back to 1 at the wrap-around, far from the first
+    vel = {("u", "v", "w")[axis]: np.ones((N, N, N))}
+    G = _tendency_of(grid, kw, dict(vel, c=c), "c")
+    last_one = np.take(G, N // 2 - 1, axis=axis)
+    first_zero = np.take(G, N // 2, axis=axis)
+    assert np.max(np.abs(last_one)) < 1e-6
+    np.testing.assert_allclose(first_zero, 1.0, rtol=1e-6)
+
+
+def test_weno_tracer_advection_fifth_order():
+    """Uniform u advecting cell averages of sin(2πx/L): the flux-divergence error falls ~32× per grid
+    doubling (fifth order of WENO5 reconstruction on smooth data)."""
+    errs = []
+    for N in (32, 64):
+        L = 1.0
+        grid = om.RectilinearGrid(size=(N, 1, 1), extent=(L, 1, 1), topology=(om.Periodic,) * 3, halo=(3, 3, 3))
+        dx = L / N
+        xf = np.arange(N + 1) * dx
+        cav = (-np.cos(2 * np.pi * xf[1:]) + np.cos(2 * np.pi * xf[:-1])) / (2 * np.pi * dx)
+        exact = -(np.sin(2 * np.pi * xf[1:]) - np.sin(2 * np.pi * xf[:-1])) / dx
+        G = _tendency_of(grid, dict(advection=om.WENO(), tracers=("c",)),
+                         {"u": np.ones((N, 1, 1)), "c": cav.reshape(N, 1, 1)}, "c")
+        errs.append(np.max(np.abs(G[:, 0, 0] - exact)))
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 4.5, (errs, rate)
+
+
+def test_weno_momentum_cross_fluxes_are_upwind():
+    """u jumps in y and is advected by uniform v > 0: the momentum flux ṽ·u (ṽ the Centered(4)
+    interpolation of A·v, upwind_biased_advective_fluxes.jl:39-48) takes the upwind u at the jump."""
+    N = 16
+    grid = om.RectilinearGrid(size=(N, N, N), extent=(N, N, N), topology=(om.Periodic,) * 3)
+    j = np.indices((N, N, N))[1]
+    u = np.where(j < N // 2, 1.0, 0.0)
+    u[j >= N - 2] = 1.0
+    G = _tendency_of(grid, dict(advection=om.WENO()), {"u": u, "v": np.ones((N, N, N))}, "u")
+    assert np.max(np.abs(G[:, N // 2 - 1, :])) < 1e-6
+    np.testing.assert_allclose(G[:, N // 2, :], 1.0, rtol=1e-6)
+
+
+def test_fplane_coriolis_sign_and_magnitude():
+    """Uniform (U, V) on an FPlane: Gu = +f V, Gv = −f U (f_plane.jl:8-54, coriolis_schemes.jl:67-68;
+    uniform fields carry no advective flux divergence)."""
+    N = 6
+    grid = om.RectilinearGrid(size=(N, N, N), extent=(N, N, N), topology=(om.Periodic,) * 3)
+    f, U, V = 1e-4, 0.3, -0.7
+    kw = dict(advection=om.WENO(), coriolis=om.FPlane(f=f))
+    fields = {"u": np.full((N, N, N), U), "v": np.full((N, N, N), V)}
+    np.testing.assert_allclose(_tendency_of(grid, kw, fields, "u"), f * V, rtol=1e-14)
+    np.testing.assert_allclose(_tendency_of(grid, kw, fields, "v"), -f * U, rtol=1e-14)
+
+
+def test_linear_eos_resting_state_and_buoyancy_sign():
+    """T = T₀ + Γz, S = S₀ at rest with the hydrostatic anomaly is steady; without the anomaly the w
+    tendency is the buoyancy b = g(αT − βS) at w points (linear_equation_of_state.jl:39-40,72-80)."""
+    grid = om.RectilinearGrid(size=(6, 6, 8), extent=(6, 6, 8))
+    kw = dict(advection=om.WENO(), buoyancy=om.SeawaterBuoyancy(equation_of_state=om.LinearEquationOfState(2e-4, 8e-4)),
+              tracers=("T", "S"))
+    init = {"T": lambda x, y, z: 20 + 0.01 * z, "S": lambda x, y, z: 35.0 + 0 * z}
+    orc = oracle_for(grid, kw, init)
+    orc.time_step(10.0, 3)
+    for q in range(3):
+        assert np.max(np.abs(orc.get(q))) < 1e-15
+    # periodic z: no anomaly, so Gw = ℑz b
+    gridp = om.RectilinearGrid(size=(4, 4, 4), extent=(4, 4, 4), topology=(om.Periodic,) * 3)
+    T, S = 21.0, 34.0
+    Gw = _tendency_of(gridp, kw, {"T": np.full((4, 4, 4), T), "S": np.full((4, 4, 4), S)}, "w")
+    b = 9.80665 * (2e-4 * T - 8e-4 * S)
+    np.testing.assert_allclose(Gw[:, :, :4], b, rtol=1e-13)
