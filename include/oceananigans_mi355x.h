@@ -189,6 +189,13 @@ int om_set_phase_timing(om_model_t *m, int32_t enable);
 int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_phase, int32_t *launches_per_phase,
                         int32_t nphases);
 
+/* launch-shape choices of this library (no reference counterpart).  Results do not depend on
+ * them: every tendency kernel gives identically rounded tendencies. */
+#define OM_TUNE_TENDENCY_KERNEL 0   /* 0: z-marching LDS-tiled kernel where supported (default); 1: thread per cell */
+#define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernel (default 16) */
+int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
+int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liboceananigans_mi355x.so")
-SOURCES = ["om_api.cpp", "om_kernels.hip", "om_fft.hip"]
+SOURCES = ["om_api.cpp", "om_kernels.hip", "om_tend_tiled.hip", "om_fft.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -16,16 +16,32 @@ FLAGS = [
 ]
 
 
+# (source, extra flags, object suffix): each .hip source is compiled once per precision; the FP64
+# object is built without FMA contraction so that its rounding follows the reference's.
+UNITS = [("om_api.cpp", [], "")]
+for _s in SOURCES[1:]:
+    UNITS.append((_s, ["-DOM_ONLY_FLOAT"], "_f32"))
+    UNITS.append((_s, ["-DOM_ONLY_DOUBLE", "-ffp-contract=off"], "_f64"))
+
+
 def build(verbose=False):
-    objs = []
-    for s in SOURCES:
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
+    for s, extra, suffix in UNITS:
         src = os.path.join(CSRC, s)
-        obj = os.path.join(CSRC, os.path.splitext(s)[0] + ".o")
-        cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", src, "-o", obj]
+        obj = os.path.join(CSRC, os.path.splitext(s)[0] + suffix + ".o")
+        jobs.append(([HIPCC, *FLAGS, *extra, "-x", "hip", "-c", src, "-o", obj], obj))
+
+    def run(job):
+        cmd, obj = job
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+        return obj
+
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(workers) as ex:
+        objs = list(ex.map(run, jobs))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

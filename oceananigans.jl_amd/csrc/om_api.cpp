@@ -159,6 +159,9 @@ template <class T> struct ModelT : ModelBase {
     std::vector<int> evphase;
     double phase_ms[OM_NPHASES] = {0};
     int phase_n[OM_NPHASES] = {0};
+    // launch-shape choices (om_set_tuning)
+    int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
+    int tune_kchunk = 16;              // z levels per block of the tiled kernel
 
     struct PhaseScope {
         ModelT *M;
@@ -343,7 +346,12 @@ template <class T> struct ModelT : ModelBase {
     uint32_t all_mask() const { return (1u << nf) - 1u; }
 
     // ---- phases
-    void compute_tendencies() { PhaseScope ps(this, OM_PHASE_TENDENCIES); launch_tendencies(ctx(), st); }
+    void compute_tendencies() {
+        PhaseScope ps(this, OM_PHASE_TENDENCIES);
+        const Ctx<T> C = ctx();
+        if (tune_tendency_kernel == 0 && tiled_tendencies_supported(C)) launch_tend_tiled(C, tune_kchunk, st);
+        else launch_tendencies(C, st);
+    }
     void update_state() {
         halos_prognostic(all_mask(), false);
         if (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
@@ -698,6 +706,24 @@ int om_divergence(om_model_t *m, double *out) {
 
 int om_set_phase_timing(om_model_t *m, int32_t enable) {
     return WITH_MODEL(m, { M->phase_timing = enable != 0; M->evused = 0; });
+}
+
+int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
+    if (!m) return OM_EINVAL;
+    if (key == OM_TUNE_TENDENCY_KERNEL && (value < 0 || value > 1)) return OM_EINVAL;
+    if (key == OM_TUNE_KCHUNK && (value < 1 || value > 1 << 20)) return OM_EINVAL;
+    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
+        else M->tune_kchunk = value;
+    });
+}
+
+int om_get_tuning(const om_model_t *mc, int32_t key, int32_t *value) {
+    om_model_t *m = const_cast<om_model_t *>(mc);
+    if (!m || !value) return OM_EINVAL;
+    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK) return OM_EINVAL;
+    return WITH_MODEL(m, { *value = key == OM_TUNE_TENDENCY_KERNEL ? M->tune_tendency_kernel : M->tune_kchunk; });
 }
 
 int om_last_step_timing(const om_model_t *mc, double *ms_total, double *ms_per_phase, int32_t *launches_per_phase,

@@ -413,9 +413,13 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     }
 }
 
+#ifndef OM_ONLY_DOUBLE
 template void launch_poisson<float>(const PoissonPlan<float> &, const Ctx<float> &, const float *, float *, int, float,
                                     hipStream_t);
+#endif
+#ifndef OM_ONLY_FLOAT
 template void launch_poisson<double>(const PoissonPlan<double> &, const Ctx<double> &, const double *, double *, int, double,
                                      hipStream_t);
+#endif
 
 }  // namespace om

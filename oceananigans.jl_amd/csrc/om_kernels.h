@@ -362,6 +362,9 @@ template <class T> struct PoissonPlan {
 };
 
 template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st);
+// z-marching LDS-tiled tendency kernel (om_tend_tiled.hip) and the configurations it covers
+template <class T> bool tiled_tendencies_supported(const Ctx<T> &C);
+template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream_t st);
 template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);

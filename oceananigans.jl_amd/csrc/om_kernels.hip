@@ -451,7 +451,13 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
     template void launch_correct<T>(const Ctx<T> &, T, hipStream_t);                               \
     template void launch_divergence<T>(const Ctx<T> &, T *, hipStream_t);                          \
     template void launch_halos<T>(const DevGrid<T> &, const HaloJob<T> &, hipStream_t);
+// Each precision is compiled in its own object: FP64 without FMA contraction (the reference's
+// rounding: Julia never fuses a*b+c implicitly), FP32 with it.
+#ifndef OM_ONLY_DOUBLE
 OM_INSTANTIATE(float)
+#endif
+#ifndef OM_ONLY_FLOAT
 OM_INSTANTIATE(double)
+#endif
 
 }  // namespace om

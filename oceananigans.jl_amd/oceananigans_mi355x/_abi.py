@@ -20,6 +20,8 @@ OM_MAX_FIELDS = 3 + OM_MAX_TRACERS
 OM_FIELD_U, OM_FIELD_V, OM_FIELD_W, OM_FIELD_TRACER0 = 0, 1, 2, 3
 OM_FIELD_GN, OM_FIELD_GM = 100, 200
 OM_FIELD_PNHS, OM_FIELD_PHY, OM_FIELD_NU, OM_FIELD_KAPPA0 = 300, 301, 302, 303
+OM_TUNE_TENDENCY_KERNEL = 0
+OM_TUNE_KCHUNK = 1
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
 
@@ -96,6 +98,8 @@ SIGNATURES = {
     "om_set_phase_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "om_last_step_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    "om_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "om_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _lib = None

@@ -290,6 +290,22 @@ class NonhydrostaticModel:
             warnings.warn("Δt == 0 may cause model blowup!")
         _abi.check(self._lib.om_time_step(self._h, float(dt), int(nsteps)), self._h)
 
+    def set_tuning(self, tendency_kernel=None, kchunk=None):
+        """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';
+        kchunk = z levels per block of the tiled kernel.  Results do not depend on them."""
+        if tendency_kernel is not None:
+            v = {"tiled": 0, "generic": 1}[tendency_kernel]
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_TENDENCY_KERNEL, v), self._h)
+        if kchunk is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_KCHUNK, int(kchunk)), self._h)
+
+    def get_tuning(self):
+        v = ctypes.c_int32()
+        _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_TENDENCY_KERNEL, ctypes.byref(v)), self._h)
+        kernel = ("tiled", "generic")[v.value]
+        _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_KCHUNK, ctypes.byref(v)), self._h)
+        return {"tendency_kernel": kernel, "kchunk": v.value}
+
     def set_phase_timing(self, enable=True):
         _abi.check(self._lib.om_set_phase_timing(self._h, 1 if enable else 0), self._h)
 

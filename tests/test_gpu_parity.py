@@ -283,3 +283,41 @@ def test_tendencies_identical_inputs_fp64(case):
         assert rel_err(model.closure_fields.nu.interior(), orc.get(om._abi.OM_FIELD_NU)) <= FP64_TOL
     if kw.get("buoyancy") is not None:
         assert rel_err(model.pressures.pHY.interior(), orc.get(om._abi.OM_FIELD_PHY)) <= FP64_TOL
+
+
+TILED_CASES = {
+    "convection_noisy_fp64": (lambda: convection(N=(16, 16, 16), noise_velocity=1e-2), 1e-14),
+    "wind_convection_ragged_fp64": (lambda: wind_convection(N=(20, 17, 13)), 1e-14),
+    "stretched_convection_ragged_fp64": (lambda: stretched_convection(N=(31, 16, 9)), 1e-14),
+    "convection_noisy_fp32": (lambda: convection(N=(33, 30, 20), noise_velocity=1e-2, float_type=np.float32), 2e-6),
+    "wind_convection_fp32": (lambda: wind_convection(N=(24, 24, 16), float_type=np.float32), 2e-6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TILED_CASES))
+def test_tiled_tendency_kernel_matches_thread_per_cell_kernel(case):
+    """The z-marching LDS-tiled tendency kernel against the thread-per-cell kernel on the same state
+    (ragged tile edges, several z chunk lengths).  Both evaluate the same flux expressions in the
+    same order; in FP32 the compiler may contract a product and a sum into an FMA differently in the
+    two, so differences are measured against the flux scale max|U|·max|c|/Δ, the size of the terms
+    whose differences form a tendency."""
+    builder, tol = TILED_CASES[case]
+    grid, kw, init = builder()
+    model = product_for(grid, kw, init)
+    om.time_step_(model, 1.0)
+    names = prognostic_names(kw)
+    model.set_tuning(tendency_kernel="generic")
+    model.update_state()
+    ref = {n: model.Gn[n].interior().copy() for n in names}
+    umax = max(np.abs(model.velocities[f].interior()).max() for f in "uvw")
+    dmin = min(grid.extent[d] / grid.N[d] for d in range(3))
+    scale = {n: umax * np.abs((model.velocities[n] if n in "uvw" else model.tracers[n]).interior()).max() / dmin
+             for n in names}
+    for kchunk in (1, 5, 16, grid.N[2] + 3):
+        model.set_tuning(tendency_kernel="tiled", kchunk=kchunk)
+        assert model.get_tuning() == {"tendency_kernel": "tiled", "kchunk": kchunk}
+        model.update_state()
+        errs = {n: np.max(np.abs(model.Gn[n].interior() - ref[n])) / scale[n] for n in names}
+        print(case, "kchunk", kchunk, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+        assert all(e <= tol for e in errs.values()), errs
+    model.close()
