@@ -10,10 +10,12 @@ tracer) is compared against the size of those terms.
 Conditioning.  In FP64 the reference evaluates the WENO smoothness indicators on the raw
 stencil values (only its FP32 path subtracts the central value first).  With a large offset
 (T ≈ 20 °C varying by 1e-3, or u = 0.1 + 1e-3 noise) the quadratic forms cancel to ~1e-6 of
-their terms, so any two FP64 evaluations that round differently (FMA contraction, summation
-order) differ by ~1e-7 in the tendencies.  The strict bar is therefore applied to
-well-conditioned variants of every configuration; the specified initial states with offsets are
-checked on their prognostic state with the conditioning bound stated in the test.
+their terms, so two FP64 evaluations that round differently differ by far more than 1e-12.  The
+FP64 kernels are therefore compiled without FMA contraction and evaluate every flux in the
+reference's order, which makes their tendencies bit-identical to the oracle's on identical
+inputs (test_tendencies_identical_inputs_fp64).  The remaining differences after a step come
+from the Poisson solves (different FFT algorithms round differently) amplified by that
+conditioning: the offset cases carry the bound stated beside them.
 """
 import numpy as np
 import pytest
@@ -83,8 +85,8 @@ CASE_BUILDERS = {
 # (test_tendencies_identical_inputs_fp64), the prognostic state after one step at the
 # conditioning bound
 OFFSET_CASES = {
-    "wind_convection": (lambda: wind_convection(N=(16, 16, 16)), 1e-10),
-    "stretched_convection": (lambda: stretched_convection(N=(16, 16, 16)), 1e-10),
+    "wind_convection": (lambda: wind_convection(N=(16, 16, 16)), 1e-12),
+    "stretched_convection": (lambda: stretched_convection(N=(16, 16, 16)), 1e-11),
 }
 
 
@@ -234,19 +236,16 @@ def test_fp32_ten_steps_against_fp64_oracle():
     compare_state(model, orc, kw, fields=("state",), tol=1e-5, label="fp32 wind 10 steps")
 
 
-def test_fp32_free_convection_ten_steps_reported():
-    """The free-convection state in FP32: buoyancy noise of 1e-8 on a 1e-3 background sits at 1e-5
-    of the stored value, below what FP32 resolves to better than ~1 %, so the noise-driven
-    velocities carry FP32 representation errors ≫ 1e-5.  b itself is held to 1e-5; the velocity
-    errors are reported and bounded at 1e-2."""
+def test_fp32_free_convection_ten_steps():
+    """The free-convection LES in FP32 (the benchmark's physics at 16³, with velocity noise) within
+    1e-5 of the FP64 oracle after 10 steps, on FP32-rounded identical inputs."""
     grid, kw, init = convection(N=(16, 16, 16), float_type=np.float32, noise_velocity=1e-2)
     model = product_for(grid, kw, init)
     orc = oracle_for(grid, kw, init, oracle_float_type=np.float64, round_inputs_to=np.float32)
     for _ in range(10):
         om.time_step_(model, 0.5)
     orc.time_step(0.5, 10)
-    errs = compare_state(model, orc, kw, fields=("state",), tol=1e-2, label="fp32 free convection 10 steps")
-    assert errs["b"] <= 1e-5
+    compare_state(model, orc, kw, fields=("state",), tol=1e-5, label="fp32 free convection 10 steps")
 
 
 def test_fp32_full_size_stays_finite_and_projected():
