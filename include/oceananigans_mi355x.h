@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define OM_ABI_VERSION 1
+#define OM_ABI_VERSION 2
 
 /* status codes */
 #define OM_OK            0
@@ -87,6 +87,14 @@ extern "C" {
 /* pressure solver */
 #define OM_SOLVER_FFT                  0
 #define OM_SOLVER_FOURIER_TRIDIAGONAL  1
+#define OM_SOLVER_CONJUGATE_GRADIENT   2   /* ConjugateGradientPoissonSolver */
+
+/* preconditioner of the conjugate-gradient Poisson solver */
+#define OM_PRECOND_DEFAULT              0  /* the reference's default: DiagonallyDominantPreconditioner
+                                              (FFT-based on an immersed grid whose underlying grid has one) */
+#define OM_PRECOND_NONE                 1
+#define OM_PRECOND_DIAGONALLY_DOMINANT  2
+#define OM_PRECOND_FFT                  3  /* FFT-based solver of the underlying grid (Fourier–tridiagonal on stretched z) */
 
 #define OM_MAX_TRACERS 8
 #define OM_MAX_FIELDS  (3 + OM_MAX_TRACERS)
@@ -127,8 +135,13 @@ typedef struct {
     int32_t bc_top[OM_MAX_FIELDS];
     double  bc_bottom_value[OM_MAX_FIELDS];
     double  bc_top_value[OM_MAX_FIELDS];
-    int32_t pressure_solver;      /* OM_SOLVER_*; stretched z requires FOURIER_TRIDIAGONAL */
+    int32_t pressure_solver;      /* OM_SOLVER_*; stretched z requires FOURIER_TRIDIAGONAL or CONJUGATE_GRADIENT */
     int32_t hydrostatic_pressure; /* -1: reference default rule; 0 off; 1 on */
+    /* ConjugateGradientPoissonSolver (conjugate_gradient_poisson_solver.jl:134-163) */
+    double  cg_reltol;            /* < 0: sqrt(eps(FT)) */
+    double  cg_abstol;            /* < 0: sqrt(eps(FT)) */
+    int32_t cg_maxiter;           /* ≤ 0: Nx·Ny·Nz */
+    int32_t cg_preconditioner;    /* OM_PRECOND_* */
     int32_t reserved[8];
 } om_config_t;
 
@@ -170,6 +183,10 @@ int om_set_clock(om_model_t *m, double time, int64_t iteration, int32_t stage,
 
 /* solver building blocks */
 int om_poisson_solve(om_model_t *m, const double *rhs_interior, double *phi_interior);
+/* the last conjugate-gradient solve: iterations, the recursively updated residual norm ‖V⁻¹r‖ the
+ * iteration stopped on, and ‖V⁻¹(b − A x)‖ recomputed in FP64 from the final x
+ * (conjugate_gradient_solver.jl:177-284; no reference counterpart for the recomputed residual) */
+int om_last_solve_info(const om_model_t *m, int32_t *iterations, double *residual_norm, double *true_residual_norm);
 int om_divergence(om_model_t *m, double *div_interior);
 
 /* timing of the last om_time_step call (HIP events on the handle's stream).

@@ -159,6 +159,14 @@ template <class T> struct ModelT : ModelBase {
     std::vector<int> evphase;
     double phase_ms[OM_NPHASES] = {0};
     int phase_n[OM_NPHASES] = {0};
+    // conjugate-gradient Poisson solver work arrays (pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT)
+    bool cg = false;
+    double *cg_r = nullptr, *cg_b = nullptr;       // residual and right-hand side (FP64, padded layout)
+    double *cg_x = nullptr, *cg_p = nullptr, *cg_q = nullptr;   // iterate, search direction, A·p (FP64)
+    T *cg_z = nullptr;                             // preconditioned residual (model precision)
+    double *cg_partial = nullptr, *cg_out = nullptr, *cg_host = nullptr;
+    int cg_iterations = 0;
+    double cg_res_last = 0, cg_true_res_last = 0;
     // launch-shape choices (om_set_tuning)
     int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
     int tune_kchunk = 16;              // z levels per block of the tiled kernel
@@ -197,6 +205,7 @@ template <class T> struct ModelT : ModelBase {
     ~ModelT() override {
         if (st) (void)hipStreamSynchronize(st);
         for (void *p : allocs) (void)hipFree(p);
+        if (cg_host) (void)hipHostFree(cg_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -216,7 +225,7 @@ template <class T> struct ModelT : ModelBase {
         return d;
     }
 
-    T *field_origin(T *base) const { return base + (size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny; }
+    template <class U> U *field_origin(U *base) const { return base + (size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny; }
     T *field_base(const T *origin) const { return const_cast<T *>(origin) - ((size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny); }
 
     void init(const om_config_t &c) {
@@ -287,13 +296,24 @@ template <class T> struct ModelT : ModelBase {
             P.post = upload(post);
         }
         P.tw_y = upload(twiddles<T>(g.Ny));
-        P.tridiagonal = c.pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL ? 1 : 0;
+        // the Fourier–tridiagonal plan serves the FT solver and, on stretched z, the CG preconditioner
+        P.tridiagonal = (c.pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL ||
+                         (c.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT && hg.stretched)) ? 1 : 0;
         P.Lz_line = g.tz == OM_BOUNDED ? 2 * g.Nz : g.Nz;
         P.tw_z = upload(twiddles<T>(P.Lz_line));
         P.lx = upload(eigen<T>(g.Nx, hg.Lx, g.tx, 1));
         P.ly = upload(eigen<T>(g.Ny, hg.Ly, g.ty, 1));
         P.lz = upload(eigen<T>(g.Nz, hg.Lz, g.tz, g.tz == OM_BOUNDED ? 2 : 1));
         P.eps10 = T(10) * std::numeric_limits<T>::epsilon();
+        if (c.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
+            cg = true;
+            cg_r = field_origin(dalloc<double>(fsize)); cg_b = field_origin(dalloc<double>(fsize));
+            cg_x = field_origin(dalloc<double>(fsize)); cg_p = field_origin(dalloc<double>(fsize));
+            cg_q = field_origin(dalloc<double>(fsize)); cg_z = field_origin(dalloc<T>(fsize));
+            cg_partial = dalloc<double>(2 * 1024);
+            cg_out = dalloc<double>(4);
+            HIPCHECK(hipHostMalloc((void **)&cg_host, 4 * sizeof(double), hipHostMallocDefault));
+        }
         HIPCHECK(hipDeviceSynchronize());
     }
 
@@ -369,9 +389,83 @@ template <class T> struct ModelT : ModelBase {
         halos_prognostic(0x7u, true);
         const T eps = std::numeric_limits<T>::epsilon();
         const T dtp = dtau > eps ? dtau : eps;
+        if (cg) {
+            // solve_for_pressure.jl:149-161: warm start from p·Δt★, b = active·δ·V, solve V∇²ϕ = b;
+            // then U −= ∇ϕ and p = ϕ / max(eps, Δt) (pressure_correction.jl:82-106)
+            {
+                PhaseScope ps(this, OM_PHASE_POISSON);
+                const T dts = std::isfinite(dtau) ? dtp : T(0);
+                cg_xio(g, s.pNHS, cg_x, dts, 0, st);
+                cg_source(ctx(), (const T *)nullptr, cg_b, st);
+                cg_solve();
+                cg_xio(g, s.pNHS, cg_x, T(1), 1, st);
+            }
+            halos_aux({s.pNHS});
+            {
+                PhaseScope ps(this, OM_PHASE_CORRECTION);
+                launch_correct(ctx(), T(1), 1, st);
+                cg_scale(g, s.pNHS, dtp, 1, st);
+            }
+            return;
+        }
         { PhaseScope ps(this, OM_PHASE_POISSON); launch_poisson(P, ctx(), (const T *)nullptr, s.pNHS, 0, dtp, st); }
         halos_aux({s.pNHS});
-        { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, st); }
+        { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, 0, st); }
+    }
+
+    // ---- conjugate-gradient solve of V∇²x = b (conjugate_gradient_solver.jl:177-284) with the
+    // zero-mean gauge; x holds the initial guess, cg_b the right-hand side.  The FP64 scalars are
+    // read back after each reduction, as the reference's host loop does.
+    double cg_read(int n) {
+        HIPCHECK(hipMemcpyAsync(cg_host, cg_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return cg_host[0];
+    }
+    int cg_precond() const {
+        const int pk = cfg.cg_preconditioner;
+        return pk == OM_PRECOND_DEFAULT ? OM_PRECOND_DIAGONALLY_DOMINANT : pk;
+    }
+    void cg_solve() {
+        double *x = cg_x;
+        const double seps = std::sqrt((double)std::numeric_limits<T>::epsilon());
+        const double reltol = cfg.cg_reltol < 0 ? (double)(T)seps : cfg.cg_reltol;
+        const double abstol = cfg.cg_abstol < 0 ? (double)(T)seps : cfg.cg_abstol;
+        const int64_t maxiter = cfg.cg_maxiter > 0 ? cfg.cg_maxiter : (int64_t)g.Nx * g.Ny * g.Nz;
+        const double ncell = (double)g.Nx * (double)g.Ny * (double)g.Nz;
+        const int pk = cg_precond();
+        cg_halo_r(g, x, st);
+        cg_residual(g, x, cg_b, cg_r, cg_partial, cg_out, st);
+        const double res0 = std::sqrt(cg_read(1));
+        const double tol = std::max(reltol * res0, abstol);
+        cg_gauge_norm(g, x, cg_r, 0.0, 0.0, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
+        double res = std::sqrt(cg_read(1));
+        int64_t it = 0;
+        double rho_prev = 0;
+        while (it < maxiter && !(res <= tol)) {
+            if (pk == OM_PRECOND_DIAGONALLY_DOMINANT) { cg_halo_r(g, cg_r, st); cg_dd(g, cg_r, cg_z, st); }
+            else if (pk == OM_PRECOND_FFT) {
+                cg_fft_src(g, cg_r, rhs_dense, st);
+                launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);
+            } else cg_copy_r(g, cg_r, cg_z, st);
+            cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out, st);
+            const double rho = cg_read(1);
+            cg_update_p(g, cg_p, cg_z, it == 0 ? 0.0 : rho / rho_prev, it == 0 ? 1 : 0, st);
+            cg_halo_r(g, cg_p, st);
+            cg_apply(g, cg_p, cg_q, cg_partial, cg_out, st);
+            const double alpha = rho / cg_read(1);
+            cg_xr(g, x, cg_p, cg_r, cg_q, alpha, cg_partial, cg_out, st);
+            cg_read(2);
+            const double mx = cg_host[0] / ncell, mr = cg_host[1] / ncell;
+            cg_gauge_norm(g, x, cg_r, mx, mr, cg_partial, cg_out, st);
+            res = std::sqrt(cg_read(1));
+            it += 1;
+            rho_prev = rho;
+        }
+        cg_iterations = (int)it;
+        cg_res_last = res;
+        cg_halo_r(g, x, st);
+        cg_true_residual(g, x, cg_b, cg_partial, cg_out, st);
+        cg_true_res_last = std::sqrt(cg_read(1));
     }
     void rk3_substep(double dt, T gamma, T zeta, bool first) {
         const T kdt = (T)dt;
@@ -520,8 +614,14 @@ static std::string validate(const om_config_t *c) {
     if (c->closure != OM_CLOSURE_NONE && c->closure != OM_CLOSURE_SCALAR_DIFFUSIVITY && c->topology[2] == OM_FLAT)
         return "AMD / Smagorinsky closures need a three-dimensional grid";
     if (c->z_faces && c->topology[2] != OM_BOUNDED) return "stretched z requires a Bounded z";
-    if (c->z_faces && c->pressure_solver != OM_SOLVER_FOURIER_TRIDIAGONAL)
-        return "a stretched z grid needs the Fourier-tridiagonal pressure solver";
+    if (c->z_faces && c->pressure_solver == OM_SOLVER_FFT)
+        return "a stretched z grid needs the Fourier-tridiagonal or the conjugate-gradient pressure solver";
+    if (c->pressure_solver < 0 || c->pressure_solver > OM_SOLVER_CONJUGATE_GRADIENT) return "unknown pressure solver";
+    if (c->pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
+        for (int d = 0; d < 3; ++d)
+            if (c->topology[d] == OM_FLAT) return "the conjugate-gradient solver needs a three-dimensional grid";
+        if (c->cg_preconditioner < OM_PRECOND_DEFAULT || c->cg_preconditioner > OM_PRECOND_FFT) return "unknown CG preconditioner";
+    }
     if (c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL && c->topology[2] != OM_BOUNDED)
         return "the Fourier-tridiagonal solver needs a Bounded z";
     if (c->buoyancy == OM_BUOYANCY_TRACER && (c->buoyancy_tracer < 0 || c->buoyancy_tracer >= c->n_tracers))
@@ -682,7 +782,14 @@ int om_poisson_solve(om_model_t *m, const double *rhs, double *phi) {
         std::vector<TT> h(n);
         for (size_t q = 0; q < n; ++q) h[q] = (TT)rhs[q];
         HIPCHECK(hipMemcpy(M->rhs_dense, h.data(), n * sizeof(TT), hipMemcpyHostToDevice));
-        launch_poisson(M->P, M->ctx(), M->rhs_dense, M->phi_dense, 1, (TT)1, M->st);
+        if (M->cg) {
+            // ∇²ϕ = rhs as V∇²ϕ = V·rhs from a zero initial guess
+            cg_source(M->ctx(), M->rhs_dense, M->cg_b, M->st);
+            cg_xio(M->g, M->cg_z, M->cg_x, (TT)0, 0, M->st);
+            M->cg_solve();
+            cg_gather(M->g, M->cg_x, M->phi_dense, M->st);
+        } else
+            launch_poisson(M->P, M->ctx(), M->rhs_dense, M->phi_dense, 1, (TT)1, M->st);
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpy(h.data(), M->phi_dense, n * sizeof(TT), hipMemcpyDeviceToHost));
@@ -706,6 +813,17 @@ int om_divergence(om_model_t *m, double *out) {
 
 int om_set_phase_timing(om_model_t *m, int32_t enable) {
     return WITH_MODEL(m, { M->phase_timing = enable != 0; M->evused = 0; });
+}
+
+int om_last_solve_info(const om_model_t *mc, int32_t *iterations, double *residual_norm, double *true_residual_norm) {
+    om_model_t *m = const_cast<om_model_t *>(mc);
+    if (!m) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (!M->cg) throw std::invalid_argument("the model's pressure solver is not the conjugate-gradient solver");
+        if (iterations) *iterations = M->cg_iterations;
+        if (residual_norm) *residual_norm = M->cg_res_last;
+        if (true_residual_norm) *true_residual_norm = M->cg_true_res_last;
+    });
 }
 
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {

@@ -1,0 +1,396 @@
+// om_cg.hip — kernels of the preconditioned conjugate-gradient Poisson solver
+// (ConjugateGradientPoissonSolver, src/Solvers/conjugate_gradient_poisson_solver.jl:38-397, driving
+// ConjugateGradientSolver, src/Solvers/conjugate_gradient_solver.jl:177-284).
+//
+// The solver works on V∇²x = b.  The iterate x, the residual r, the right-hand side b, the search
+// direction p and A·p are FP64 in both precisions; only the preconditioner (and its output z) runs
+// in the model's precision.  In an FP32 model the iteration therefore converges below the FP32
+// floor ε·κ(A) (a residual of ~1e-4 for an exact FP32 inverse on a 32³ grid), down to the
+// requested tolerance; the pressure field receives x rounded to FP32.  Every inner product and norm
+// accumulates in FP64 through a two-stage reduction with a fixed grid shape (per-block partial sums,
+// then one block summing the partials in a fixed order), so a solve is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "om_kernels.h"
+
+namespace om {
+
+constexpr int kRedBlocks = 1024;   // partial sums per reduction
+constexpr int kRedThreads = 256;
+
+// sum of v over the block, valid in thread 0 (64-wide waves, then across the 4 waves in LDS)
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double wsum[kRedThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wsum[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kRedThreads / 64; ++w) s += wsum[w];
+    __syncthreads();
+    return s;
+}
+
+// interior cell n (0 .. Nx·Ny·Nz-1) → (i, j, k)
+template <class T> __device__ __forceinline__ void cell_of(const DevGrid<T> &g, int64_t n, int &i, int &j, int &k) {
+    i = (int)(n % g.Nx);
+    const int64_t r = n / g.Nx;
+    j = (int)(r % g.Ny);
+    k = (int)(r / g.Ny);
+}
+
+template <class T> __device__ __forceinline__ T vol(const DevGrid<T> &g, int k) { return g.Az * g.dzc[k]; }
+
+// V∇²ᶜᶜᶜ c (conjugate_gradient_poisson_solver.jl:62-66) evaluated in type E
+template <class E, class T, class F>
+__device__ __forceinline__ E vlap(const DevGrid<T> &g, const F *c, int64_t o, int k) {
+    const E idx = E(1) / E(g.dx), idy = E(1) / E(g.dy);
+    const E Ax = E(g.dy) * E(g.dzc[k]), Ay = E(g.dx) * E(g.dzc[k]), Az = E(g.dx) * E(g.dy);
+    const E c0 = E(c[o]);
+    const E tx = Ax * ((E(c[o + 1]) - c0) * idx) - Ax * ((c0 - E(c[o - 1])) * idx);
+    const E ty = Ay * ((E(c[o + g.sy]) - c0) * idy) - Ay * ((c0 - E(c[o - g.sy])) * idy);
+    const E tz = Az * ((E(c[o + g.sz]) - c0) * (E(1) / E(g.dzf[k + 1]))) - Az * ((c0 - E(c[o - g.sz])) * (E(1) / E(g.dzf[k])));
+    return (tx + ty) + tz;
+}
+
+// ---------------------------------------------------------------------------------------------
+// mode 0: f ·= s, 1: f /= s, 2: f = s (interior)
+template <class T> __global__ void __launch_bounds__(256) k_cg_scale(DevGrid<T> g, T *f, T s, int mode) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        f[o] = mode == 2 ? s : (mode == 1 ? f[o] / s : f[o] * s);
+    }
+}
+
+template <class T> __global__ void __launch_bounds__(256) k_cg_gather(DevGrid<T> g, const double *x, T *dense) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        dense[t] = (T)x[idx3(i, j, k, g.sy, g.sz)];
+    }
+}
+
+// x = (double)(f · s) (the warm start, solve_for_pressure.jl:150-153), or f = (T)x
+template <class T> __global__ void __launch_bounds__(256) k_cg_xio(DevGrid<T> g, T *f, double *x, T s, int store) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        if (store) f[o] = (T)x[o];
+        else x[o] = (double)(f[o] * s);
+    }
+}
+
+// b = active · divᶜᶜᶜ(U) · V (solve_for_pressure.jl:44-50), or b = rhs · V for a given dense rhs
+template <class T> __global__ void __launch_bounds__(256) k_cg_source(Ctx<T> C, const T *dense_rhs, double *b) {
+    const DevGrid<T> &g = C.g;
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        T d;
+        if (dense_rhs) d = dense_rhs[t];
+        else {
+            const T *u = C.s.U[0], *v = C.s.U[1], *w = C.s.U[2];
+            const T Ax = g.dy * g.dzc[k], Ay = g.dx * g.dzc[k];
+            const T dxs = Ax * u[o + 1] - Ax * u[o];
+            const T dys = Ay * v[o + g.sy] - Ay * v[o];
+            const T dzs = g.Az * w[o + g.sz] - g.Az * w[o];
+            d = (T(1) / vol(g, k)) * ((dxs + dys) + dzs);
+        }
+        b[o] = (double)((T(1) * d) * vol(g, k));
+    }
+}
+
+// r = b − A x; partial sums of (r V⁻¹)² (the initial VolumeInverseNorm, which rescales r in place)
+template <class T> __global__ void __launch_bounds__(256) k_cg_residual(DevGrid<T> g, const double *x, const double *b, double *r,
+                                                                        double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double q = vlap<double>(g, x, o, k);
+        double rv = b[o] - q;
+        const double rs = rv * (double)(T(1) / vol(g, k));
+        acc += rs * rs;
+        r[o] = rs * (double)vol(g, k);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// DiagonallyDominantPreconditioner (conjugate_gradient_poisson_solver.jl:256-296); r has its halos
+template <class T> __device__ __forceinline__ double dd_ac(const DevGrid<T> &g, int k) {
+    const double ax = (double)(g.dy * g.dzc[k]) * (double)(T(1) / g.dx);
+    const double ay = (double)(g.dx * g.dzc[k]) * (double)(T(1) / g.dy);
+    const double azm = (double)g.Az * (double)(T(1) / g.dzf[k]);
+    const double azp = (double)g.Az * (double)(T(1) / g.dzf[k + 1]);
+    return ((((-ax - ax) - ay) - ay) - azm) - azp;
+}
+template <class T> __global__ void __launch_bounds__(256) k_cg_dd(DevGrid<T> g, const double *r, T *z) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double ax = (double)(g.dy * g.dzc[k]) * (double)(T(1) / g.dx);
+        const double ay = (double)(g.dx * g.dzc[k]) * (double)(T(1) / g.dy);
+        const double azm = (double)g.Az * (double)(T(1) / g.dzf[k]);
+        const double azp = (double)g.Az * (double)(T(1) / g.dzf[k + 1]);
+        const double ac = dd_ac(g, k), acm = dd_ac(g, k - 1), acp = dd_ac(g, k + 1);
+        double acc = r[o];
+        acc = acc - ax / ac * r[o - 1];
+        acc = acc - ax / ac * r[o + 1];
+        acc = acc - ay / ac * r[o - g.sy];
+        acc = acc - ay / ac * r[o + g.sy];
+        acc = acc - azm / acm * r[o - g.sz];
+        acc = acc - azp / acp * r[o + g.sz];
+        z[o] = (T)(1.0 / fabs(ac) * acc);
+    }
+}
+
+// FFT preconditioner source: r · V⁻¹ into a dense interior array (conjugate_gradient_poisson_solver.jl:169-211)
+template <class T> __global__ void __launch_bounds__(256) k_cg_fft_src(DevGrid<T> g, const double *r, T *dense) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        dense[t] = (T)(r[idx3(i, j, k, g.sy, g.sz)] * (double)(T(1) / vol(g, k)));
+    }
+}
+
+template <class T> __global__ void __launch_bounds__(256) k_cg_copy_r(DevGrid<T> g, const double *r, T *z) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        z[o] = (T)r[o];
+    }
+}
+
+// partial sums of z·r
+template <class T> __global__ void __launch_bounds__(256) k_cg_dot_zr(DevGrid<T> g, const T *z, const double *r, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        acc += (double)z[o] * r[o];
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z (first iteration) or p = z + β p
+template <class T> __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *p, const T *z, double beta, int first) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        p[o] = first ? (double)z[o] : (double)z[o] + beta * p[o];
+    }
+}
+
+// q = A p (p has its halos); partial sums of p·q
+template <class T> __global__ void __launch_bounds__(256) k_cg_apply(DevGrid<T> g, const double *p, double *q, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double qq = vlap<double>(g, p, o, k);
+        q[o] = qq;
+        acc += p[o] * qq;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// x += α p, r −= α q; partial sums of x and of r (for the zero-mean gauge)
+template <class T> __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *x, const double *p, double *r, const double *q,
+                                                                  double alpha, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double sx = 0, sr = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double xn = x[o] + alpha * p[o];
+        const double rn = r[o] - alpha * q[o];
+        x[o] = xn;
+        r[o] = rn;
+        sx += xn;
+        sr += rn;
+    }
+    const double a = block_sum(sx);
+    const double b = block_sum(sr);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = a; partial[kRedBlocks + blockIdx.x] = b; }
+}
+
+// zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep
+template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *x, double *r, double mx, double mr,
+                                                                          double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        x[o] = (x[o] - mx) * 1.0;
+        const double rg = (r[o] - mr) * 1.0;
+        const double rs = rg * (double)(T(1) / vol(g, k));
+        acc += rs * rs;
+        r[o] = rs * (double)vol(g, k);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ‖V⁻¹(b − A x)‖² partial sums with A applied in FP64 to x (x has its halos)
+template <class T> __global__ void __launch_bounds__(256) k_cg_true_residual(DevGrid<T> g, const double *x, const double *b, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double e = (b[o] - vlap<double>(g, x, o, k)) / (double)vol(g, k);
+        acc += e * e;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[v] = Σ partial[v·kRedBlocks + b] over b, in a fixed order
+static __global__ void __launch_bounds__(kRedThreads) k_cg_finish(const double *partial, int nvals, double *out) {
+    for (int v = 0; v < nvals; ++v) {
+        double acc = 0;
+        for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) acc += partial[v * kRedBlocks + b];
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) out[v] = s;
+    }
+}
+
+// halos of an FP64 cell field for the 7-point operators: periodic wrap in x and y (one layer),
+// Bounded z one-layer copy (NoFlux), Periodic z one-layer wrap
+template <class T> __global__ void __launch_bounds__(256) k_cg_halo_r(DevGrid<T> g, double *r, int pass) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pass == 0) {   // z faces over the interior x-y extent
+        const int i = a % g.Nx, j = a / g.Nx;
+        if (j >= g.Ny) return;
+        const int64_t c = idx3(i, j, 0, g.sy, g.sz);
+        if (g.tz == OM_PERIODIC) {
+            r[c - g.sz] = r[c + (int64_t)(g.Nz - 1) * g.sz];
+            r[c + (int64_t)g.Nz * g.sz] = r[c];
+        } else {
+            r[c - g.sz] = r[c];
+            r[c + (int64_t)g.Nz * g.sz] = r[c + (int64_t)(g.Nz - 1) * g.sz];
+        }
+    } else if (pass == 1) {   // x halos over y, z interior
+        const int j = a % g.Ny, k = a / g.Ny;
+        if (k >= g.Nz) return;
+        const int64_t c = idx3(0, j, k, g.sy, g.sz);
+        r[c - 1] = r[c + g.Nx - 1];
+        r[c + g.Nx] = r[c];
+    } else {   // y halos over x, z interior
+        const int i = a % g.Nx, k = a / g.Nx;
+        if (k >= g.Nz) return;
+        const int64_t c = idx3(i, 0, k, g.sy, g.sz);
+        r[c - g.sy] = r[c + (int64_t)(g.Ny - 1) * g.sy];
+        r[c + (int64_t)g.Ny * g.sy] = r[c];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+static dim3 red_grid() { return dim3(kRedBlocks); }
+static dim3 red_blk() { return dim3(kRedThreads); }
+
+template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_scale<T>, red_grid(), red_blk(), 0, st, g, f, s, mode);
+}
+template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_gather<T>, red_grid(), red_blk(), 0, st, g, x, dense);
+}
+template <class T> void cg_xio(const DevGrid<T> &g, T *f, double *x, T s, int store, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_xio<T>, red_grid(), red_blk(), 0, st, g, f, x, s, store);
+}
+template <class T> void cg_source(const Ctx<T> &C, const T *dense_rhs, double *b, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_source<T>, red_grid(), red_blk(), 0, st, C, dense_rhs, b);
+}
+template <class T> void cg_residual(const DevGrid<T> &g, const double *x, const double *b, double *r, double *partial, double *out,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, r, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st) {
+    const int nxy = g.Nx * g.Ny, nyz = g.Ny * g.Nz, nxz = g.Nx * g.Nz;
+    if (g.tz != OM_FLAT) hipLaunchKernelGGL(k_cg_halo_r<T>, dim3((nxy + 255) / 256), dim3(256), 0, st, g, r, 0);
+    hipLaunchKernelGGL(k_cg_halo_r<T>, dim3((nyz + 255) / 256), dim3(256), 0, st, g, r, 1);
+    hipLaunchKernelGGL(k_cg_halo_r<T>, dim3((nxz + 255) / 256), dim3(256), 0, st, g, r, 2);
+}
+template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_dd<T>, red_grid(), red_blk(), 0, st, g, r, z);
+}
+template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_fft_src<T>, red_grid(), red_blk(), 0, st, g, r, dense);
+}
+template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_copy_r<T>, red_grid(), red_blk(), 0, st, g, r, z);
+}
+template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_dot_zr<T>, red_grid(), red_blk(), 0, st, g, z, r, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_update_p<T>, red_grid(), red_blk(), 0, st, g, p, z, beta, first);
+}
+template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_apply<T>, red_grid(), red_blk(), 0, st, g, p, q, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, double alpha,
+                              double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, alpha, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+}
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
+                                      double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, mx, mr, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *partial, double *out,
+                                         hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_true_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+
+#define OM_CG_INSTANTIATE(T)                                                                                     \
+    template void cg_scale<T>(const DevGrid<T> &, T *, T, int, hipStream_t);                                    \
+    template void cg_gather<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                           \
+    template void cg_xio<T>(const DevGrid<T> &, T *, double *, T, int, hipStream_t);                            \
+    template void cg_source<T>(const Ctx<T> &, const T *, double *, hipStream_t);                               \
+    template void cg_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *, double *, \
+                                 hipStream_t);                                                                   \
+    template void cg_halo_r<T>(const DevGrid<T> &, double *, hipStream_t);                                      \
+    template void cg_dd<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                               \
+    template void cg_fft_src<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                          \
+    template void cg_copy_r<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                           \
+    template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
+    template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, double, int, hipStream_t);            \
+    template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \
+    template void cg_xr<T>(const DevGrid<T> &, double *, const double *, double *, const double *, double,       \
+                           double *, double *, hipStream_t);                                                     \
+    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, double, double, double *, double *,  \
+                                   hipStream_t);                                                                 \
+    template void cg_true_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *,   \
+                                      hipStream_t);
+
+#ifndef OM_ONLY_DOUBLE
+OM_CG_INSTANTIATE(float)
+#endif
+#ifndef OM_ONLY_FLOAT
+OM_CG_INSTANTIATE(double)
+#endif
+
+}  // namespace om

@@ -368,7 +368,28 @@ template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream
 template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);
-template <class T> void launch_correct(const Ctx<T> &C, T dtp, hipStream_t st);
+// U −= dtp·∇p (phi = 0: p is the pressure) or U −= ∇ϕ (phi = 1: ϕ = p·Δt, as the reference stores it)
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st);
+// conjugate-gradient Poisson solver kernels (om_cg.hip)
+template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st);
+template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st);
+template <class T> void cg_xio(const DevGrid<T> &g, T *f, double *x, T s, int store, hipStream_t st);
+template <class T> void cg_source(const Ctx<T> &C, const T *dense_rhs, double *b, hipStream_t st);
+template <class T> void cg_residual(const DevGrid<T> &g, const double *x, const double *b, double *r, double *partial,
+                                    double *out, hipStream_t st);
+template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st);
+template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
+template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st);
+template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
+template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);
+template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st);
+template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st);
+template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, double alpha,
+                              double *partial, double *out, hipStream_t st);
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
+                                      double *out, hipStream_t st);
+template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *partial, double *out,
+                                         hipStream_t st);
 template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st);
 template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st);
 // Solve ∇²ϕ = source; writes ϕ / divisor into `out` (the nonhydrostatic pressure p/Δt when

@@ -369,7 +369,7 @@ template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs
 // pNHS = (p Δt) / max(ϵ, Δt) (pressure_correction.jl:103), so the kernel only reads the pressure
 // and never writes it: no thread overwrites a neighbour's value while it is being read.
 // ----------------------------------------------------------------------------------------
-template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T dtp) {
+template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T dtp, int phi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
@@ -377,6 +377,12 @@ template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T 
     const int64_t o = C.at(i, j, k);
     const T *p = C.s.pNHS;
     const T pc = p[o];
+    if (phi) {   // ∂ϕ (pressure_correction.jl:82-89)
+        if (!C.flx()) C.s.U[0][o] -= (pc - p[C.at(i - 1, j, k)]) * C.g.idx;
+        if (!C.fly()) C.s.U[1][o] -= (pc - p[C.at(i, j - 1, k)]) * C.g.idy;
+        if (!C.flz()) C.s.U[2][o] -= (pc - p[C.at(i, j, k - 1)]) * C.g.idzf[k];
+        return;
+    }
     if (!C.flx()) C.s.U[0][o] -= (dtp * (pc - p[C.at(i - 1, j, k)])) * C.g.idx;
     if (!C.fly()) C.s.U[1][o] -= (dtp * (pc - p[C.at(i, j - 1, k)])) * C.g.idy;
     if (!C.flz()) C.s.U[2][o] -= (dtp * (pc - p[C.at(i, j, k - 1)])) * C.g.idzf[k];
@@ -419,9 +425,9 @@ template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int f
     RKArgs<T> a{dt, gamma, zeta, first, nfields};
     hipLaunchKernelGGL(k_rk3<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, a);
 }
-template <class T> void launch_correct(const Ctx<T> &C, T dtp, hipStream_t st) {
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, dtp);
+    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, dtp, phi);
 }
 template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st) {
     dim3 blk(64, 4, 1);
@@ -448,7 +454,7 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
     template void launch_closure<T>(const Ctx<T> &, hipStream_t);                                  \
     template void launch_hydrostatic<T>(const Ctx<T> &, hipStream_t);                              \
     template void launch_rk3<T>(const Ctx<T> &, T, T, T, int, int, hipStream_t);                    \
-    template void launch_correct<T>(const Ctx<T> &, T, hipStream_t);                               \
+    template void launch_correct<T>(const Ctx<T> &, T, int, hipStream_t);                               \
     template void launch_divergence<T>(const Ctx<T> &, T *, hipStream_t);                          \
     template void launch_halos<T>(const DevGrid<T> &, const HaloJob<T> &, hipStream_t);
 // Each precision is compiled in its own object: FP64 without FMA contraction (the reference's

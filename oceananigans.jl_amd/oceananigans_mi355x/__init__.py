@@ -13,6 +13,10 @@ from .physics import (  # noqa: F401
     AnisotropicMinimumDissipation,
     BuoyancyTracer,
     Centered,
+    ConjugateGradientPoissonSolver,
+    DiagonallyDominantPreconditioner,
+    FFTBasedPoissonSolver,
+    FourierTridiagonalPoissonSolver,
     FieldBoundaryConditions,
     FluxBoundaryCondition,
     FPlane,

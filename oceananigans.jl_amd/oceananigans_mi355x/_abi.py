@@ -13,13 +13,14 @@ OM_ADV_NONE, OM_ADV_CENTERED2, OM_ADV_WENO5 = 0, 1, 2
 OM_CLOSURE_NONE, OM_CLOSURE_AMD, OM_CLOSURE_SMAGORINSKY_LILLY, OM_CLOSURE_SCALAR_DIFFUSIVITY = 0, 1, 2, 3
 OM_BUOYANCY_NONE, OM_BUOYANCY_TRACER, OM_BUOYANCY_LINEAR_EOS = 0, 1, 2
 OM_BC_DEFAULT, OM_BC_FLUX, OM_BC_VALUE, OM_BC_GRADIENT = 0, 1, 2, 3
-OM_SOLVER_FFT, OM_SOLVER_FOURIER_TRIDIAGONAL = 0, 1
 OM_MAX_TRACERS = 8
 OM_MAX_FIELDS = 3 + OM_MAX_TRACERS
 
 OM_FIELD_U, OM_FIELD_V, OM_FIELD_W, OM_FIELD_TRACER0 = 0, 1, 2, 3
 OM_FIELD_GN, OM_FIELD_GM = 100, 200
 OM_FIELD_PNHS, OM_FIELD_PHY, OM_FIELD_NU, OM_FIELD_KAPPA0 = 300, 301, 302, 303
+OM_SOLVER_FFT, OM_SOLVER_FOURIER_TRIDIAGONAL, OM_SOLVER_CONJUGATE_GRADIENT = 0, 1, 2
+OM_PRECOND_DEFAULT, OM_PRECOND_NONE, OM_PRECOND_DIAGONALLY_DOMINANT, OM_PRECOND_FFT = 0, 1, 2, 3
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
@@ -56,6 +57,10 @@ class OmConfig(ctypes.Structure):
         ("bc_top_value", ctypes.c_double * OM_MAX_FIELDS),
         ("pressure_solver", ctypes.c_int32),
         ("hydrostatic_pressure", ctypes.c_int32),
+        ("cg_reltol", ctypes.c_double),
+        ("cg_abstol", ctypes.c_double),
+        ("cg_maxiter", ctypes.c_int32),
+        ("cg_preconditioner", ctypes.c_int32),
         ("reserved", ctypes.c_int32 * 8),
     ]
 
@@ -99,6 +104,8 @@ SIGNATURES = {
     "om_last_step_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "om_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "om_last_solve_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]),
     "om_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
 }
 

@@ -11,7 +11,8 @@ import numpy as np
 
 from . import _abi
 from .grids import Bounded, Flat, Periodic, RectilinearGrid
-from .physics import (AnisotropicMinimumDissipation, BuoyancyTracer, Centered, FieldBoundaryConditions, FPlane,
+from .physics import (AnisotropicMinimumDissipation, BuoyancyTracer, Centered, ConjugateGradientPoissonSolver,
+                      FFTBasedPoissonSolver, FieldBoundaryConditions, FourierTridiagonalPoissonSolver, FPlane,
                       ScalarDiffusivity, SeawaterBuoyancy, SmagorinskyLilly, WENO)
 
 VELOCITY_NAMES = ("u", "v", "w")
@@ -119,12 +120,25 @@ def build_config(grid, advection=None, closure=None, buoyancy=None, coriolis=Non
             c.bc_top[f] = fbc.top.kind
             c.bc_top_value[f] = fbc.top.condition
     # pressure solver (NonhydrostaticModels.jl:40-56: FFT for uniform grids, Fourier–tridiagonal for stretched z)
+    c.cg_reltol, c.cg_abstol, c.cg_maxiter, c.cg_preconditioner = -1.0, -1.0, 0, _abi.OM_PRECOND_DEFAULT
     if pressure_solver is None:
         c.pressure_solver = _abi.OM_SOLVER_FOURIER_TRIDIAGONAL if grid.stretched else _abi.OM_SOLVER_FFT
-    elif pressure_solver in ("FFTBasedPoissonSolver", _abi.OM_SOLVER_FFT):
+    elif isinstance(pressure_solver, FFTBasedPoissonSolver) or pressure_solver in ("FFTBasedPoissonSolver", _abi.OM_SOLVER_FFT):
         c.pressure_solver = _abi.OM_SOLVER_FFT
-    elif pressure_solver in ("FourierTridiagonalPoissonSolver", _abi.OM_SOLVER_FOURIER_TRIDIAGONAL):
+    elif isinstance(pressure_solver, FourierTridiagonalPoissonSolver) or \
+            pressure_solver in ("FourierTridiagonalPoissonSolver", _abi.OM_SOLVER_FOURIER_TRIDIAGONAL):
         c.pressure_solver = _abi.OM_SOLVER_FOURIER_TRIDIAGONAL
+    elif isinstance(pressure_solver, ConjugateGradientPoissonSolver):
+        if any(t is Flat for t in grid.topology):
+            raise ValueError("ConjugateGradientPoissonSolver needs a three-dimensional grid")
+        c.pressure_solver = _abi.OM_SOLVER_CONJUGATE_GRADIENT
+        c.cg_preconditioner = pressure_solver.preconditioner_kind()
+        if pressure_solver.reltol is not None:
+            c.cg_reltol = float(pressure_solver.reltol)
+        if pressure_solver.abstol is not None:
+            c.cg_abstol = float(pressure_solver.abstol)
+        if pressure_solver.maxiter is not None:
+            c.cg_maxiter = int(pressure_solver.maxiter)
     else:
         raise ValueError(f"unsupported pressure solver {pressure_solver!r}")
     c.hydrostatic_pressure = {"default": -1, None: 0, False: 0, True: 1}[hydrostatic_pressure_anomaly]
@@ -289,6 +303,13 @@ class NonhydrostaticModel:
             import warnings
             warnings.warn("Δt == 0 may cause model blowup!")
         _abi.check(self._lib.om_time_step(self._h, float(dt), int(nsteps)), self._h)
+
+    def last_solve_info(self):
+        """(iterations, ‖V⁻¹r‖ the iteration stopped on, ‖V⁻¹(b − A x)‖ recomputed in FP64) of the last
+        conjugate-gradient pressure solve."""
+        it, r, t = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        _abi.check(self._lib.om_last_solve_info(self._h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t)), self._h)
+        return it.value, r.value, t.value
 
     def set_tuning(self, tendency_kernel=None, kchunk=None):
         """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';

@@ -109,3 +109,51 @@ class FieldBoundaryConditions:
             if bc is not None:
                 raise ValueError(f"{side} boundary conditions: x and y are Periodic on the MI355X path")
         self.top, self.bottom = top, bottom
+
+
+# ---------------------------------------------------------------------------------------------
+# pressure solvers (src/Solvers/fft_based_poisson_solver.jl, fourier_tridiagonal_poisson_solver.jl,
+# conjugate_gradient_poisson_solver.jl)
+# ---------------------------------------------------------------------------------------------
+class FFTBasedPoissonSolver:
+    """The FFT solver of a grid with uniform spacing (used directly, or as a CG preconditioner)."""
+
+    def __init__(self, grid=None):
+        self.grid = grid
+
+
+class FourierTridiagonalPoissonSolver:
+    """FFT in x, y and a tridiagonal solve in a stretched z."""
+
+    def __init__(self, grid=None):
+        self.grid = grid
+
+
+class DiagonallyDominantPreconditioner:
+    """Pointwise truncated-Neumann-series preconditioner (conjugate_gradient_poisson_solver.jl:256-296)."""
+
+
+class ConjugateGradientPoissonSolver:
+    """ConjugateGradientPoissonSolver(grid; preconditioner, reltol, abstol, maxiter)
+    (conjugate_gradient_poisson_solver.jl:134-163).  preconditioner: "default" (the reference's
+    choice), None, DiagonallyDominantPreconditioner(), or an FFTBasedPoissonSolver /
+    FourierTridiagonalPoissonSolver of the grid.  reltol and abstol default to sqrt(eps(FT)),
+    maxiter to Nx·Ny·Nz."""
+
+    def __init__(self, grid=None, preconditioner="default", reltol=None, abstol=None, maxiter=None):
+        self.grid = grid
+        self.preconditioner = preconditioner
+        self.reltol, self.abstol, self.maxiter = reltol, abstol, maxiter
+
+    def preconditioner_kind(self):
+        from . import _abi
+        p = self.preconditioner
+        if isinstance(p, str) and p == "default":
+            return _abi.OM_PRECOND_DEFAULT
+        if p is None:
+            return _abi.OM_PRECOND_NONE
+        if isinstance(p, DiagonallyDominantPreconditioner):
+            return _abi.OM_PRECOND_DIAGONALLY_DOMINANT
+        if isinstance(p, (FFTBasedPoissonSolver, FourierTridiagonalPoissonSolver)):
+            return _abi.OM_PRECOND_FFT
+        raise ValueError(f"unsupported preconditioner {p!r}")

@@ -19,6 +19,7 @@
 //   pressure               src/Solvers/fft_based_poisson_solver.jl, poisson_eigenvalues.jl,
 //                          fourier_tridiagonal_poisson_solver.jl, batched_tridiagonal_solver.jl,
 //                          src/Models/NonhydrostaticModels/pressure_correction.jl, solve_for_pressure.jl
+//   conjugate gradient     src/Solvers/conjugate_gradient_poisson_solver.jl, conjugate_gradient_solver.jl
 //   time stepping          src/TimeSteppers/runge_kutta_3.jl, nonhydrostatic_rk3_substep.jl
 #include <algorithm>
 #include <cmath>
@@ -239,6 +240,9 @@ template <class T> struct Model {
     Fld<T> U[3 + OM_MAX_TRACERS];      // u, v, w, tracers
     Fld<T> Gn[3 + OM_MAX_TRACERS], Gm[3 + OM_MAX_TRACERS];
     Fld<T> pNHS, pHY, nu, kap[OM_MAX_TRACERS], rhs;
+    Fld<T> cg_r, cg_p, cg_q, cg_z, cg_b;    // conjugate-gradient work fields
+    int cg_iterations = 0;
+    double cg_residual = 0, cg_true_residual = 0;
     bool has_pHY = false;
     double time = 0; int64_t iteration = 0; int stage = 1; double last_dt = INFINITY, last_stage_dt = INFINITY;
     std::string err;
@@ -1104,8 +1108,18 @@ template <class T> static T divergence(const Model<T> &m, int i, int j, int k) {
     return (T(1) / m.V_c(k)) * ((dxs + dys) + dzs);
 }
 
+template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b);
+
 template <class T> static void solve_pressure_field(Model<T> &m, const std::vector<T> &rhs, Fld<T> &phi) {
     const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    if (m.cfg.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
+        for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+            m.cg_b(i, j, k) = rhs[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] * m.V_c(k);
+            phi(i, j, k) = T(0);
+        }
+        cg_solve(m, phi, m.cg_b);
+        return;
+    }
     std::vector<cplx> a((size_t)Nx * Ny * Nz);
     const bool ft = m.cfg.pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL;
     for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
@@ -1121,11 +1135,185 @@ template <class T> static void solve_pressure_field(Model<T> &m, const std::vect
     }
 }
 
+// ----------------------------------------------------------------------------------------
+// ConjugateGradientPoissonSolver (conjugate_gradient_poisson_solver.jl:38-397) driving the
+// ConjugateGradientSolver (conjugate_gradient_solver.jl:177-284).  Solves V∇²x = b with b = V·rhs.
+// ----------------------------------------------------------------------------------------
+template <class T> static bool cg_active(const Model<T> &, int, int, int) { return true; }   // no immersed cells
+
+// V∇²ᶜᶜᶜ (conjugate_gradient_poisson_solver.jl:62-66): δx(Ax ∂x c) + δy(Ay ∂y c) + δz(Az ∂z c)
+template <class T> static T cg_laplacian(const Model<T> &m, const Fld<T> &c, int i, int j, int k) {
+    const T idx = T(1) / m.g.dx, idy = T(1) / m.g.dy;
+    T tx = T(0), ty = T(0), tz = T(0);
+    if (m.g.topo[0] != F) tx = m.Ax_c(k) * ((c(i + 1, j, k) - c(i, j, k)) * idx) - m.Ax_c(k) * ((c(i, j, k) - c(i - 1, j, k)) * idx);
+    if (m.g.topo[1] != F) ty = m.Ay_c(k) * ((c(i, j + 1, k) - c(i, j, k)) * idy) - m.Ay_c(k) * ((c(i, j, k) - c(i, j - 1, k)) * idy);
+    if (m.g.topo[2] != F)
+        tz = m.Az() * ((c(i, j, k + 1) - c(i, j, k)) * (T(1) / m.g.dzF(k + 1))) - m.Az() * ((c(i, j, k) - c(i, j, k - 1)) * (T(1) / m.g.dzF(k)));
+    return (tx + ty) + tz;
+}
+
+template <class T> static void cg_apply_laplacian(Model<T> &m, Fld<T> &out, Fld<T> &c) {
+    fill_aux_halos(m, c);
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) out(i, j, k) = cg_laplacian(m, c, i, j, k);
+}
+
+// interior reductions in FT, summed in a fixed order (Field dot / norm / mean, field.jl:695-712)
+template <class T, class F> static T cg_sum(const Model<T> &m, F f) {
+    T s = T(0);
+    for (int k = 1; k <= m.Nz(); ++k) for (int j = 1; j <= m.Ny(); ++j) for (int i = 1; i <= m.Nx(); ++i) s += f(i, j, k);
+    return s;
+}
+
+// VolumeInverseNorm (conjugate_gradient_poisson_solver.jl:13-34): r ← r V⁻¹, ‖r‖, r ← r V
+template <class T> static T cg_norm(Model<T> &m, Fld<T> &r) {
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) r(i, j, k) = r(i, j, k) * (T(1) / m.V_c(k));
+    T n = std::sqrt(cg_sum<T>(m, [&](int i, int j, int k) { return r(i, j, k) * r(i, j, k); }));
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) r(i, j, k) = r(i, j, k) * m.V_c(k);
+    return n;
+}
+
+// DiagonallyDominantPreconditioner (conjugate_gradient_poisson_solver.jl:256-296)
+template <class T> static void cg_precondition_dd(Model<T> &m, Fld<T> &z, Fld<T> &r) {
+    fill_aux_halos(m, r);
+    auto Axm = [&](int, int, int k) { return m.Ax_c(k) * (T(1) / m.g.dx); };
+    auto Aym = [&](int, int, int k) { return m.Ay_c(k) * (T(1) / m.g.dy); };
+    auto Azm = [&](int, int, int k) { return m.Az() * (T(1) / m.g.dzF(k)); };
+    auto Azp = [&](int, int, int k) { return m.Az() * (T(1) / m.g.dzF(k + 1)); };
+    // a Flat direction has no coupling (its δ operators vanish)
+    const bool fx = m.g.topo[0] == F, fy = m.g.topo[1] == F, fz = m.g.topo[2] == F;
+    auto Ac = [&](int i, int j, int k) {
+        T ax = fx ? T(0) : Axm(i, j, k), ay = fy ? T(0) : Aym(i, j, k);
+        T azm = fz ? T(0) : Azm(i, j, k), azp = fz ? T(0) : Azp(i, j, k);
+        return ((((-ax - ax) - ay) - ay) - azm) - azp;
+    };
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+        T acc = r(i, j, k);
+        if (!fx) {
+            acc = acc - Axm(i, j, k) / Ac(i - 1, j, k) * r(i - 1, j, k);
+            acc = acc - Axm(i + 1, j, k) / Ac(i + 1, j, k) * r(i + 1, j, k);
+        }
+        if (!fy) {
+            acc = acc - Aym(i, j, k) / Ac(i, j - 1, k) * r(i, j - 1, k);
+            acc = acc - Aym(i, j + 1, k) / Ac(i, j + 1, k) * r(i, j + 1, k);
+        }
+        if (!fz) {
+            acc = acc - Azm(i, j, k) / Ac(i, j, k - 1) * r(i, j, k - 1);
+            acc = acc - Azp(i, j, k) / Ac(i, j, k + 1) * r(i, j, k + 1);
+        }
+        z(i, j, k) = (T(1) / std::abs(Ac(i, j, k)) * acc) * (T)(cg_active(m, i, j, k) ? 1 : 0);
+    }
+}
+
+// FFT-based preconditioner (conjugate_gradient_poisson_solver.jl:169-211): solve with rhs r·V⁻¹;
+// the Fourier–tridiagonal solve!(x, solver, b) multiplies its source by Δzᶜ
+// (fourier_tridiagonal_poisson_solver.jl:231-290), so both are the exact inverse of V∇².
+template <class T> static void cg_precondition_fft(Model<T> &m, Fld<T> &z, Fld<T> &r) {
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    std::vector<cplx> a((size_t)Nx * Ny * Nz);
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+        T v = r(i, j, k) * (T(1) / m.V_c(k));
+        if (m.g.stretched) v = v * m.g.dzC(k);
+        a[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] = cplx((double)v, 0.0);
+    }
+    if (m.g.stretched) fourier_tridiagonal_poisson(m, a); else fft_poisson(m, a);
+    for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i)
+        z(i, j, k) = (T)a[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))].real();
+}
+
+template <class T> static int cg_preconditioner_kind(const Model<T> &m) {
+    int pk = m.cfg.cg_preconditioner;
+    if (pk == OM_PRECOND_DEFAULT) pk = OM_PRECOND_DIAGONALLY_DOMINANT;   // no immersed boundary here
+    return pk;
+}
+
+// solve!(x, ::ConjugateGradientSolver, b) (conjugate_gradient_solver.jl:177-284) with the zero-mean
+// gauge (conjugate_gradient_poisson_solver.jl:87-96); x holds the initial guess.
+template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
+    Fld<T> &r = m.cg_r, &p = m.cg_p, &q = m.cg_q, &z = m.cg_z;
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    const T seps = std::sqrt(std::numeric_limits<T>::epsilon());
+    const T reltol = m.cfg.cg_reltol < 0 ? seps : (T)m.cfg.cg_reltol;
+    const T abstol = m.cfg.cg_abstol < 0 ? seps : (T)m.cfg.cg_abstol;
+    const int64_t maxiter = m.cfg.cg_maxiter > 0 ? m.cfg.cg_maxiter : (int64_t)Nx * Ny * Nz;
+    const int pk = cg_preconditioner_kind(m);
+    auto interior = [&](auto fn) {
+        for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) fn(i, j, k);
+    };
+    int64_t iteration = 0;
+    cg_apply_laplacian(m, q, x);
+    interior([&](int i, int j, int k) { r(i, j, k) = b(i, j, k) - q(i, j, k); });
+    const T res0 = cg_norm(m, r);
+    const T tol = std::max(reltol * res0, abstol);
+    T rho_prev = T(0);
+    T res = cg_norm(m, r);
+    while (iteration < maxiter && !(res <= tol)) {
+        if (pk == OM_PRECOND_DIAGONALLY_DOMINANT) cg_precondition_dd(m, z, r);
+        else if (pk == OM_PRECOND_FFT) cg_precondition_fft(m, z, r);
+        else interior([&](int i, int j, int k) { z(i, j, k) = r(i, j, k); });
+        const T rho = cg_sum<T>(m, [&](int i, int j, int k) { return z(i, j, k) * r(i, j, k); });
+        if (iteration == 0) interior([&](int i, int j, int k) { p(i, j, k) = z(i, j, k); });
+        else {
+            const T beta = rho / rho_prev;
+            interior([&](int i, int j, int k) { p(i, j, k) = z(i, j, k) + beta * p(i, j, k); });
+        }
+        cg_apply_laplacian(m, q, p);
+        const T alpha = rho / cg_sum<T>(m, [&](int i, int j, int k) { return p(i, j, k) * q(i, j, k); });
+        interior([&](int i, int j, int k) {
+            x(i, j, k) += alpha * p(i, j, k);
+            r(i, j, k) -= alpha * q(i, j, k);
+        });
+        const T n = (T)(Nx * (double)Ny * Nz);
+        const T mx = cg_sum<T>(m, [&](int i, int j, int k) { return x(i, j, k); }) / n;
+        const T mr = cg_sum<T>(m, [&](int i, int j, int k) { return r(i, j, k); }) / n;
+        interior([&](int i, int j, int k) {
+            const T act = (T)(cg_active(m, i, j, k) ? 1 : 0);
+            x(i, j, k) = (x(i, j, k) - mx) * act;
+            r(i, j, k) = (r(i, j, k) - mr) * act;
+        });
+        iteration += 1;
+        rho_prev = rho;
+        res = cg_norm(m, r);
+    }
+    m.cg_iterations = (int)iteration;
+    m.cg_residual = (double)res;
+    // ‖V⁻¹(b − A x)‖ recomputed in double precision
+    fill_aux_halos(m, x);
+    double s = 0;
+    interior([&](int i, int j, int k) {
+        const double idx = 1.0 / (double)m.g.dx, idy = 1.0 / (double)m.g.dy;
+        auto X = [&](int a, int b2, int c) { return (double)x(a, b2, c); };
+        double lap = 0;
+        if (m.g.topo[0] != F) lap += (double)m.Ax_c(k) * ((X(i + 1, j, k) - X(i, j, k)) * idx - (X(i, j, k) - X(i - 1, j, k)) * idx);
+        if (m.g.topo[1] != F) lap += (double)m.Ay_c(k) * ((X(i, j + 1, k) - X(i, j, k)) * idy - (X(i, j, k) - X(i, j - 1, k)) * idy);
+        if (m.g.topo[2] != F)
+            lap += (double)m.Az() * ((X(i, j, k + 1) - X(i, j, k)) / (double)m.g.dzF(k + 1) - (X(i, j, k) - X(i, j, k - 1)) / (double)m.g.dzF(k));
+        const double e = ((double)b(i, j, k) - lap) / (double)m.V_c(k);
+        s += e * e;
+    });
+    m.cg_true_residual = std::sqrt(s);
+}
+
 // pressure_correction.jl:6-20
 template <class T> static void compute_pressure_correction(Model<T> &m, T dt) {
-    (void)dt;
     for (int f = 0; f < 3; ++f) fill_prognostic_halos(m, f, true);
     const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    if (m.cfg.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
+        // solve_for_pressure.jl:149-161: warm start p·Δt★, rhs = active·δ·V
+        const T eps = std::numeric_limits<T>::epsilon();
+        const T dts = std::isfinite(dt) ? std::max(eps, dt) : T(0);
+        for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) m.pNHS(i, j, k) *= dts;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i)
+            m.cg_b(i, j, k) = (T)(cg_active(m, i, j, k) ? 1 : 0) * divergence(m, i, j, k) * m.V_c(k);
+        cg_solve(m, m.pNHS, m.cg_b);
+        fill_aux_halos(m, m.pNHS);
+        return;
+    }
     std::vector<T> rhs((size_t)Nx * Ny * Nz);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i)
@@ -1243,6 +1431,9 @@ template <class T> static Model<T> *mk(const om_config_t *c) {
     int N[3] = {m->g.N[0], m->g.N[1], m->g.N[2]}, H[3] = {m->g.H[0], m->g.H[1], m->g.H[2]};
     for (int f = 0; f < 3 + m->nt; ++f) { m->U[f].init(N, H); m->Gn[f].init(N, H); m->Gm[f].init(N, H); }
     m->pNHS.init(N, H); m->pHY.init(N, H); m->nu.init(N, H); m->rhs.init(N, H);
+    if (c->pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
+        m->cg_r.init(N, H); m->cg_p.init(N, H); m->cg_q.init(N, H); m->cg_z.init(N, H); m->cg_b.init(N, H);
+    }
     for (int n = 0; n < m->nt; ++n) m->kap[n].init(N, H);
     int rule = c->hydrostatic_pressure;
     if (rule < 0) m->has_pHY = (c->topology[2] != P) && (c->topology[2] != F) && c->buoyancy != OM_BUOYANCY_NONE;
@@ -1354,6 +1545,11 @@ void orc_set_finalize(void *hp, int enforce_incompressibility) {
             update_state(*m);
         }
     });
+}
+
+void orc_last_solve_info(void *hp, int *iterations, double *residual, double *true_residual) {
+    auto *h = (orc_model *)hp;
+    DISPATCH(h, { *iterations = m->cg_iterations; *residual = m->cg_residual; *true_residual = m->cg_true_residual; });
 }
 
 void orc_get_clock(void *hp, double *time, int64_t *it, int *stage) {

@@ -53,6 +53,7 @@ def lib():
         "orc_get_clock": (None, [vp, pd, P(ctypes.c_int64), P(i)]),
         "orc_divergence": (None, [vp, pd]),
         "orc_poisson_solve": (None, [vp, pd, pd]),
+        "orc_last_solve_info": (None, [vp, P(i), pd, pd]),
         "orc_grid_z": (None, [vp, pd, pd, pd, pd]),
         "orc_weno5": (d, [pd, i, i]),
         "orc_weno3": (d, [pd, i, i]),
@@ -188,6 +189,12 @@ class OracleModel:
         out = np.empty_like(r)
         self.L.orc_poisson_solve(self.h, _p(r), _p(out))
         return out.reshape((g.Nx, g.Ny, g.Nz), order="F")
+
+    def last_solve_info(self):
+        """(iterations, recursive residual norm, recomputed FP64 residual norm) of the last CG solve."""
+        it, r, t = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self.L.orc_last_solve_info(self.h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t))
+        return it.value, r.value, t.value
 
     def grid_z(self):
         g = self.grid

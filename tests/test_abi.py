@@ -31,7 +31,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     lib = _abi.load_library()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.om_abi_version() == 1
+    assert lib.om_abi_version() == 2
 
 
 def test_config_struct_layout_matches_c():

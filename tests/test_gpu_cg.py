@@ -1,0 +1,91 @@
+"""Conjugate-gradient Poisson solver (ConjugateGradientPoissonSolver,
+src/Solvers/conjugate_gradient_poisson_solver.jl; ConjugateGradientSolver,
+src/Solvers/conjugate_gradient_solver.jl) on the GPU against the oracle, through the C ABI.
+
+The GPU accumulates inner products in FP64 with a different summation order than the oracle, so
+iterates agree to rounding, not bitwise; tight tolerances (reltol 1e-13) make the compared
+solutions independent of where each run stops."""
+import numpy as np
+import pytest
+
+import oceananigans_mi355x as om
+from cases import convection, oracle_for, prognostic_names, product_for, rel_err, stretched_convection
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(name):
+    if name == "stretched":
+        grid, _, _ = stretched_convection(N=(16, 8, 12))
+        return grid
+    tz = {"PPB": om.Bounded, "PPP": om.Periodic}[name]
+    return om.RectilinearGrid(size=(16, 8, 12), x=(0, 2.0), y=(0, 1.0), z=(-3.0, 0), topology=(om.Periodic, om.Periodic, tz))
+
+
+def _precond(kind, grid):
+    if kind == "fft":
+        return om.FourierTridiagonalPoissonSolver() if grid.stretched else om.FFTBasedPoissonSolver()
+    return {"default": "default", "none": None, "dd": om.DiagonallyDominantPreconditioner()}[kind]
+
+
+@pytest.mark.parametrize("gname", ["PPB", "PPP", "stretched"])
+@pytest.mark.parametrize("pkind", ["default", "none", "fft"])
+def test_cg_poisson_solve_matches_oracle(gname, pkind):
+    from oceananigans_mi355x.synthetic import uniform
+    grid = _grid(gname)
+    kw = dict(advection=om.WENO(),
+              pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=_precond(pkind, grid), reltol=1e-13, abstol=0))
+    model = om.NonhydrostaticModel(grid, **kw)
+    orc = oracle_for(grid, kw, {})
+    rhs = uniform(7, grid.N, -1, 1)
+    dz = np.diff(grid.z_faces) if grid.stretched else np.full(grid.Nz, grid.extent[2] / grid.Nz)
+    rhs -= (rhs * dz).sum() / (dz.sum() * grid.Nx * grid.Ny)          # compatible with the Neumann null space
+    phi = model.poisson_solve(rhs)
+    it, res, true_res = model.last_solve_info()
+    phi_o = orc.poisson_solve(rhs)
+    it_o, res_o, true_o = orc.last_solve_info()
+    e = rel_err(phi, phi_o)
+    print(gname, pkind, "iterations gpu/oracle", it, it_o, "residual", res, true_res, "rel err", e, flush=True)
+    assert e <= 1e-11
+    assert abs(it - it_o) <= max(1, 0.02 * it_o)   # ill-conditioned cases may stop an iteration or two apart
+    assert true_res <= 1e-11 * np.linalg.norm(rhs) * 1.0
+    if pkind == "fft":
+        assert it == 1
+
+
+@pytest.mark.parametrize("pkind", ["default", "fft"])
+def test_cg_model_step_matches_oracle_fp64(pkind):
+    """One RK3 step of the convection LES with the CG pressure solver (warm start from p·Δt, zero-mean
+    gauge, U −= ∇ϕ then p = ϕ/Δt) against the oracle."""
+    grid, kw, init = convection(N=(16, 16, 16), noise_velocity=1e-2)
+    kw = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=_precond(pkind, grid), reltol=1e-13, abstol=0))
+    model = product_for(grid, kw, init)
+    orc = oracle_for(grid, kw, init)
+    om.time_step_(model, 1.0)
+    orc.time_step(1.0, 1)
+    names = prognostic_names(kw)
+    errs = {n: rel_err(model.velocities[n].interior() if q < 3 else model.tracers[n].interior(), orc.get(q))
+            for q, n in enumerate(names)}
+    errs["pNHS"] = rel_err(model.pressures.pNHS.interior(), orc.get(om._abi.OM_FIELD_PNHS))
+    print(pkind, {k: f"{v:.2e}" for k, v in errs.items()}, "iterations", model.last_solve_info(), orc.last_solve_info(), flush=True)
+    assert max(errs.values()) <= 1e-11
+
+
+def test_cg_fp32_model_reaches_fp64_tolerance():
+    """FP32 model, CG with the FP32 FFT preconditioner and FP64 iterate, residual and inner products:
+    asked for a 1e-8 relative residual (beyond FP32's ~1e-4 floor for one exact FP32 inverse), the
+    recursive and the recomputed FP64 residuals agree and the state matches the FFT solver's."""
+    grid, kw, init = convection(N=(32, 32, 24), noise_velocity=1e-2, float_type=np.float32)
+    a = product_for(grid, kw, init)
+    kwc = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=om.FFTBasedPoissonSolver(), reltol=1e-8, abstol=0))
+    b = product_for(grid, kwc, init)
+    for _ in range(3):
+        om.time_step_(a, 1.0)
+        om.time_step_(b, 1.0)
+    it, res, true_res = b.last_solve_info()
+    errs = {n: rel_err(b.velocities[n].interior(), a.velocities[n].interior()) for n in "uvw"}
+    print("cg fp32 vs fft fp32", errs, "iterations", it, "residual", res, "recomputed", true_res, flush=True)
+    assert 1 <= it <= 10
+    assert true_res <= 10 * res + 1e-30
+    assert max(errs.values()) <= 1e-5
+    assert np.abs(b.divergence()).max() <= 1e-4 * max(np.abs(b.velocities.w.interior()).max(), 1e-30)
