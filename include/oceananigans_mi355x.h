@@ -184,8 +184,9 @@ int om_set_clock(om_model_t *m, double time, int64_t iteration, int32_t stage,
 /* solver building blocks */
 int om_poisson_solve(om_model_t *m, const double *rhs_interior, double *phi_interior);
 /* the last conjugate-gradient solve: iterations, the recursively updated residual norm ‖V⁻¹r‖ the
- * iteration stopped on, and ‖V⁻¹(b − A x)‖ recomputed in FP64 from the final x
- * (conjugate_gradient_solver.jl:177-284; no reference counterpart for the recomputed residual) */
+ * iteration stopped on, and ‖V⁻¹(e − ē)‖ with e = b − A x recomputed in FP64 from the final x, in the
+ * iteration's zero-mean gauge (conjugate_gradient_solver.jl:177-284; no reference counterpart for the
+ * recomputed residual) */
 int om_last_solve_info(const om_model_t *m, int32_t *iterations, double *residual_norm, double *true_residual_norm);
 int om_divergence(om_model_t *m, double *div_interior);
 
@@ -208,8 +209,9 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
 
 /* launch-shape choices of this library (no reference counterpart).  Results do not depend on
  * them: every tendency kernel gives identically rounded tendencies. */
-#define OM_TUNE_TENDENCY_KERNEL 0   /* 0: z-marching LDS-tiled kernel where supported (default); 1: thread per cell */
-#define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernel (default 16) */
+#define OM_TUNE_TENDENCY_KERNEL 0   /* 0: z-marching LDS-tiled tendency and closure kernels where supported
+                                       (default); 1: thread-per-cell kernels */
+#define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernels (default 16) */
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 

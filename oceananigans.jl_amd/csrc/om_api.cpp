@@ -375,7 +375,12 @@ template <class T> struct ModelT : ModelBase {
     void update_state() {
         halos_prognostic(all_mask(), false);
         if (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
-            { PhaseScope ps(this, OM_PHASE_CLOSURE); launch_closure(ctx(), st); }
+            {
+                PhaseScope ps(this, OM_PHASE_CLOSURE);
+                const Ctx<T> C = ctx();
+                if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) launch_closure_tiled(C, tune_kchunk, st);
+                else launch_closure(C, st);
+            }
             halos_aux({s.nu});
             if (cfg.closure == OM_CLOSURE_AMD) halos_kappa();
         }
@@ -464,7 +469,9 @@ template <class T> struct ModelT : ModelBase {
         cg_iterations = (int)it;
         cg_res_last = res;
         cg_halo_r(g, x, st);
-        cg_true_residual(g, x, cg_b, cg_partial, cg_out, st);
+        cg_true_residual(g, x, cg_b, cg_q, cg_partial, cg_out, st);
+        const double emean = cg_read(1) / ncell;
+        cg_gauged_norm(g, cg_q, emean, cg_partial, cg_out, st);
         cg_true_res_last = std::sqrt(cg_read(1));
     }
     void rk3_substep(double dt, T gamma, T zeta, bool first) {

@@ -246,15 +246,32 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGri
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// ‖V⁻¹(b − A x)‖² partial sums with A applied in FP64 to x (x has its halos)
-template <class T> __global__ void __launch_bounds__(256) k_cg_true_residual(DevGrid<T> g, const double *x, const double *b, double *partial) {
+// e = b − A x with A applied in FP64 to x (x has its halos); partial sums of e
+template <class T> __global__ void __launch_bounds__(256) k_cg_true_residual(DevGrid<T> g, const double *x, const double *b,
+                                                                             double *e, double *partial) {
     const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double acc = 0;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k; cell_of(g, t, i, j, k);
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double e = (b[o] - vlap<double>(g, x, o, k)) / (double)vol(g, k);
-        acc += e * e;
+        const double v = b[o] - vlap<double>(g, x, o, k);
+        e[o] = v;
+        acc += v;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ‖V⁻¹(e − ē)‖² partial sums: the recomputed residual in the zero-mean gauge the iteration uses
+// (the mean of b − A x is the part of b outside the operator's range, which no x removes)
+template <class T> __global__ void __launch_bounds__(256) k_cg_gauged_norm(DevGrid<T> g, const double *e, double mean,
+                                                                           double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const double v = (e[idx3(i, j, k, g.sy, g.sz)] - mean) / (double)vol(g, k);
+        acc += v * v;
     }
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -359,9 +376,14 @@ template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r,
     hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, mx, mr, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
 }
-template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *partial, double *out,
-                                         hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_true_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, partial);
+template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
+                                         double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_true_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, e, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
+template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, double mean, double *partial, double *out,
+                                       hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_gauged_norm<T>, red_grid(), red_blk(), 0, st, g, e, mean, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
 }
 
@@ -384,7 +406,8 @@ template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, c
     template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, double, double, double *, double *,  \
                                    hipStream_t);                                                                 \
     template void cg_true_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *,   \
-                                      hipStream_t);
+                                      double *, hipStream_t);                                                    \
+    template void cg_gauged_norm<T>(const DevGrid<T> &, const double *, double, double *, double *, hipStream_t);
 
 #ifndef OM_ONLY_DOUBLE
 OM_CG_INSTANTIATE(float)

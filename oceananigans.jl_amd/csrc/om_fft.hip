@@ -34,17 +34,23 @@ __device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, int R, int n, const C2<T> *__restr
     const int nt = blockDim.x * blockDim.y;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     if ((n & (n - 1)) == 0) {
+        // power of two: every index split is a shift and a mask (integer division by a runtime
+        // value costs tens of instructions per butterfly on CDNA)
         const int half = n >> 1;
-        for (int l = half, m = 1; l >= 1; l >>= 1, m <<= 1) {
+        const int lg = __builtin_ctz((unsigned)n), lh = lg - 1;
+        for (int sg = 0; sg < lg; ++sg) {
+            const int m = 1 << sg, lm = (half >> sg) << sg;   // lm = l·m = n/2
             for (int t = tid; t < R * half; t += nt) {
-                const int r = t / half, bf = t - r * half;
-                const int jj = bf / m, kk = bf - jj * m;
-                const C2<T> c0 = a[r * n + kk + jj * m];
-                const C2<T> c1 = a[r * n + kk + jj * m + l * m];
-                C2<T> w = tw[jj * m];
+                const int r = t >> lh, bf = t & (half - 1);
+                const int jj = bf >> sg, kk = bf & (m - 1);
+                const int base = (r << lg) + kk + (jj << sg);
+                const C2<T> c0 = a[base];
+                const C2<T> c1 = a[base + lm];
+                C2<T> w = tw[jj << sg];
                 if (inverse) w.y = -w.y;
-                b[r * n + kk + 2 * jj * m] = cadd<T>(c0, c1);
-                b[r * n + kk + 2 * jj * m + m] = cmul<T>(w, csub<T>(c0, c1));
+                const int ob = (r << lg) + kk + (jj << (sg + 1));
+                b[ob] = cadd<T>(c0, c1);
+                b[ob + m] = cmul<T>(w, csub<T>(c0, c1));
             }
             __syncthreads();
             C2<T> *tmp = a; a = b; b = tmp;
@@ -78,6 +84,7 @@ template <class T> struct XFwdArgs {
     const C2<T> *post;    // exp(-2πi k / Nx), k = 0 .. Nx/2
     int scale_dzc;
     int R;
+    FastDiv fNx, fNy, fMx;
 };
 
 template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const T *src, int i, int j, int k) {
@@ -108,11 +115,11 @@ template <class T> __global__ void __launch_bounds__(256) k_xfwd(XFwdArgs<T> A) 
     if (!half) for (int t = tid; t < R * M; t += blockDim.x) a[t] = cmk<T>(T(0), T(0));
     __syncthreads();
     for (int t = tid; t < R * Nx; t += blockDim.x) {
-        const int r = t / Nx, i = t - r * Nx;
+        const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
         const int row = row0 + r;
         T x = T(0);
         if (row < nrows) {
-            const int j = row % Ny, k = row / Ny;
+            const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
             x = source_at<T>(C, A.src, i, j, k);
             if (A.scale_dzc) x = C.g.dzc[k] * x;
         }
@@ -123,7 +130,7 @@ template <class T> __global__ void __launch_bounds__(256) k_xfwd(XFwdArgs<T> A) 
     __syncthreads();
     C2<T> *Z = lds_fft<T>(a, b, R, M, A.tw, false);
     for (int t = tid; t < R * Mx; t += blockDim.x) {
-        const int r = t / Mx, kk = t - r * Mx;
+        const int r = (int)fdiv((uint32_t)t, A.fMx), kk = t - r * Mx;
         const int row = row0 + r;
         if (row >= nrows) continue;
         if (!half) { A.S[(int64_t)row * Mx + kk] = Z[r * M + kk]; continue; }
@@ -149,6 +156,7 @@ template <class T> struct XBwdArgs {
     T scale;           // 1 / (Nx · Ny · Nz-normalisations)
     T divisor;         // the written value is (x · scale) / divisor
     int R;
+    FastDiv fNx, fNy, fM;
 };
 
 template <class T> __global__ void __launch_bounds__(256) k_xbwd(XBwdArgs<T> A) {
@@ -164,7 +172,7 @@ template <class T> __global__ void __launch_bounds__(256) k_xbwd(XBwdArgs<T> A) 
     const int nrows = Ny * Nz;
     const int tid = threadIdx.x;
     for (int t = tid; t < R * M; t += blockDim.x) {
-        const int r = t / M, kk = t - r * M;
+        const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
         const int row = row0 + r;
         C2<T> Zk = cmk<T>(T(0), T(0));
         if (row < nrows) {
@@ -185,10 +193,10 @@ template <class T> __global__ void __launch_bounds__(256) k_xbwd(XBwdArgs<T> A) 
     __syncthreads();
     C2<T> *z = lds_fft<T>(a, b, R, M, A.tw, true);
     for (int t = tid; t < R * Nx; t += blockDim.x) {
-        const int r = t / Nx, i = t - r * Nx;
+        const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
         const int row = row0 + r;
         if (row >= nrows) continue;
-        const int j = row % Ny, k = row / Ny;
+        const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
         T x;
         if (half) { const C2<T> zz = z[r * M + (i >> 1)]; x = (i & 1) ? zz.y : zz.x; }
         else x = z[r * M + i].x;
@@ -210,7 +218,7 @@ template <class T> struct YArgs {
     int inverse;
     int fused_divide;
     const T *lx, *ly;   // eigenvalues (fused)
-    int R;
+    int R, lgR;         // rows per block (a power of two) and its log2
 };
 
 template <class T> __global__ void __launch_bounds__(256) k_ypass(YArgs<T> A) {
@@ -222,7 +230,7 @@ template <class T> __global__ void __launch_bounds__(256) k_ypass(YArgs<T> A) {
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
     for (int t = tid; t < R * Ny; t += blockDim.x) {
-        const int r = t % R, j = t / R;
+        const int r = t & (R - 1), j = t >> A.lgR;
         const int kx = kx0 + r;
         a[r * Ny + j] = kx < Mx ? A.S[((int64_t)k * Ny + j) * Mx + kx] : cmk<T>(T(0), T(0));
     }
@@ -231,7 +239,7 @@ template <class T> __global__ void __launch_bounds__(256) k_ypass(YArgs<T> A) {
     if (A.fused_divide) {
         C2<T> *other = (res == a) ? b : a;
         for (int t = tid; t < R * Ny; t += blockDim.x) {
-            const int r = t % R, j = t / R;
+            const int r = t & (R - 1), j = t >> A.lgR;
             const int kx = kx0 + r;
             C2<T> v = res[r * Ny + j];
             if (kx < Mx) {
@@ -244,7 +252,7 @@ template <class T> __global__ void __launch_bounds__(256) k_ypass(YArgs<T> A) {
         res = lds_fft<T>(res, other, R, Ny, A.tw, true);
     }
     for (int t = tid; t < R * Ny; t += blockDim.x) {
-        const int r = t % R, j = t / R;
+        const int r = t & (R - 1), j = t >> A.lgR;
         const int kx = kx0 + r;
         if (kx < Mx) A.S[((int64_t)k * Ny + j) * Mx + kx] = res[r * Ny + j];
     }
@@ -260,7 +268,7 @@ template <class T> struct ZArgs {
     const C2<T> *tw;    // table for length L
     int Mx, L;          // L = 2Nz (Bounded) or Nz (Periodic)
     const T *lx, *ly, *lz;   // lz has L entries
-    int R;
+    int R, lgR;
 };
 
 template <class T> __global__ void __launch_bounds__(256) k_zpass(ZArgs<T> A) {
@@ -273,7 +281,7 @@ template <class T> __global__ void __launch_bounds__(256) k_zpass(ZArgs<T> A) {
     const int j = blockIdx.y;
     const int tid = threadIdx.x;
     for (int t = tid; t < R * Nz; t += blockDim.x) {
-        const int r = t % R, k = t / R;
+        const int r = t & (R - 1), k = t >> A.lgR;
         const int kx = kx0 + r;
         const C2<T> v = kx < Mx ? A.S[((int64_t)k * Ny + j) * Mx + kx] : cmk<T>(T(0), T(0));
         a[r * L + k] = v;
@@ -283,7 +291,7 @@ template <class T> __global__ void __launch_bounds__(256) k_zpass(ZArgs<T> A) {
     C2<T> *res = lds_fft<T>(a, b, R, L, A.tw, false);
     C2<T> *other = (res == a) ? b : a;
     for (int t = tid; t < R * L; t += blockDim.x) {
-        const int r = t % R, k = t / R;
+        const int r = t & (R - 1), k = t >> A.lgR;
         const int kx = kx0 + r;
         C2<T> v = res[r * L + k];
         if (kx < Mx) {
@@ -295,7 +303,7 @@ template <class T> __global__ void __launch_bounds__(256) k_zpass(ZArgs<T> A) {
     __syncthreads();
     res = lds_fft<T>(res, other, R, L, A.tw, true);
     for (int t = tid; t < R * Nz; t += blockDim.x) {
-        const int r = t % R, k = t / R;
+        const int r = t & (R - 1), k = t >> A.lgR;
         const int kx = kx0 + r;
         if (kx < Mx) A.S[((int64_t)k * Ny + j) * Mx + kx] = res[r * L + k];
     }
@@ -370,8 +378,12 @@ template <class T> __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A)
 // ---------------------------------------------------------------------------------------------
 static int rows_for(int n, size_t elem, size_t budget) {
     int r = (int)(budget / (2 * (size_t)n * elem));
-    return r < 1 ? 1 : r;
+    if (r < 1) return 1;
+    int p = 1;
+    while (p * 2 <= r) p *= 2;   // a power of two, so row indices split with shifts
+    return p;
 }
+static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
 template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
                                        T divisor, hipStream_t st) {
@@ -384,12 +396,13 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     const int nrows = Ny * Nz;
     auto rows = [&](int n) { int r = rows_for(n, e, budget); return r > 16 ? 16 : r; };
     {
-        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, rows(M)};
+        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, rows(M),
+                      make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx)};
         hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + A.R - 1) / A.R), dim3(256), 2 * A.R * M * e, st, A);
     }
     const bool z_active = g.tz != OM_FLAT;
     {
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 0, z_active ? 0 : 1, P.lx, P.ly, rows(Ny)};
+        YArgs<T> A{g, P.S, P.tw_y, Mx, 0, z_active ? 0 : 1, P.lx, P.ly, rows(Ny), ilog2(rows(Ny))};
         hipLaunchKernelGGL(k_ypass<T>, dim3((Mx + A.R - 1) / A.R, Nz), dim3(256), 2 * A.R * Ny * e, st, A);
     }
     T scale = T(1) / (T(M) * T(Ny));
@@ -400,15 +413,16 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
             while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
             hipLaunchKernelGGL(k_tridiag<T>, dim3((Mx + bw - 1) / bw, Ny), dim3(bw), (size_t)Nz * bw * sizeof(T), st, A);
         } else {
-            ZArgs<T> A{g, P.S, P.tw_z, Mx, P.Lz_line, P.lx, P.ly, P.lz, rows(P.Lz_line)};
+            ZArgs<T> A{g, P.S, P.tw_z, Mx, P.Lz_line, P.lx, P.ly, P.lz, rows(P.Lz_line), ilog2(rows(P.Lz_line))};
             hipLaunchKernelGGL(k_zpass<T>, dim3((Mx + A.R - 1) / A.R, Ny), dim3(256), 2 * A.R * P.Lz_line * e, st, A);
             scale = T(1) / ((T(M) * T(Ny)) * T(P.Lz_line));
         }
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 1, 0, P.lx, P.ly, rows(Ny)};
+        YArgs<T> A{g, P.S, P.tw_y, Mx, 1, 0, P.lx, P.ly, rows(Ny), ilog2(rows(Ny))};
         hipLaunchKernelGGL(k_ypass<T>, dim3((Mx + A.R - 1) / A.R, Nz), dim3(256), 2 * A.R * Ny * e, st, A);
     }
     {
-        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, rows(M)};
+        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, rows(M),
+                      make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)M)};
         hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + A.R - 1) / A.R), dim3(256), 2 * A.R * M * e, st, A);
     }
 }

@@ -17,6 +17,26 @@ template <class T> __device__ __forceinline__ T wdiv(T a, T b);
 template <> __device__ __forceinline__ float wdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <> __device__ __forceinline__ double wdiv<double>(double a, double b) { return a / b; }
 
+// Unsigned division by a loop-invariant divisor d ≥ 1 with one multiply-high, a subtract and two
+// shifts (Granlund & Montgomery, "Division by invariant integers using multiplication", 1994, §4).
+struct FastDiv {
+    uint32_t d, mul, s1, s2;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    FastDiv f;
+    f.d = d;
+    f.mul = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 1 ? l - 1 : 0;
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
+    const uint32_t t1 = __umulhi(f.mul, n);
+    return (t1 + ((n - t1) >> f.s1)) >> f.s2;
+}
+
 // Reconstruction coefficients rounded to FT; the last entry of each tuple is 1 - Σ(others)
 // evaluated in FT, as the reference builds them (reconstruction_coefficients.jl:74-76).
 template <class T> struct RC {
@@ -94,6 +114,9 @@ template <class T> __device__ __forceinline__ T weno3(T s0, T s1, T s2, T s3, bo
     const T sinv = T(1) / (al0 + al1);
     return (al0 * sinv) * (K::d0 * a0 + K::d1 * a1) + (al1 * sinv) * (K::e0 * b0 + K::e1 * b1);
 }
+
+// ℑ of four values, as the reference's nested two-point averages
+template <class T> __device__ __forceinline__ T avg4(T a, T b, T c, T d) { return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d)); }
 
 template <class T> __device__ __forceinline__ T cen4(T m2, T m1, T z0, T p1) {
     typedef RC<T> K;
@@ -366,6 +389,9 @@ template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st);
 template <class T> bool tiled_tendencies_supported(const Ctx<T> &C);
 template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream_t st);
 template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st);
+// z-marching LDS-tiled eddy viscosity / diffusivity kernel (om_closure_tiled.hip)
+template <class T> bool tiled_closure_supported(const Ctx<T> &C);
+template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStream_t st);
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);
 // U −= dtp·∇p (phi = 0: p is the pressure) or U −= ∇ϕ (phi = 1: ϕ = p·Δt, as the reference stores it)
@@ -388,8 +414,10 @@ template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, d
                               double *partial, double *out, hipStream_t st);
 template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
                                       double *out, hipStream_t st);
-template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *partial, double *out,
-                                         hipStream_t st);
+template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
+                                         double *out, hipStream_t st);
+template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, double mean, double *partial, double *out,
+                                       hipStream_t st);
 template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st);
 template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st);
 // Solve ∇²ϕ = source; writes ϕ / divisor into `out` (the nonhydrostatic pressure p/Δt when

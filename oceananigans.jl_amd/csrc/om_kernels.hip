@@ -127,7 +127,6 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
 // Eddy viscosity / diffusivity (AMD: anisotropic_minimum_dissipation.jl:161-358;
 // Smagorinsky-Lilly: smagorinsky.jl:90-104, lilly_coefficient.jl:129-142)
 // ----------------------------------------------------------------------------------------
-template <class T> __device__ __forceinline__ T avg4(T a, T b, T c, T d) { return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d)); }
 
 template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -241,7 +241,9 @@ class NonhydrostaticModel:
         self.tracers = _NamedFields((n, Field(self, 3 + i, n)) for i, n in enumerate(self.tracer_names))
         self.pressures = _NamedFields(pNHS=Field(self, _abi.OM_FIELD_PNHS, "pNHS"),
                                       pHY=Field(self, _abi.OM_FIELD_PHY, "pHY′"))
-        self.closure_fields = _NamedFields(nu=Field(self, _abi.OM_FIELD_NU, "νₑ"))
+        self.closure_fields = _NamedFields([("nu", Field(self, _abi.OM_FIELD_NU, "νₑ"))] +
+                                           [("kappa_" + n, Field(self, _abi.OM_FIELD_KAPPA0 + i, "κₑ " + n))
+                                            for i, n in enumerate(self.tracer_names)])
         self.Gn = _NamedFields((n, Field(self, _abi.OM_FIELD_GN + i, "G" + n)) for i, n in enumerate(names))
         self.Gm = _NamedFields((n, Field(self, _abi.OM_FIELD_GM + i, "G⁻" + n)) for i, n in enumerate(names))
         self.clock = Clock(self)

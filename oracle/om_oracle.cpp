@@ -1281,9 +1281,10 @@ template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
     }
     m.cg_iterations = (int)iteration;
     m.cg_residual = (double)res;
-    // ‖V⁻¹(b − A x)‖ recomputed in double precision
+    // ‖V⁻¹(e − ē)‖ with e = b − A x recomputed in double precision (the zero-mean gauge of the iteration)
     fill_aux_halos(m, x);
-    double s = 0;
+    std::vector<double> ev((size_t)Nx * Ny * Nz);
+    double esum = 0;
     interior([&](int i, int j, int k) {
         const double idx = 1.0 / (double)m.g.dx, idy = 1.0 / (double)m.g.dy;
         auto X = [&](int a, int b2, int c) { return (double)x(a, b2, c); };
@@ -1292,7 +1293,14 @@ template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
         if (m.g.topo[1] != F) lap += (double)m.Ay_c(k) * ((X(i, j + 1, k) - X(i, j, k)) * idy - (X(i, j, k) - X(i, j - 1, k)) * idy);
         if (m.g.topo[2] != F)
             lap += (double)m.Az() * ((X(i, j, k + 1) - X(i, j, k)) / (double)m.g.dzF(k + 1) - (X(i, j, k) - X(i, j, k - 1)) / (double)m.g.dzF(k));
-        const double e = ((double)b(i, j, k) - lap) / (double)m.V_c(k);
+        const double e = (double)b(i, j, k) - lap;
+        ev[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] = e;
+        esum += e;
+    });
+    const double emean = esum / ((double)Nx * Ny * Nz);
+    double s = 0;
+    interior([&](int i, int j, int k) {
+        const double e = (ev[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] - emean) / (double)m.V_c(k);
         s += e * e;
     });
     m.cg_true_residual = std::sqrt(s);

@@ -295,7 +295,7 @@ TILED_CASES = {
 
 @pytest.mark.parametrize("case", sorted(TILED_CASES))
 def test_tiled_tendency_kernel_matches_thread_per_cell_kernel(case):
-    """The z-marching LDS-tiled tendency kernel against the thread-per-cell kernel on the same state
+    """The z-marching LDS-tiled tendency and closure kernels against the thread-per-cell kernels on the same state
     (ragged tile edges, several z chunk lengths).  Both evaluate the same flux expressions in the
     same order; in FP32 the compiler may contract a product and a sum into an FMA differently in the
     two, so differences are measured against the flux scale max|U|·max|c|/Δ, the size of the terms
@@ -308,6 +308,9 @@ def test_tiled_tendency_kernel_matches_thread_per_cell_kernel(case):
     model.set_tuning(tendency_kernel="generic")
     model.update_state()
     ref = {n: model.Gn[n].interior().copy() for n in names}
+    closure_names = ["nu"] + (["kappa_" + n for n in names[3:]]
+                              if isinstance(kw.get("closure"), om.AnisotropicMinimumDissipation) else [])
+    ref_closure = {c: model.closure_fields[c].interior().copy() for c in closure_names}
     umax = max(np.abs(model.velocities[f].interior()).max() for f in "uvw")
     dmin = min(grid.extent[d] / grid.N[d] for d in range(3))
     scale = {n: umax * np.abs((model.velocities[n] if n in "uvw" else model.tracers[n]).interior()).max() / dmin
@@ -317,6 +320,7 @@ def test_tiled_tendency_kernel_matches_thread_per_cell_kernel(case):
         assert model.get_tuning() == {"tendency_kernel": "tiled", "kchunk": kchunk}
         model.update_state()
         errs = {n: np.max(np.abs(model.Gn[n].interior() - ref[n])) / scale[n] for n in names}
+        errs.update({c: rel_err(model.closure_fields[c].interior(), ref_closure[c]) for c in closure_names})
         print(case, "kchunk", kchunk, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
         assert all(e <= tol for e in errs.values()), errs
     model.close()
