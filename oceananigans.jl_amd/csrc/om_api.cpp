@@ -609,6 +609,8 @@ static std::string validate(const om_config_t *c) {
     if (c->topology[0] != OM_PERIODIC || c->topology[1] != OM_PERIODIC)
         return "x and y must be Periodic on the MI355X path";
     if (c->N[0] < 2) return "Nx must be at least 2";
+    for (int d = 0; d < 3; ++d)
+        if (c->topology[d] == OM_PERIODIC && c->N[d] < c->H[d]) return "a Periodic dimension needs at least as many cells as halo points";
     if (c->n_tracers < 0 || c->n_tracers > OM_MAX_TRACERS) return "n_tracers out of range";
     int need = c->advection == OM_ADV_WENO5 ? 3 : 1;
     if (c->closure != OM_CLOSURE_NONE && need < 2) need = 2;

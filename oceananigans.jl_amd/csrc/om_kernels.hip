@@ -336,6 +336,104 @@ template <class T> __global__ void k_halo_periodic(DevGrid<T> g, HaloJob<T> J, i
 }
 
 // ----------------------------------------------------------------------------------------
+// All halo regions of all fields of a job in one launch.  The reference fills z first (over the
+// interior x-y extent), then the periodic x halos over the whole parent y-z extent, then the
+// periodic y halos over the whole parent x-z extent (boundary_condition_ordering.jl:17-46); so the
+// final value of any halo point (i, j, k) is the z-filled value of the interior column
+// (wrap(i), wrap(j)) at level k.  Each point is computed from interior values only (never from a
+// location this launch writes), so the points are independent.
+// ----------------------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T halo_zvalue(const DevGrid<T> &g, const HaloJob<T> &J, int n, const T *c, int64_t col, int k) {
+    if (g.tz == OM_BOUNDED) {
+        const int kb = J.kind_bottom[n], kt = J.kind_top[n];
+        if (kb == -2) return c[col + (int64_t)k * g.sz];
+        if (kb == -1) return (k == 0 || k == g.Nz) ? T(0) : c[col + (int64_t)k * g.sz];
+        if (k == -1) {
+            const T c1 = c[col];
+            T h = c1;
+            if (kb == OM_BC_VALUE || kb == OM_BC_GRADIENT) {
+                const T D = g.dzf[0];
+                const T grad = kb == OM_BC_GRADIENT ? J.val_bottom[n] : (c1 - J.val_bottom[n]) / (D / T(2));
+                h = c1 + grad * (-D);
+            }
+            return h;
+        }
+        if (k == g.Nz) {
+            const T cN = c[col + (int64_t)(g.Nz - 1) * g.sz];
+            T h = cN;
+            if (kt == OM_BC_VALUE || kt == OM_BC_GRADIENT) {
+                const T D = g.dzf[g.Nz];
+                const T grad = kt == OM_BC_GRADIENT ? J.val_top[n] : (J.val_top[n] - cN) / (D / T(2));
+                h = cN + grad * D;
+            }
+            return h;
+        }
+        return c[col + (int64_t)k * g.sz];
+    }
+    if (g.tz == OM_PERIODIC) {
+        int kk = k;
+        if (k < 0) kk = k + g.Nz;
+        else if (k >= g.Nz && k < g.Nz + g.Hz) kk = k - g.Nz;
+        return c[col + (int64_t)kk * g.sz];
+    }
+    return c[col + (int64_t)k * g.sz];
+}
+
+template <class T> __global__ void __launch_bounds__(256) k_halo_fused(DevGrid<T> g, HaloJob<T> J, int64_t n_ring, int64_t n_z) {
+    const int n = blockIdx.y;
+    T *c = J.f[n];
+    const int pnx = g.Nx + 2 * g.Hx, pnz = g.Nz + 2 * g.Hz + 1;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    auto wrap = [](int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); };
+    if (t < n_ring) {
+        // x-y halo columns over the whole parent z extent: rows j < 0 or j ≥ Ny (full parent width),
+        // then the side columns i < 0 or i ≥ Nx of the interior rows
+        const int64_t per_k = n_ring / pnz;
+        const int kk = (int)(t / per_k);
+        int64_t r = t - (int64_t)kk * per_k;
+        const int k = kk - g.Hz;
+        int i, j;
+        const int64_t band = (int64_t)g.Hy * pnx;
+        if (r < 2 * band) {
+            const int top = r >= band;
+            if (top) r -= band;
+            const int jj = (int)(r / pnx);
+            i = (int)(r - (int64_t)jj * pnx) - g.Hx;
+            j = top ? g.Ny + jj : jj - g.Hy;
+        } else {
+            r -= 2 * band;
+            const int w2 = 2 * g.Hx;
+            const int jj = (int)(r / w2);
+            const int q = (int)(r - (int64_t)jj * w2);
+            j = jj;
+            i = q < g.Hx ? q - g.Hx : g.Nx + (q - g.Hx);
+        }
+        const int is = wrap(i, g.Nx), js = wrap(j, g.Ny);
+        const int64_t src = idx3(is, js, 0, g.sy, g.sz);
+        c[idx3(i, j, k, g.sy, g.sz)] = halo_zvalue(g, J, n, c, src, k);
+        return;
+    }
+    const int64_t u = t - n_ring;
+    if (u >= n_z) return;
+    // z halo levels of the interior columns
+    int k;
+    const int64_t nxy = (int64_t)g.Nx * g.Ny;
+    const int layer = (int)(u / nxy);
+    const int64_t ij = u - (int64_t)layer * nxy;
+    if (g.tz == OM_BOUNDED) {
+        const int kb = J.kind_bottom[n];
+        if (kb == -2) return;
+        k = layer == 0 ? (kb == -1 ? 0 : -1) : g.Nz;
+    } else {
+        k = layer < g.Hz ? -1 - layer : g.Nz + (layer - g.Hz);
+    }
+    const int j = (int)(ij / g.Nx), i = (int)(ij - (int64_t)j * g.Nx);
+    const int64_t col = idx3(i, j, 0, g.sy, g.sz);
+    c[col + (int64_t)k * g.sz] = halo_zvalue(g, J, n, c, col, k);
+}
+
+// ----------------------------------------------------------------------------------------
 // RK3 update of all prognostic fields, with the flux-BC tendency contributions added to Gⁿ
 // first (compute_flux_bcs.jl:113-162, runge_kutta_3.jl:196-204, nonhydrostatic_rk3_substep.jl:37-46)
 // ----------------------------------------------------------------------------------------
@@ -434,6 +532,16 @@ template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t s
 }
 template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st) {
     if (J.nf == 0) return;
+    if (g.tx != OM_BOUNDED && g.ty != OM_BOUNDED) {
+        const int64_t pnx = g.Nx + 2 * g.Hx, pny = g.Ny + 2 * g.Hy, pnz = g.Nz + 2 * g.Hz + 1;
+        const int64_t n_ring = (pnx * pny - (int64_t)g.Nx * g.Ny) * pnz;
+        const int64_t layers = g.tz == OM_BOUNDED ? 2 : (g.tz == OM_PERIODIC ? 2 * g.Hz : 0);
+        const int64_t n_z = layers * g.Nx * g.Ny;
+        const int64_t total = n_ring + n_z;
+        if (total == 0) return;
+        hipLaunchKernelGGL(k_halo_fused<T>, dim3((unsigned)((total + 255) / 256), J.nf), dim3(256), 0, st, g, J, n_ring, n_z);
+        return;
+    }
     if (g.tz == OM_BOUNDED) {
         dim3 blk(64, 4, 1);
         hipLaunchKernelGGL(k_halo_z<T>, dim3((g.Nx + 63) / 64, (g.Ny + 3) / 4, J.nf), blk, 0, st, g, J);
