@@ -142,6 +142,10 @@ typedef struct {
     double  cg_abstol;            /* < 0: sqrt(eps(FT)) */
     int32_t cg_maxiter;           /* ≤ 0: Nx·Ny·Nz */
     int32_t cg_preconditioner;    /* OM_PRECOND_* */
+    /* GridFittedBottom immersed boundary (grid_fitted_bottom.jl:17-20,118-147): bottom height z_b(x, y)
+     * at the Nx·Ny cell centres (x fastest), or NULL for none.  Cells whose centre lies at or below
+     * z_b are immersed (CenterImmersedCondition).  Requires WENO(order=5) or no advection. */
+    const double *bottom_height;
     int32_t reserved[8];
 } om_config_t;
 

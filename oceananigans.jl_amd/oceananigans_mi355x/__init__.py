@@ -8,7 +8,7 @@ trailing underscore (``set_``, ``time_step_``, ``run_``).  All arithmetic runs i
 moves host arrays and sequences calls.
 """
 from ._abi import OmError, load_library  # noqa: F401
-from .grids import Bounded, Flat, MI355X, Periodic, RectilinearGrid  # noqa: F401
+from .grids import Bounded, Flat, GridFittedBottom, ImmersedBoundaryGrid, MI355X, Periodic, RectilinearGrid  # noqa: F401
 from .physics import (  # noqa: F401
     AnisotropicMinimumDissipation,
     BuoyancyTracer,

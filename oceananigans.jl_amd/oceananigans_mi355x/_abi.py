@@ -61,6 +61,7 @@ class OmConfig(ctypes.Structure):
         ("cg_abstol", ctypes.c_double),
         ("cg_maxiter", ctypes.c_int32),
         ("cg_preconditioner", ctypes.c_int32),
+        ("bottom_height", ctypes.POINTER(ctypes.c_double)),
         ("reserved", ctypes.c_int32 * 8),
     ]
 

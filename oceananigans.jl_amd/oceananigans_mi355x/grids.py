@@ -130,3 +130,50 @@ class RectilinearGrid:
         t = ", ".join(t.__name__ for t in self.topology)
         return (f"{self.N[0]}×{self.N[1]}×{self.N[2]} RectilinearGrid{{{np.dtype(self.float_type).name}, {t}}} "
                 f"on {self.architecture!r} with {self.H[0]}×{self.H[1]}×{self.H[2]} halo")
+
+
+class GridFittedBottom:
+    """GridFittedBottom(bottom_height) (src/ImmersedBoundaries/grid_fitted_bottom.jl:17-20): cells
+    whose centre lies at or below the bottom height are immersed (CenterImmersedCondition).
+    bottom_height: a function (x, y) → z_b or an (Nx, Ny) array at the cell centres."""
+
+    def __init__(self, bottom_height):
+        self.bottom_height = bottom_height
+
+    def heights(self, grid):
+        h = self.bottom_height
+        if callable(h):
+            X, Y = np.meshgrid(grid.xnodes("c"), grid.ynodes("c"), indexing="ij")
+            return np.asarray(h(X, Y), dtype=np.float64) * np.ones((grid.Nx, grid.Ny))
+        h = np.asarray(h, dtype=np.float64)
+        if h.shape != (grid.Nx, grid.Ny):
+            raise ValueError(f"bottom height must have shape {(grid.Nx, grid.Ny)}")
+        return h
+
+    def __repr__(self):
+        return f"GridFittedBottom({self.bottom_height!r})"
+
+
+class ImmersedBoundaryGrid:
+    """ImmersedBoundaryGrid(grid, immersed_boundary) (src/ImmersedBoundaries/immersed_boundary_grid.jl):
+    the underlying RectilinearGrid with a GridFittedBottom.  Every grid attribute is the underlying
+    grid's."""
+
+    def __init__(self, grid, immersed_boundary):
+        if not isinstance(grid, RectilinearGrid):
+            raise TypeError("the underlying grid must be a RectilinearGrid")
+        if not isinstance(immersed_boundary, GridFittedBottom):
+            raise TypeError("only GridFittedBottom immersed boundaries are supported")
+        if grid.topology[2] is not Bounded or any(t is Flat for t in grid.topology):
+            raise ValueError("a GridFittedBottom needs a three-dimensional grid with a Bounded z")
+        self.underlying_grid = grid
+        self.immersed_boundary = immersed_boundary
+
+    def __getattr__(self, name):
+        return getattr(self.__dict__["underlying_grid"], name)
+
+    def bottom_heights(self):
+        return self.immersed_boundary.heights(self.underlying_grid)
+
+    def __repr__(self):
+        return f"ImmersedBoundaryGrid({self.underlying_grid!r}, {self.immersed_boundary!r})"

@@ -10,7 +10,7 @@ import inspect
 import numpy as np
 
 from . import _abi
-from .grids import Bounded, Flat, Periodic, RectilinearGrid
+from .grids import Bounded, Flat, ImmersedBoundaryGrid, Periodic, RectilinearGrid
 from .physics import (AnisotropicMinimumDissipation, BuoyancyTracer, Centered, ConjugateGradientPoissonSolver,
                       FFTBasedPoissonSolver, FieldBoundaryConditions, FourierTridiagonalPoissonSolver, FPlane,
                       ScalarDiffusivity, SeawaterBuoyancy, SmagorinskyLilly, WENO)
@@ -22,8 +22,8 @@ def build_config(grid, advection=None, closure=None, buoyancy=None, coriolis=Non
                  boundary_conditions=None, pressure_solver=None, hydrostatic_pressure_anomaly="default"):
     """Return (om_config_t, keepalive) for a model description.  keepalive holds arrays the
     struct points to (z faces)."""
-    if not isinstance(grid, RectilinearGrid):
-        raise TypeError("grid must be a RectilinearGrid")
+    if not isinstance(grid, (RectilinearGrid, ImmersedBoundaryGrid)):
+        raise TypeError("grid must be a RectilinearGrid or an ImmersedBoundaryGrid")
     if isinstance(tracers, str):
         tracers = (tracers,)
     tracers = tuple(tracers)
@@ -38,6 +38,13 @@ def build_config(grid, advection=None, closure=None, buoyancy=None, coriolis=Non
         c.origin[d] = grid.origin[d]
         c.extent[d] = grid.extent[d]
     keep = []
+    c.bottom_height = None
+    if isinstance(grid, ImmersedBoundaryGrid):
+        if advection is not None and not (isinstance(advection, WENO) and advection.order == 5):
+            raise ValueError("an immersed bottom needs WENO(order=5) advection (or none)")
+        hb = np.ascontiguousarray(grid.bottom_heights().reshape(-1, order="F"), dtype=np.float64)
+        keep.append(hb)
+        c.bottom_height = hb.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     if grid.z_faces is not None:
         zf = np.ascontiguousarray(grid.z_faces, dtype=np.float64)
         keep.append(zf)

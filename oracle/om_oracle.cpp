@@ -20,6 +20,10 @@
 //                          fourier_tridiagonal_poisson_solver.jl, batched_tridiagonal_solver.jl,
 //                          src/Models/NonhydrostaticModels/pressure_correction.jl, solve_for_pressure.jl
 //   conjugate gradient     src/Solvers/conjugate_gradient_poisson_solver.jl, conjugate_gradient_solver.jl
+//   immersed bottom        src/ImmersedBoundaries/grid_fitted_bottom.jl, immersed_boundary_interface.jl,
+//                          conditional_differences.jl, mask_immersed_field.jl, src/Grids/inactive_node.jl,
+//                          src/Advection/immersed_advective_fluxes.jl, src/TurbulenceClosures/immersed_diffusive_fluxes.jl,
+//                          src/Coriolis/coriolis_schemes.jl
 //   time stepping          src/TimeSteppers/runge_kutta_3.jl, nonhydrostatic_rk3_substep.jl
 #include <algorithm>
 #include <cmath>
@@ -244,6 +248,9 @@ template <class T> struct Model {
     int cg_iterations = 0;
     double cg_residual = 0, cg_true_residual = 0;
     bool has_pHY = false;
+    // GridFittedBottom: kbot_[(i-1) + Nx (j-1)] = number of immersed cells at the bottom of column (i, j)
+    bool ib = false;
+    std::vector<int> kbot_;
     double time = 0; int64_t iteration = 0; int stage = 1; double last_dt = INFINITY, last_stage_dt = INFINITY;
     std::string err;
     int nthreads = 1;
@@ -262,6 +269,67 @@ template <class T> struct Model {
     T Az() const { return g.dx * g.dy; }
     T V_c(int k) const { return (g.dx * g.dy) * g.dzC(k); }
     T V_f(int k) const { return (g.dx * g.dy) * g.dzF(k); }
+
+    int kbot(int i, int j) const {   // halo columns take the periodic images (fill_halo_regions!(bottom_field))
+        const int Nx = g.N[0], Ny = g.N[1];
+        const int ii = ((i - 1) % Nx + Nx) % Nx, jj = ((j - 1) % Ny + Ny) % Ny;
+        return kbot_[(size_t)ii + (size_t)Nx * jj];
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// Immersed-boundary predicates (src/Grids/inactive_node.jl:36-137, immersed_boundary_interface.jl)
+// Locations: lx, ly, lz = 0 (Center) or 1 (Face).  `under` evaluates the predicate on the
+// underlying grid (only cells outside a Bounded z are inactive there).
+// ----------------------------------------------------------------------------------------
+template <class T> struct IBP {
+    const Model<T> &m;
+    explicit IBP(const Model<T> &mm) : m(mm) {}
+    bool outside(int k) const { return m.bz() && (k < 1 || k > m.Nz()); }
+    // _immersed_cell: zᶜ(k) ≤ numerical bottom ⟺ k ≤ number of immersed cells (grid_fitted_bottom.jl:141-147)
+    bool immersed(int i, int j, int k) const { return m.ib && k <= m.kbot(i, j); }
+    bool inactive_cell(int i, int j, int k, bool under) const { return (!under && immersed(i, j, k)) || outside(k); }
+    bool inactive_node(int i, int j, int k, int lx, int ly, int lz, bool under) const {
+        if (lx && ly && lz) return inactive_node(i, j, k, 0, 1, 1, under) && inactive_node(i - 1, j, k, 0, 1, 1, under);
+        if (lx && ly) return inactive_node(i, j, k, 0, 1, 0, under) && inactive_node(i - 1, j, k, 0, 1, 0, under);
+        if (lx && lz) return inactive_node(i, j, k, 0, 0, 1, under) && inactive_node(i - 1, j, k, 0, 0, 1, under);
+        if (ly && lz) return inactive_node(i, j, k, 0, 1, 0, under) && inactive_node(i, j, k - 1, 0, 1, 0, under);
+        if (lx) return inactive_cell(i, j, k, under) && inactive_cell(i - 1, j, k, under);
+        if (ly) return inactive_cell(i, j, k, under) && inactive_cell(i, j - 1, k, under);
+        if (lz) return inactive_cell(i, j, k, under) && inactive_cell(i, j, k - 1, under);
+        return inactive_cell(i, j, k, under);
+    }
+    bool peripheral_node(int i, int j, int k, int lx, int ly, int lz, bool under) const {
+        if (lx && ly && lz) return peripheral_node(i, j, k, 0, 1, 1, under) || peripheral_node(i - 1, j, k, 0, 1, 1, under);
+        if (lx && ly) return peripheral_node(i, j, k, 0, 1, 0, under) || peripheral_node(i - 1, j, k, 0, 1, 0, under);
+        if (lx && lz) return peripheral_node(i, j, k, 0, 0, 1, under) || peripheral_node(i - 1, j, k, 0, 0, 1, under);
+        if (ly && lz) return peripheral_node(i, j, k, 0, 1, 0, under) || peripheral_node(i, j, k - 1, 0, 1, 0, under);
+        if (lx) return inactive_cell(i, j, k, under) || inactive_cell(i - 1, j, k, under);
+        if (ly) return inactive_cell(i, j, k, under) || inactive_cell(i, j - 1, k, under);
+        if (lz) return inactive_cell(i, j, k, under) || inactive_cell(i, j, k - 1, under);
+        return inactive_cell(i, j, k, under);
+    }
+    bool imm_inactive(int i, int j, int k, int lx, int ly, int lz) const {
+        return m.ib && inactive_node(i, j, k, lx, ly, lz, false) && !inactive_node(i, j, k, lx, ly, lz, true);
+    }
+    bool imm_peripheral(int i, int j, int k, int lx, int ly, int lz) const {
+        return m.ib && peripheral_node(i, j, k, lx, ly, lz, false) && !peripheral_node(i, j, k, lx, ly, lz, true);
+    }
+    // near_{x,y,z}_immersed_boundary_{symmetric,biased}{ᶜ,ᶠ} for a scheme with `buffer`
+    // (immersed_advective_fluxes.jl:127-215): any inactive node among the 2·buffer stencil points;
+    // the node is a cell centre when the interpolation point is a face and a face when it is a centre.
+    bool near(int d, bool center_side, int buffer, int i, int j, int k) const {
+        for (int n = 1; n <= 2 * buffer; ++n) {
+            const int c = center_side ? n - buffer : n - buffer - 1;
+            const int l = center_side ? 1 : 0;
+            bool in;
+            if (d == 0) in = inactive_node(i + c, j, k, l, 0, 0, false);
+            else if (d == 1) in = inactive_node(i, j + c, k, 0, l, 0, false);
+            else in = inactive_node(i, j, k + c, 0, 0, l, false);
+            if (in) return true;
+        }
+        return false;
+    }
 };
 
 // ----------------------------------------------------------------------------------------
@@ -391,6 +459,29 @@ template <class T> struct Adv {
     bool fy() const { return m.g.topo[1] == F; }
     bool fz() const { return m.g.topo[2] == F; }
 
+    // Scheme of a reconstruction along d at a face (cs = false) or a centre (cs = true): on an immersed
+    // grid the buffer chain WENO5 → WENO3 → Centered(2) is driven by the inactive nodes in the stencil
+    // (this also reproduces the Bounded-z wall switching); otherwise `topo` (the wall switching) is used.
+    ZScheme bsch(int d, bool cs, int i, int j, int k, ZScheme topo) const {
+        if (!m.ib) return topo;
+        IBP<T> P(m);
+        if (!P.near(d, cs, 3, i, j, k)) return Z5;
+        if (!P.near(d, cs, 2, i, j, k)) return Z3;
+        return Z2;
+    }
+    ZScheme ssch(int d, bool cs, int i, int j, int k, ZScheme topo) const {
+        if (!m.ib) return topo;
+        return IBP<T>(m).near(d, cs, 3, i, j, k) ? Z2 : Z4;
+    }
+    // conditional_flux: zero on the immersed periphery of the flux location
+    bool zf(int i, int j, int k, int lx, int ly, int lz) const { return m.ib && IBP<T>(m).imm_peripheral(i, j, k, lx, ly, lz); }
+    // conditional δ: zero when either differenced point is an immersed inactive node
+    bool zd(int i1, int j1, int k1, int i2, int j2, int k2, int lx, int ly, int lz) const {
+        if (!m.ib) return false;
+        IBP<T> P(m);
+        return P.imm_inactive(i1, j1, k1, lx, ly, lz) || P.imm_inactive(i2, j2, k2, lx, ly, lz);
+    }
+
     // ---- u momentum fluxes
     T F_Uu(int i, int j, int k) {  // ccc
         if (fx()) return 0;
@@ -399,9 +490,10 @@ template <class T> struct Adv {
             T Ui = T(0.5) * (U(i, j, k) + U(i + 1, j, k));
             return m.Ax_c(k) * Ui * Ui;
         }
+        if (zf(i, j, k, 0, 0, 0)) return 0;
         T Axk = m.Ax_c(k);
-        T ut = rec_symmetric(m.K, Z4, [&](int o) { return Axk * U(i + 1 + o, j, k); });
-        T ur = rec_biased(m.K, Z5, bias_left(ut), [&](int o) { return U(i + 1 + o, j, k); });
+        T ut = rec_symmetric(m.K, ssch(0, true, i, j, k, Z4), [&](int o) { return Axk * U(i + 1 + o, j, k); });
+        T ur = rec_biased(m.K, bsch(0, true, i, j, k, Z5), bias_left(ut), [&](int o) { return U(i + 1 + o, j, k); });
         return ut * ur;
     }
     T F_Vu(int i, int j, int k) {  // ffc
@@ -412,9 +504,10 @@ template <class T> struct Adv {
             T uj = T(0.5) * (U(i, j - 1, k) + U(i, j, k));
             return m.Ay_c(k) * vi * uj;
         }
+        if (zf(i, j, k, 1, 1, 0)) return 0;
         T Ayk = m.Ay_c(k);
-        T vt = fx() ? Ayk * Vv(i, j, k) : rec_symmetric(m.K, Z4, [&](int o) { return Ayk * Vv(i + o, j, k); });
-        T ur = rec_biased(m.K, Z5, bias_left(vt), [&](int o) { return U(i, j + o, k); });
+        T vt = fx() ? Ayk * Vv(i, j, k) : rec_symmetric(m.K, ssch(0, false, i, j, k, Z4), [&](int o) { return Ayk * Vv(i + o, j, k); });
+        T ur = rec_biased(m.K, bsch(1, false, i, j, k, Z5), bias_left(vt), [&](int o) { return U(i, j + o, k); });
         return vt * ur;
     }
     T F_Wu(int i, int j, int k) {  // fcf
@@ -425,9 +518,10 @@ template <class T> struct Adv {
             T uk = T(0.5) * (U(i, j, k - 1) + U(i, j, k));
             return m.Az() * wi * uk;
         }
+        if (zf(i, j, k, 1, 0, 1)) return 0;
         T Az = m.Az();
-        T wt = fx() ? Az * W(i, j, k) : rec_symmetric(m.K, Z4, [&](int o) { return Az * W(i + o, j, k); });
-        T ur = rec_biased(m.K, zb_face(m, k), bias_left(wt), [&](int o) { return U(i, j, k + o); });
+        T wt = fx() ? Az * W(i, j, k) : rec_symmetric(m.K, ssch(0, false, i, j, k, Z4), [&](int o) { return Az * W(i + o, j, k); });
+        T ur = rec_biased(m.K, bsch(2, false, i, j, k, zb_face(m, k)), bias_left(wt), [&](int o) { return U(i, j, k + o); });
         return wt * ur;
     }
     // ---- v momentum fluxes
@@ -439,9 +533,10 @@ template <class T> struct Adv {
             T vi = T(0.5) * (Vv(i - 1, j, k) + Vv(i, j, k));
             return m.Ax_c(k) * uj * vi;
         }
+        if (zf(i, j, k, 1, 1, 0)) return 0;
         T Axk = m.Ax_c(k);
-        T ut = fy() ? Axk * U(i, j, k) : rec_symmetric(m.K, Z4, [&](int o) { return Axk * U(i, j + o, k); });
-        T vr = rec_biased(m.K, Z5, bias_left(ut), [&](int o) { return Vv(i + o, j, k); });
+        T ut = fy() ? Axk * U(i, j, k) : rec_symmetric(m.K, ssch(1, false, i, j, k, Z4), [&](int o) { return Axk * U(i, j + o, k); });
+        T vr = rec_biased(m.K, bsch(0, false, i, j, k, Z5), bias_left(ut), [&](int o) { return Vv(i + o, j, k); });
         return ut * vr;
     }
     T F_Vv(int i, int j, int k) {  // ccc
@@ -451,9 +546,10 @@ template <class T> struct Adv {
             T Vj = T(0.5) * (Vv(i, j, k) + Vv(i, j + 1, k));
             return m.Ay_c(k) * Vj * Vj;
         }
+        if (zf(i, j, k, 0, 0, 0)) return 0;
         T Ayk = m.Ay_c(k);
-        T vt = rec_symmetric(m.K, Z4, [&](int o) { return Ayk * Vv(i, j + 1 + o, k); });
-        T vr = rec_biased(m.K, Z5, bias_left(vt), [&](int o) { return Vv(i, j + 1 + o, k); });
+        T vt = rec_symmetric(m.K, ssch(1, true, i, j, k, Z4), [&](int o) { return Ayk * Vv(i, j + 1 + o, k); });
+        T vr = rec_biased(m.K, bsch(1, true, i, j, k, Z5), bias_left(vt), [&](int o) { return Vv(i, j + 1 + o, k); });
         return vt * vr;
     }
     T F_Wv(int i, int j, int k) {  // cff
@@ -464,9 +560,10 @@ template <class T> struct Adv {
             T vk = T(0.5) * (Vv(i, j, k - 1) + Vv(i, j, k));
             return m.Az() * wj * vk;
         }
+        if (zf(i, j, k, 0, 1, 1)) return 0;
         T Az = m.Az();
-        T wt = fy() ? Az * W(i, j, k) : rec_symmetric(m.K, Z4, [&](int o) { return Az * W(i, j + o, k); });
-        T vr = rec_biased(m.K, zb_face(m, k), bias_left(wt), [&](int o) { return Vv(i, j, k + o); });
+        T wt = fy() ? Az * W(i, j, k) : rec_symmetric(m.K, ssch(1, false, i, j, k, Z4), [&](int o) { return Az * W(i, j + o, k); });
+        T vr = rec_biased(m.K, bsch(2, false, i, j, k, zb_face(m, k)), bias_left(wt), [&](int o) { return Vv(i, j, k + o); });
         return wt * vr;
     }
     // ---- w momentum fluxes
@@ -478,9 +575,10 @@ template <class T> struct Adv {
             T wi = T(0.5) * (W(i - 1, j, k) + W(i, j, k));
             return m.Ax_f(k) * uk * wi;
         }
+        if (zf(i, j, k, 1, 0, 1)) return 0;
         T ut = fz() ? m.Ax_c(k) * U(i, j, k)
-                    : rec_symmetric(m.K, zs_face(m, k), [&](int o) { return m.Ax_c(k + o) * U(i, j, k + o); });
-        T wr = rec_biased(m.K, Z5, bias_left(ut), [&](int o) { return W(i + o, j, k); });
+                    : rec_symmetric(m.K, ssch(2, false, i, j, k, zs_face(m, k)), [&](int o) { return m.Ax_c(k + o) * U(i, j, k + o); });
+        T wr = rec_biased(m.K, bsch(0, false, i, j, k, Z5), bias_left(ut), [&](int o) { return W(i + o, j, k); });
         return ut * wr;
     }
     T F_Vw(int i, int j, int k) {  // cff
@@ -491,9 +589,10 @@ template <class T> struct Adv {
             T wj = T(0.5) * (W(i, j - 1, k) + W(i, j, k));
             return m.Ay_f(k) * vk * wj;
         }
+        if (zf(i, j, k, 0, 1, 1)) return 0;
         T vt = fz() ? m.Ay_c(k) * Vv(i, j, k)
-                    : rec_symmetric(m.K, zs_face(m, k), [&](int o) { return m.Ay_c(k + o) * Vv(i, j, k + o); });
-        T wr = rec_biased(m.K, Z5, bias_left(vt), [&](int o) { return W(i, j + o, k); });
+                    : rec_symmetric(m.K, ssch(2, false, i, j, k, zs_face(m, k)), [&](int o) { return m.Ay_c(k + o) * Vv(i, j, k + o); });
+        T wr = rec_biased(m.K, bsch(1, false, i, j, k, Z5), bias_left(vt), [&](int o) { return W(i, j + o, k); });
         return vt * wr;
     }
     T F_Ww(int i, int j, int k) {  // ccc
@@ -503,9 +602,10 @@ template <class T> struct Adv {
             T Wk = T(0.5) * (W(i, j, k) + W(i, j, k + 1));
             return m.Az() * Wk * Wk;
         }
+        if (zf(i, j, k, 0, 0, 0)) return 0;
         T Az = m.Az();
-        T wt = rec_symmetric(m.K, zs_center(m, k), [&](int o) { return Az * W(i, j, k + 1 + o); });
-        T wr = rec_biased(m.K, zb_center(m, k), bias_left(wt), [&](int o) { return W(i, j, k + 1 + o); });
+        T wt = rec_symmetric(m.K, ssch(2, true, i, j, k, zs_center(m, k)), [&](int o) { return Az * W(i, j, k + 1 + o); });
+        T wr = rec_biased(m.K, bsch(2, true, i, j, k, zb_center(m, k)), bias_left(wt), [&](int o) { return W(i, j, k + 1 + o); });
         return wt * wr;
     }
     // ---- tracer fluxes (upwind: advective_tracer_flux_x = Ax·u·c_face; centered: Ax·u·ℑc)
@@ -513,39 +613,51 @@ template <class T> struct Adv {
         if (fx()) return 0;
         T ui = u()(i, j, k);
         if (!weno()) return (m.Ax_c(k) * ui) * (T(0.5) * (c(i - 1, j, k) + c(i, j, k)));
-        T cr = rec_biased(m.K, Z5, bias_left(ui), [&](int o) { return c(i + o, j, k); });
+        if (zf(i, j, k, 1, 0, 0)) return 0;
+        T cr = rec_biased(m.K, bsch(0, false, i, j, k, Z5), bias_left(ui), [&](int o) { return c(i + o, j, k); });
         return m.Ax_c(k) * ui * cr;
     }
     T F_cy(const Fld<T> &c, int i, int j, int k) {
         if (fy()) return 0;
         T vj = v()(i, j, k);
         if (!weno()) return (m.Ay_c(k) * vj) * (T(0.5) * (c(i, j - 1, k) + c(i, j, k)));
-        T cr = rec_biased(m.K, Z5, bias_left(vj), [&](int o) { return c(i, j + o, k); });
+        if (zf(i, j, k, 0, 1, 0)) return 0;
+        T cr = rec_biased(m.K, bsch(1, false, i, j, k, Z5), bias_left(vj), [&](int o) { return c(i, j + o, k); });
         return m.Ay_c(k) * vj * cr;
     }
     T F_cz(const Fld<T> &c, int i, int j, int k) {
         if (fz()) return 0;
         T wk = w()(i, j, k);
         if (!weno()) return (m.Az() * wk) * (T(0.5) * (c(i, j, k - 1) + c(i, j, k)));
-        T cr = rec_biased(m.K, zb_face(m, k), bias_left(wk), [&](int o) { return c(i, j, k + o); });
+        if (zf(i, j, k, 0, 0, 1)) return 0;
+        T cr = rec_biased(m.K, bsch(2, false, i, j, k, zb_face(m, k)), bias_left(wk), [&](int o) { return c(i, j, k + o); });
         return m.Az() * wk * cr;
     }
 
+    // the differences are conditional δ's on an immersed grid (conditional_differences.jl:15-44)
     T div_Uu(int i, int j, int k) {
-        return (T(1) / m.V_c(k)) * ((F_Uu(i, j, k) - F_Uu(i - 1, j, k)) + (F_Vu(i, j + 1, k) - F_Vu(i, j, k)) +
-                                    (F_Wu(i, j, k + 1) - F_Wu(i, j, k)));
+        T sx = zd(i, j, k, i - 1, j, k, 0, 0, 0) ? T(0) : F_Uu(i, j, k) - F_Uu(i - 1, j, k);
+        T sy = zd(i, j + 1, k, i, j, k, 1, 1, 0) ? T(0) : F_Vu(i, j + 1, k) - F_Vu(i, j, k);
+        T sz = zd(i, j, k + 1, i, j, k, 1, 0, 1) ? T(0) : F_Wu(i, j, k + 1) - F_Wu(i, j, k);
+        return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
     T div_Uv(int i, int j, int k) {
-        return (T(1) / m.V_c(k)) * ((F_Uv(i + 1, j, k) - F_Uv(i, j, k)) + (F_Vv(i, j, k) - F_Vv(i, j - 1, k)) +
-                                    (F_Wv(i, j, k + 1) - F_Wv(i, j, k)));
+        T sx = zd(i + 1, j, k, i, j, k, 1, 1, 0) ? T(0) : F_Uv(i + 1, j, k) - F_Uv(i, j, k);
+        T sy = zd(i, j, k, i, j - 1, k, 0, 0, 0) ? T(0) : F_Vv(i, j, k) - F_Vv(i, j - 1, k);
+        T sz = zd(i, j, k + 1, i, j, k, 0, 1, 1) ? T(0) : F_Wv(i, j, k + 1) - F_Wv(i, j, k);
+        return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
     T div_Uw(int i, int j, int k) {
-        return (T(1) / m.V_f(k)) * ((F_Uw(i + 1, j, k) - F_Uw(i, j, k)) + (F_Vw(i, j + 1, k) - F_Vw(i, j, k)) +
-                                    (F_Ww(i, j, k) - F_Ww(i, j, k - 1)));
+        T sx = zd(i + 1, j, k, i, j, k, 1, 0, 1) ? T(0) : F_Uw(i + 1, j, k) - F_Uw(i, j, k);
+        T sy = zd(i, j + 1, k, i, j, k, 0, 1, 1) ? T(0) : F_Vw(i, j + 1, k) - F_Vw(i, j, k);
+        T sz = zd(i, j, k, i, j, k - 1, 0, 0, 0) ? T(0) : F_Ww(i, j, k) - F_Ww(i, j, k - 1);
+        return (T(1) / m.V_f(k)) * ((sx + sy) + sz);
     }
     T div_Uc(const Fld<T> &c, int i, int j, int k) {
-        return (T(1) / m.V_c(k)) * ((F_cx(c, i + 1, j, k) - F_cx(c, i, j, k)) + (F_cy(c, i, j + 1, k) - F_cy(c, i, j, k)) +
-                                    (F_cz(c, i, j, k + 1) - F_cz(c, i, j, k)));
+        T sx = zd(i + 1, j, k, i, j, k, 1, 0, 0) ? T(0) : F_cx(c, i + 1, j, k) - F_cx(c, i, j, k);
+        T sy = zd(i, j + 1, k, i, j, k, 0, 1, 0) ? T(0) : F_cy(c, i, j + 1, k) - F_cy(c, i, j, k);
+        T sz = zd(i, j, k + 1, i, j, k, 0, 0, 1) ? T(0) : F_cz(c, i, j, k + 1) - F_cz(c, i, j, k);
+        return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
 };
 
@@ -563,19 +675,27 @@ template <class T> struct Clo {
     bool fz() const { return m.g.topo[2] == F; }
     bool fx() const { return m.g.topo[0] == F; }
     bool fy() const { return m.g.topo[1] == F; }
+    // conditional δ (conditional_differences.jl:15-44): zero when either differenced point is an
+    // immersed inactive node at the location of the differenced quantity
+    bool zd(int i1, int j1, int k1, int i2, int j2, int k2, int lx, int ly, int lz) const {
+        if (!m.ib) return false;
+        IBP<T> P(m);
+        return P.imm_inactive(i1, j1, k1, lx, ly, lz) || P.imm_inactive(i2, j2, k2, lx, ly, lz);
+    }
+    bool zf(int i, int j, int k, int lx, int ly, int lz) const { return m.ib && IBP<T>(m).imm_peripheral(i, j, k, lx, ly, lz); }
     // derivatives (velocity_tracer_gradients.jl:6-18); Flat directions give zero differences
-    T dx_u(int i, int j, int k) const { return fx() ? T(0) : (u()(i + 1, j, k) - u()(i, j, k)) * idx(); }
-    T dy_v(int i, int j, int k) const { return fy() ? T(0) : (v()(i, j + 1, k) - v()(i, j, k)) * idy(); }
-    T dz_w(int i, int j, int k) const { return fz() ? T(0) : (w()(i, j, k + 1) - w()(i, j, k)) * (T(1) / m.g.dzC(k)); }
-    T dx_v(int i, int j, int k) const { return fx() ? T(0) : (v()(i, j, k) - v()(i - 1, j, k)) * idx(); }
-    T dx_w(int i, int j, int k) const { return fx() ? T(0) : (w()(i, j, k) - w()(i - 1, j, k)) * idx(); }
-    T dy_u(int i, int j, int k) const { return fy() ? T(0) : (u()(i, j, k) - u()(i, j - 1, k)) * idy(); }
-    T dy_w(int i, int j, int k) const { return fy() ? T(0) : (w()(i, j, k) - w()(i, j - 1, k)) * idy(); }
-    T dz_u(int i, int j, int k) const { return fz() ? T(0) : (u()(i, j, k) - u()(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
-    T dz_v(int i, int j, int k) const { return fz() ? T(0) : (v()(i, j, k) - v()(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
-    T dxc(const Fld<T> &c, int i, int j, int k) const { return fx() ? T(0) : (c(i, j, k) - c(i - 1, j, k)) * idx(); }
-    T dyc(const Fld<T> &c, int i, int j, int k) const { return fy() ? T(0) : (c(i, j, k) - c(i, j - 1, k)) * idy(); }
-    T dzc(const Fld<T> &c, int i, int j, int k) const { return fz() ? T(0) : (c(i, j, k) - c(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
+    T dx_u(int i, int j, int k) const { return fx() || zd(i, j, k, i + 1, j, k, 1, 0, 0) ? T(0) : (u()(i + 1, j, k) - u()(i, j, k)) * idx(); }
+    T dy_v(int i, int j, int k) const { return fy() || zd(i, j, k, i, j + 1, k, 0, 1, 0) ? T(0) : (v()(i, j + 1, k) - v()(i, j, k)) * idy(); }
+    T dz_w(int i, int j, int k) const { return fz() || zd(i, j, k, i, j, k + 1, 0, 0, 1) ? T(0) : (w()(i, j, k + 1) - w()(i, j, k)) * (T(1) / m.g.dzC(k)); }
+    T dx_v(int i, int j, int k) const { return fx() || zd(i, j, k, i - 1, j, k, 0, 1, 0) ? T(0) : (v()(i, j, k) - v()(i - 1, j, k)) * idx(); }
+    T dx_w(int i, int j, int k) const { return fx() || zd(i, j, k, i - 1, j, k, 0, 0, 1) ? T(0) : (w()(i, j, k) - w()(i - 1, j, k)) * idx(); }
+    T dy_u(int i, int j, int k) const { return fy() || zd(i, j, k, i, j - 1, k, 1, 0, 0) ? T(0) : (u()(i, j, k) - u()(i, j - 1, k)) * idy(); }
+    T dy_w(int i, int j, int k) const { return fy() || zd(i, j, k, i, j - 1, k, 0, 0, 1) ? T(0) : (w()(i, j, k) - w()(i, j - 1, k)) * idy(); }
+    T dz_u(int i, int j, int k) const { return fz() || zd(i, j, k, i, j, k - 1, 1, 0, 0) ? T(0) : (u()(i, j, k) - u()(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
+    T dz_v(int i, int j, int k) const { return fz() || zd(i, j, k, i, j, k - 1, 0, 1, 0) ? T(0) : (v()(i, j, k) - v()(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
+    T dxc(const Fld<T> &c, int i, int j, int k) const { return fx() || zd(i, j, k, i - 1, j, k, 0, 0, 0) ? T(0) : (c(i, j, k) - c(i - 1, j, k)) * idx(); }
+    T dyc(const Fld<T> &c, int i, int j, int k) const { return fy() || zd(i, j, k, i, j - 1, k, 0, 0, 0) ? T(0) : (c(i, j, k) - c(i, j - 1, k)) * idy(); }
+    T dzc(const Fld<T> &c, int i, int j, int k) const { return fz() || zd(i, j, k, i, j, k - 1, 0, 0, 0) ? T(0) : (c(i, j, k) - c(i, j, k - 1)) * (T(1) / m.g.dzF(k)); }
     // strain (Σ₁₂ ffc, Σ₁₃ fcf, Σ₂₃ cff)
     T S12(int i, int j, int k) const { return T(0.5) * (dy_u(i, j, k) + dx_v(i, j, k)); }
     T S13(int i, int j, int k) const { return T(0.5) * (dz_u(i, j, k) + dx_w(i, j, k)); }
@@ -714,42 +834,43 @@ template <class T> struct Clo {
     T kap_cfc(int n, int i, int j, int k) const;
     T kap_ccf(int n, int i, int j, int k) const;
 
-    T f_ux(int i, int j, int k) const { return T(-2) * (nu_ccc(i, j, k) * dx_u(i, j, k)); }
-    T f_uy(int i, int j, int k) const { return T(-2) * (nu_ffc(i, j, k) * S12(i, j, k)); }
-    T f_uz(int i, int j, int k) const { return T(-2) * (nu_fcf(i, j, k) * S13(i, j, k)); }
-    T f_vx(int i, int j, int k) const { return T(-2) * (nu_ffc(i, j, k) * S12(i, j, k)); }
-    T f_vy(int i, int j, int k) const { return T(-2) * (nu_ccc(i, j, k) * dy_v(i, j, k)); }
-    T f_vz(int i, int j, int k) const { return T(-2) * (nu_cff(i, j, k) * S23(i, j, k)); }
-    T f_wx(int i, int j, int k) const { return T(-2) * (nu_fcf(i, j, k) * S13(i, j, k)); }
-    T f_wy(int i, int j, int k) const { return T(-2) * (nu_cff(i, j, k) * S23(i, j, k)); }
-    T f_wz(int i, int j, int k) const { return T(-2) * (nu_ccc(i, j, k) * dz_w(i, j, k)); }
+    // viscous fluxes; zero on the immersed periphery of their location (immersed_diffusive_fluxes.jl:25-42)
+    T f_ux(int i, int j, int k) const { return zf(i, j, k, 0, 0, 0) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dx_u(i, j, k)); }
+    T f_uy(int i, int j, int k) const { return zf(i, j, k, 1, 1, 0) ? T(0) : T(-2) * (nu_ffc(i, j, k) * S12(i, j, k)); }
+    T f_uz(int i, int j, int k) const { return zf(i, j, k, 1, 0, 1) ? T(0) : T(-2) * (nu_fcf(i, j, k) * S13(i, j, k)); }
+    T f_vx(int i, int j, int k) const { return zf(i, j, k, 1, 1, 0) ? T(0) : T(-2) * (nu_ffc(i, j, k) * S12(i, j, k)); }
+    T f_vy(int i, int j, int k) const { return zf(i, j, k, 0, 0, 0) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dy_v(i, j, k)); }
+    T f_vz(int i, int j, int k) const { return zf(i, j, k, 0, 1, 1) ? T(0) : T(-2) * (nu_cff(i, j, k) * S23(i, j, k)); }
+    T f_wx(int i, int j, int k) const { return zf(i, j, k, 1, 0, 1) ? T(0) : T(-2) * (nu_fcf(i, j, k) * S13(i, j, k)); }
+    T f_wy(int i, int j, int k) const { return zf(i, j, k, 0, 1, 1) ? T(0) : T(-2) * (nu_cff(i, j, k) * S23(i, j, k)); }
+    T f_wz(int i, int j, int k) const { return zf(i, j, k, 0, 0, 0) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dz_w(i, j, k)); }
 
     // closure_kernel_operators.jl:20-46
     T div_tau1(int i, int j, int k) const {
-        T sx = fx() ? T(0) : m.Ax_c(k) * f_ux(i, j, k) - m.Ax_c(k) * f_ux(i - 1, j, k);
-        T sy = fy() ? T(0) : m.Ay_c(k) * f_uy(i, j + 1, k) - m.Ay_c(k) * f_uy(i, j, k);
-        T sz = fz() ? T(0) : m.Az() * f_uz(i, j, k + 1) - m.Az() * f_uz(i, j, k);
+        T sx = fx() || zd(i, j, k, i - 1, j, k, 0, 0, 0) ? T(0) : m.Ax_c(k) * f_ux(i, j, k) - m.Ax_c(k) * f_ux(i - 1, j, k);
+        T sy = fy() || zd(i, j + 1, k, i, j, k, 1, 1, 0) ? T(0) : m.Ay_c(k) * f_uy(i, j + 1, k) - m.Ay_c(k) * f_uy(i, j, k);
+        T sz = fz() || zd(i, j, k + 1, i, j, k, 1, 0, 1) ? T(0) : m.Az() * f_uz(i, j, k + 1) - m.Az() * f_uz(i, j, k);
         return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
     T div_tau2(int i, int j, int k) const {
-        T sx = fx() ? T(0) : m.Ax_c(k) * f_vx(i + 1, j, k) - m.Ax_c(k) * f_vx(i, j, k);
-        T sy = fy() ? T(0) : m.Ay_c(k) * f_vy(i, j, k) - m.Ay_c(k) * f_vy(i, j - 1, k);
-        T sz = fz() ? T(0) : m.Az() * f_vz(i, j, k + 1) - m.Az() * f_vz(i, j, k);
+        T sx = fx() || zd(i + 1, j, k, i, j, k, 1, 1, 0) ? T(0) : m.Ax_c(k) * f_vx(i + 1, j, k) - m.Ax_c(k) * f_vx(i, j, k);
+        T sy = fy() || zd(i, j, k, i, j - 1, k, 0, 0, 0) ? T(0) : m.Ay_c(k) * f_vy(i, j, k) - m.Ay_c(k) * f_vy(i, j - 1, k);
+        T sz = fz() || zd(i, j, k + 1, i, j, k, 0, 1, 1) ? T(0) : m.Az() * f_vz(i, j, k + 1) - m.Az() * f_vz(i, j, k);
         return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
     T div_tau3(int i, int j, int k) const {
-        T sx = fx() ? T(0) : m.Ax_f(k) * f_wx(i + 1, j, k) - m.Ax_f(k) * f_wx(i, j, k);
-        T sy = fy() ? T(0) : m.Ay_f(k) * f_wy(i, j + 1, k) - m.Ay_f(k) * f_wy(i, j, k);
-        T sz = fz() ? T(0) : m.Az() * f_wz(i, j, k) - m.Az() * f_wz(i, j, k - 1);
+        T sx = fx() || zd(i + 1, j, k, i, j, k, 1, 0, 1) ? T(0) : m.Ax_f(k) * f_wx(i + 1, j, k) - m.Ax_f(k) * f_wx(i, j, k);
+        T sy = fy() || zd(i, j + 1, k, i, j, k, 0, 1, 1) ? T(0) : m.Ay_f(k) * f_wy(i, j + 1, k) - m.Ay_f(k) * f_wy(i, j, k);
+        T sz = fz() || zd(i, j, k, i, j, k - 1, 0, 0, 0) ? T(0) : m.Az() * f_wz(i, j, k) - m.Az() * f_wz(i, j, k - 1);
         return (T(1) / m.V_f(k)) * ((sx + sy) + sz);
     }
     T div_q(int n, const Fld<T> &c, int i, int j, int k) const {
-        auto qx = [&](int a) { return -kap_fcc(n, a, j, k) * dxc(c, a, j, k); };
-        auto qy = [&](int b) { return -kap_cfc(n, i, b, k) * dyc(c, i, b, k); };
-        auto qz = [&](int cc) { return -kap_ccf(n, i, j, cc) * dzc(c, i, j, cc); };
-        T sx = fx() ? T(0) : m.Ax_c(k) * qx(i + 1) - m.Ax_c(k) * qx(i);
-        T sy = fy() ? T(0) : m.Ay_c(k) * qy(j + 1) - m.Ay_c(k) * qy(j);
-        T sz = fz() ? T(0) : m.Az() * qz(k + 1) - m.Az() * qz(k);
+        auto qx = [&](int a) { return zf(a, j, k, 1, 0, 0) ? T(0) : -kap_fcc(n, a, j, k) * dxc(c, a, j, k); };
+        auto qy = [&](int b) { return zf(i, b, k, 0, 1, 0) ? T(0) : -kap_cfc(n, i, b, k) * dyc(c, i, b, k); };
+        auto qz = [&](int cc) { return zf(i, j, cc, 0, 0, 1) ? T(0) : -kap_ccf(n, i, j, cc) * dzc(c, i, j, cc); };
+        T sx = fx() || zd(i + 1, j, k, i, j, k, 1, 0, 0) ? T(0) : m.Ax_c(k) * qx(i + 1) - m.Ax_c(k) * qx(i);
+        T sy = fy() || zd(i, j + 1, k, i, j, k, 0, 1, 0) ? T(0) : m.Ay_c(k) * qy(j + 1) - m.Ay_c(k) * qy(j);
+        T sz = fz() || zd(i, j, k + 1, i, j, k, 0, 0, 1) ? T(0) : m.Az() * qz(k + 1) - m.Az() * qz(k);
         return (T(1) / m.V_c(k)) * ((sx + sy) + sz);
     }
 };
@@ -871,7 +992,8 @@ template <class T> static void compute_tendencies(Model<T> &m) {
             if (adv) G -= A.div_Uu(i, j, k);
             if (cor) {
                 // - x_f_cross_U = + f ℑxyᶠᶜᵃ(Ay v) / Ayᶠᶜᶜ   (coriolis_schemes.jl:67)
-                auto q = [&](int a, int b) { return m.Ay_c(k) * m.U[1](a, b, k); };
+                // masked_Ay_qᶜᶠᶜ: zero on the immersed periphery (coriolis_schemes.jl:56-59)
+                auto q = [&](int a, int b) { return IBP<T>(m).imm_peripheral(a, b, k, 0, 1, 0) ? T(0) : m.Ay_c(k) * m.U[1](a, b, k); };
                 T vv = m.g.topo[0] == F ? q(i, j) + q(i, j + 1) : T(0);
                 T Ixf_j = m.g.topo[0] == F ? q(i, j) : T(0.5) * (q(i - 1, j) + q(i, j));
                 T Ixf_j1 = m.g.topo[0] == F ? q(i, j + 1) : T(0.5) * (q(i - 1, j + 1) + q(i, j + 1));
@@ -880,7 +1002,7 @@ template <class T> static void compute_tendencies(Model<T> &m) {
                 T fI = T(0.5) * (f + f);
                 G -= -fI * Ixy * (T(1) / m.Ay_c(k));
             }
-            if (m.has_pHY) G -= (m.pHY(i, j, k) - m.pHY(i - 1, j, k)) * (T(1) / m.g.dx);
+            if (m.has_pHY) G -= (C.zd(i, j, k, i - 1, j, k, 0, 0, 0) ? T(0) : m.pHY(i, j, k) - m.pHY(i - 1, j, k)) * (T(1) / m.g.dx);
             if (clo) G -= C.div_tau1(i, j, k);
             m.Gn[0](i, j, k) = G;
         }
@@ -890,14 +1012,14 @@ template <class T> static void compute_tendencies(Model<T> &m) {
             if (adv) G -= A.div_Uv(i, j, k);
             if (cor) {
                 // - y_f_cross_U = - f ℑxyᶜᶠᵃ(Ax u) / Axᶜᶠᶜ   (coriolis_schemes.jl:68)
-                auto q = [&](int a, int b) { return m.Ax_c(k) * m.U[0](a, b, k); };
+                auto q = [&](int a, int b) { return IBP<T>(m).imm_peripheral(a, b, k, 1, 0, 0) ? T(0) : m.Ax_c(k) * m.U[0](a, b, k); };
                 T Ixc_jm = m.g.topo[0] == F ? q(i, j - 1) : T(0.5) * (q(i, j - 1) + q(i + 1, j - 1));
                 T Ixc_j = m.g.topo[0] == F ? q(i, j) : T(0.5) * (q(i, j) + q(i + 1, j));
                 T Ixy = m.g.topo[1] == F ? Ixc_j : T(0.5) * (Ixc_jm + Ixc_j);
                 T fI = T(0.5) * (f + f);
                 G -= fI * Ixy * (T(1) / m.Ax_c(k));
             }
-            if (m.has_pHY) G -= (m.pHY(i, j, k) - m.pHY(i, j - 1, k)) * (T(1) / m.g.dy);
+            if (m.has_pHY) G -= (C.zd(i, j, k, i, j - 1, k, 0, 0, 0) ? T(0) : m.pHY(i, j, k) - m.pHY(i, j - 1, k)) * (T(1) / m.g.dy);
             if (clo) G -= C.div_tau2(i, j, k);
             m.Gn[1](i, j, k) = G;
         }
@@ -1102,9 +1224,13 @@ template <class T> static void fourier_tridiagonal_poisson(const Model<T> &m, st
 
 // rhs = divᶜᶜᶜ(U) (solve_for_pressure.jl:12-18 / :36-42 scaled by Δzᶜ for Fourier-tridiagonal)
 template <class T> static T divergence(const Model<T> &m, int i, int j, int k) {
-    T dxs = m.g.topo[0] == F ? T(0) : m.Ax_c(k) * m.U[0](i + 1, j, k) - m.Ax_c(k) * m.U[0](i, j, k);
-    T dys = m.g.topo[1] == F ? T(0) : m.Ay_c(k) * m.U[1](i, j + 1, k) - m.Ay_c(k) * m.U[1](i, j, k);
-    T dzs = m.g.topo[2] == F ? T(0) : m.Az() * m.U[2](i, j, k + 1) - m.Az() * m.U[2](i, j, k);
+    IBP<T> P(m);
+    auto zd = [&](int i1, int j1, int k1, int i2, int j2, int k2, int lx, int ly, int lz) {
+        return m.ib && (P.imm_inactive(i1, j1, k1, lx, ly, lz) || P.imm_inactive(i2, j2, k2, lx, ly, lz));
+    };
+    T dxs = m.g.topo[0] == F || zd(i, j, k, i + 1, j, k, 1, 0, 0) ? T(0) : m.Ax_c(k) * m.U[0](i + 1, j, k) - m.Ax_c(k) * m.U[0](i, j, k);
+    T dys = m.g.topo[1] == F || zd(i, j, k, i, j + 1, k, 0, 1, 0) ? T(0) : m.Ay_c(k) * m.U[1](i, j + 1, k) - m.Ay_c(k) * m.U[1](i, j, k);
+    T dzs = m.g.topo[2] == F || zd(i, j, k, i, j, k + 1, 0, 0, 1) ? T(0) : m.Az() * m.U[2](i, j, k + 1) - m.Az() * m.U[2](i, j, k);
     return (T(1) / m.V_c(k)) * ((dxs + dys) + dzs);
 }
 
@@ -1139,16 +1265,28 @@ template <class T> static void solve_pressure_field(Model<T> &m, const std::vect
 // ConjugateGradientPoissonSolver (conjugate_gradient_poisson_solver.jl:38-397) driving the
 // ConjugateGradientSolver (conjugate_gradient_solver.jl:177-284).  Solves V∇²x = b with b = V·rhs.
 // ----------------------------------------------------------------------------------------
-template <class T> static bool cg_active(const Model<T> &, int, int, int) { return true; }   // no immersed cells
+template <class T> static bool cg_active(const Model<T> &m, int i, int j, int k) { return !IBP<T>(m).inactive_cell(i, j, k, false); }
 
 // V∇²ᶜᶜᶜ (conjugate_gradient_poisson_solver.jl:62-66): δx(Ax ∂x c) + δy(Ay ∂y c) + δz(Az ∂z c)
+// On an immersed grid both the inner ∂ and the outer δ are conditional (conditional_differences.jl):
+// the gradient vanishes across a face next to an immersed cell, and so does the difference of the
+// face fluxes around a cell bordered by an immersed inactive face.
 template <class T> static T cg_laplacian(const Model<T> &m, const Fld<T> &c, int i, int j, int k) {
     const T idx = T(1) / m.g.dx, idy = T(1) / m.g.dy;
+    IBP<T> P(m);
+    auto zc = [&](int a, int b, int cc, int a2, int b2, int c2) {   // cell pair at (c, c, c)
+        return m.ib && (P.imm_inactive(a, b, cc, 0, 0, 0) || P.imm_inactive(a2, b2, c2, 0, 0, 0));
+    };
+    auto zfp = [&](int a, int b, int cc, int a2, int b2, int c2, int lx, int ly, int lz) {   // face pair
+        return m.ib && (P.imm_inactive(a, b, cc, lx, ly, lz) || P.imm_inactive(a2, b2, c2, lx, ly, lz));
+    };
+    auto gx = [&](int a) { return zc(a, j, k, a - 1, j, k) ? T(0) : (c(a, j, k) - c(a - 1, j, k)) * idx; };
+    auto gy = [&](int b) { return zc(i, b, k, i, b - 1, k) ? T(0) : (c(i, b, k) - c(i, b - 1, k)) * idy; };
+    auto gz = [&](int cc) { return zc(i, j, cc, i, j, cc - 1) ? T(0) : (c(i, j, cc) - c(i, j, cc - 1)) * (T(1) / m.g.dzF(cc)); };
     T tx = T(0), ty = T(0), tz = T(0);
-    if (m.g.topo[0] != F) tx = m.Ax_c(k) * ((c(i + 1, j, k) - c(i, j, k)) * idx) - m.Ax_c(k) * ((c(i, j, k) - c(i - 1, j, k)) * idx);
-    if (m.g.topo[1] != F) ty = m.Ay_c(k) * ((c(i, j + 1, k) - c(i, j, k)) * idy) - m.Ay_c(k) * ((c(i, j, k) - c(i, j - 1, k)) * idy);
-    if (m.g.topo[2] != F)
-        tz = m.Az() * ((c(i, j, k + 1) - c(i, j, k)) * (T(1) / m.g.dzF(k + 1))) - m.Az() * ((c(i, j, k) - c(i, j, k - 1)) * (T(1) / m.g.dzF(k)));
+    if (m.g.topo[0] != F && !zfp(i, j, k, i + 1, j, k, 1, 0, 0)) tx = m.Ax_c(k) * gx(i + 1) - m.Ax_c(k) * gx(i);
+    if (m.g.topo[1] != F && !zfp(i, j, k, i, j + 1, k, 0, 1, 0)) ty = m.Ay_c(k) * gy(j + 1) - m.Ay_c(k) * gy(j);
+    if (m.g.topo[2] != F && !zfp(i, j, k, i, j, k + 1, 0, 0, 1)) tz = m.Az() * gz(k + 1) - m.Az() * gz(k);
     return (tx + ty) + tz;
 }
 
@@ -1225,9 +1363,11 @@ template <class T> static void cg_precondition_fft(Model<T> &m, Fld<T> &z, Fld<T
         z(i, j, k) = (T)a[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))].real();
 }
 
+// the reference's default: the FFT-based solver of the underlying grid on an immersed grid, the
+// diagonally dominant preconditioner otherwise (conjugate_gradient_poisson_solver.jl:141-147)
 template <class T> static int cg_preconditioner_kind(const Model<T> &m) {
     int pk = m.cfg.cg_preconditioner;
-    if (pk == OM_PRECOND_DEFAULT) pk = OM_PRECOND_DIAGONALLY_DOMINANT;   // no immersed boundary here
+    if (pk == OM_PRECOND_DEFAULT) pk = m.ib ? OM_PRECOND_FFT : OM_PRECOND_DIAGONALLY_DOMINANT;
     return pk;
 }
 
@@ -1308,6 +1448,9 @@ template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
 
 // pressure_correction.jl:6-20
 template <class T> static void compute_pressure_correction(Model<T> &m, T dt) {
+    mask_immersed(m, m.U[0], 1, 0, 0);
+    mask_immersed(m, m.U[1], 0, 1, 0);
+    mask_immersed(m, m.U[2], 0, 0, 1);
     for (int f = 0; f < 3; ++f) fill_prognostic_halos(m, f, true);
     const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
     if (m.cfg.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
@@ -1334,19 +1477,32 @@ template <class T> static void compute_pressure_correction(Model<T> &m, T dt) {
 template <class T> static void make_pressure_correction(Model<T> &m, T dt) {
     const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
     Fld<T> &p = m.pNHS;
+    IBP<T> P(m);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
-        if (m.g.topo[0] != F) m.U[0](i, j, k) -= (p(i, j, k) - p(i - 1, j, k)) * (T(1) / m.g.dx);
-        if (m.g.topo[1] != F) m.U[1](i, j, k) -= (p(i, j, k) - p(i, j - 1, k)) * (T(1) / m.g.dy);
-        if (m.g.topo[2] != F) m.U[2](i, j, k) -= (p(i, j, k) - p(i, j, k - 1)) * (T(1) / m.g.dzF(k));
+        auto zd = [&](int i2, int j2, int k2) { return m.ib && (P.imm_inactive(i, j, k, 0, 0, 0) || P.imm_inactive(i2, j2, k2, 0, 0, 0)); };
+        if (m.g.topo[0] != F) m.U[0](i, j, k) -= (zd(i - 1, j, k) ? T(0) : p(i, j, k) - p(i - 1, j, k)) * (T(1) / m.g.dx);
+        if (m.g.topo[1] != F) m.U[1](i, j, k) -= (zd(i, j - 1, k) ? T(0) : p(i, j, k) - p(i, j - 1, k)) * (T(1) / m.g.dy);
+        if (m.g.topo[2] != F) m.U[2](i, j, k) -= (zd(i, j, k - 1) ? T(0) : p(i, j, k) - p(i, j, k - 1)) * (T(1) / m.g.dzF(k));
     }
     T eps = std::numeric_limits<T>::epsilon();
     T dtp = std::max(eps, dt);
     for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) p(i, j, k) /= dtp;
 }
 
+// mask_immersed_field! (mask_immersed_field.jl:65-80): zero on the immersed periphery of the field's
+// location, over the field's interior indices
+template <class T> static void mask_immersed(Model<T> &m, Fld<T> &c, int lx, int ly, int lz) {
+    if (!m.ib) return;
+    IBP<T> P(m);
+    const int khi = (lz && m.bz()) ? m.Nz() + 1 : m.Nz();
+    for (int k = 1; k <= khi; ++k) for (int j = 1; j <= m.Ny(); ++j) for (int i = 1; i <= m.Nx(); ++i)
+        if (P.imm_peripheral(i, j, k, lx, ly, lz)) c(i, j, k) = T(0);
+}
+
 // update_nonhydrostatic_model_state.jl:22-62
 template <class T> static void update_state(Model<T> &m) {
+    for (int n = 0; n < m.nt; ++n) mask_immersed(m, m.U[3 + n], 0, 0, 0);
     fill_all_prognostic_halos(m, false);
     compute_closure_fields(m);
     update_hydrostatic_pressure(m);
@@ -1443,6 +1599,19 @@ template <class T> static Model<T> *mk(const om_config_t *c) {
         m->cg_r.init(N, H); m->cg_p.init(N, H); m->cg_q.init(N, H); m->cg_z.init(N, H); m->cg_b.init(N, H);
     }
     for (int n = 0; n < m->nt; ++n) m->kap[n].init(N, H);
+    if (c->bottom_height) {
+        // numerical bottom: the face above the last cell whose centre lies at or below z_b
+        // (grid_fitted_bottom.jl:118-139, CenterImmersedCondition)
+        m->ib = true;
+        m->kbot_.assign((size_t)N[0] * N[1], 0);
+        for (int j = 1; j <= N[1]; ++j) for (int i = 1; i <= N[0]; ++i) {
+            const double zb = c->bottom_height[(size_t)(i - 1) + (size_t)N[0] * (j - 1)];
+            int kb = 0;
+            using FT = decltype(m->g.dx);
+            for (int k = 1; k <= N[2]; ++k) if ((FT)m->g.zC(k) <= (FT)zb) kb = k;
+            m->kbot_[(size_t)(i - 1) + (size_t)N[0] * (j - 1)] = kb;
+        }
+    }
     int rule = c->hydrostatic_pressure;
     if (rule < 0) m->has_pHY = (c->topology[2] != P) && (c->topology[2] != F) && c->buoyancy != OM_BUOYANCY_NONE;
     else m->has_pHY = rule == 1;

@@ -30,6 +30,35 @@ def stretched_convection(N=(16, 16, 16), float_type=np.float64, seed=3, ratio=1.
     return workloads.stretched_convection(N=N, float_type=float_type, seed=seed, ratio=ratio, Lz=Lz)
 
 
+def seamount(N=(16, 16, 12), float_type=np.float64, stretched=True, seed=3, noise=1e-3, N2=1e-6):
+    """The immersed-seamount configuration at small size: stretched z (ratio 1.02, finest at the
+    surface), GridFittedBottom Gaussian seamount, WENO5 + AMD + BuoyancyTracer, uniform inflow
+    u = 0.1 plus seeded noise, CG pressure solver."""
+    Lx = Ly = 10e3
+    Lz = 1e3
+    Nz = N[2]
+    if stretched:
+        r = 1.02
+        d = [Lz * (r - 1) / (r ** Nz - 1) * r ** q for q in range(Nz)]     # finest at the surface
+        faces = [0.0]
+        for dz in d:
+            faces.append(faces[-1] - dz)
+        z = np.array(faces[::-1])
+        z = z - z[0] - Lz
+        z[-1] = 0.0
+    else:
+        z = (-Lz, 0.0)
+    under = om.RectilinearGrid(float_type=float_type, size=N, x=(0, Lx), y=(0, Ly), z=z)
+    hill = lambda x, y: -Lz + 600 * np.exp(-((x - Lx / 2) ** 2 + (y - Ly / 2) ** 2) / 1500 ** 2)
+    grid = om.ImmersedBoundaryGrid(under, om.GridFittedBottom(hill))
+    kw = dict(advection=om.WENO(), closure=om.AnisotropicMinimumDissipation(), buoyancy=om.BuoyancyTracer(),
+              tracers=("b",), pressure_solver=om.ConjugateGradientPoissonSolver(reltol=1e-13, abstol=0))
+    from oceananigans_mi355x.synthetic import uniform
+    shape_u = under.N
+    init = {"u": 0.1 + noise * uniform(seed, shape_u, -1, 1), "b": lambda x, y, z: N2 * z}
+    return grid, kw, init
+
+
 def taylor_green(N=16, float_type=np.float64):
     """Taylor–Green vortex with ν = 1 (test/test_dynamics.jl:214-259): Centered(2), ScalarDiffusivity."""
     grid = om.RectilinearGrid(float_type=float_type, size=(N, N, 2), extent=(1, 1, 1))
@@ -66,6 +95,8 @@ def oracle_for(grid, kw, init, oracle_float_type=np.float64, round_inputs_to=Non
                                 y=(grid.origin[1], grid.origin[1] + grid.extent[1]),
                                 z=grid.z_faces if grid.z_faces is not None else (grid.origin[2], grid.origin[2] + grid.extent[2]),
                                 topology=grid.topology, halo=tuple(h for h, t in zip(grid.H, grid.topology) if t is not om.Flat))
+        if isinstance(grid, om.ImmersedBoundaryGrid):
+            g2 = om.ImmersedBoundaryGrid(g2, grid.immersed_boundary)
     cfg, keep = om.build_config(g2, **kw)
     m = OracleModel(cfg, g2)
     m._keep = keep

@@ -553,3 +553,110 @@ def test_conjugate_gradient_matches_sparse_direct_solve(gname, precond):
     assert true_res <= 1e-12 * np.linalg.norm(rf)
     if precond == "fft":
         assert it == 1
+
+
+# ---------------------------------------------------------------------------------------------
+# immersed bottom (GridFittedBottom)
+# ---------------------------------------------------------------------------------------------
+def _seamount_grid(N=(12, 10, 8), stretched=True):
+    from cases import seamount
+    grid, kw, init = seamount(N=N, stretched=stretched)
+    return grid, kw, init
+
+
+def test_immersed_resting_stratified_fluid_stays_at_rest():
+    """b = N²z at rest over a grid-fitted seamount with the hydrostatic anomaly: the conditional
+    differences leave no spurious pressure-gradient force, so the state is exactly steady
+    (the immersed counterpart of test_dynamics.jl:261-352)."""
+    grid, kw, _ = _seamount_grid()
+    orc = oracle_for(grid, kw, {"b": lambda x, y, z: 1e-6 * z})
+    b0 = orc.get(3)
+    orc.time_step(30.0, 3)
+    for q in range(3):
+        assert np.max(np.abs(orc.get(q))) == 0
+    np.testing.assert_array_equal(orc.get(3), b0)
+
+
+def _immersed_cells(grid, o):
+    """Boolean (Nx, Ny, Nz) immersed mask from the numerical bottom rule (zᶜ ≤ z_b)."""
+    dzc, dzf, zc, zf = o.grid_z()
+    H = grid.H[2]
+    zcent = zc[H:H + grid.Nz]
+    zb = grid.bottom_heights()
+    return zcent[None, None, :] <= zb[:, :, None]
+
+
+def test_immersed_cg_matches_sparse_direct_solve():
+    """CG on V∇² with the conditional (Neumann) faces next to immersed cells equals scipy's direct
+    solve of the same operator restricted to the active cells; immersed cells hold 0 (the gauge)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    grid, kw, _ = _seamount_grid()
+    cfg, keep = om.build_config(grid, advection=om.WENO(),
+                                pressure_solver=om.ConjugateGradientPoissonSolver(reltol=1e-14, abstol=0))
+    o = O.OracleModel(cfg, grid)
+    imm = _immersed_cells(grid, o)
+    assert imm.any() and not imm.all()
+    Nx, Ny, Nz = grid.N
+    dzc, dzf, zc, zf = o.grid_z()
+    H = grid.H[2]
+    dzc_i, dzf_i = dzc[H:H + Nz], dzf[H:H + Nz + 1]
+    dx, dy = grid.extent[0] / Nx, grid.extent[1] / Ny
+    act = ~imm
+    ids = -np.ones((Nx, Ny, Nz), dtype=int)
+    ids[act] = np.arange(act.sum())
+    rows, cols, vals = [], [], []
+    for k in range(Nz):
+        for j in range(Ny):
+            for i in range(Nx):
+                if not act[i, j, k]:
+                    continue
+                c = ids[i, j, k]
+                nb = [((i - 1) % Nx, j, k, dy * dzc_i[k] / dx), ((i + 1) % Nx, j, k, dy * dzc_i[k] / dx),
+                      (i, (j - 1) % Ny, k, dx * dzc_i[k] / dy), (i, (j + 1) % Ny, k, dx * dzc_i[k] / dy)]
+                if k > 0:
+                    nb.append((i, j, k - 1, dx * dy / dzf_i[k]))
+                if k < Nz - 1:
+                    nb.append((i, j, k + 1, dx * dy / dzf_i[k + 1]))
+                for a, b, cc, coef in nb:
+                    if act[a, b, cc]:
+                        rows += [c, c]; cols += [ids[a, b, cc], c]; vals += [coef, -coef]
+    n = int(act.sum())
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    V = dx * dy * np.broadcast_to(dzc_i[None, None, :], (Nx, Ny, Nz))
+    rhs = np.random.default_rng(12).normal(size=(Nx, Ny, Nz))
+    rhs[imm] = 0
+    rhs[act] -= (V[act] * rhs[act]).sum() / V[act].sum()
+    aug = sp.bmat([[A, np.ones((n, 1))], [np.ones((1, n)), None]]).tocsc()
+    x = spl.spsolve(aug, np.concatenate([V[act] * rhs[act], [0.0]]))[:n]
+    phi = o.poisson_solve(rhs)
+    it, res, true_res = o.last_solve_info()
+    assert np.all(phi[imm] == 0)
+    assert np.abs(phi[act] - phi[act].mean() - (x - x.mean())).max() <= 1e-11 * np.abs(x).max()
+    assert it >= 2   # the FFT preconditioner of the underlying grid is no longer exact
+
+
+def test_immersed_projection_and_tracer_conservation():
+    """After steps over the seamount: velocities vanish on the immersed periphery, the active cells
+    are divergence-free, and Σ V b over the fluid is conserved (no-flux immersed and wall faces)."""
+    grid, kw, init = _seamount_grid()
+    init = dict(init, b=lambda x, y, z: 1e-6 * z + 1e-8 * np.sin(x / 700.0) * np.cos(y / 900.0))
+    orc = oracle_for(grid, kw, init)
+    cfg_o = O.OracleModel(*(om.build_config(grid, **kw)[0], grid))
+    imm = _immersed_cells(grid, cfg_o)
+    dzc = cfg_o.grid_z()[0][grid.H[2]:grid.H[2] + grid.Nz]
+    V = np.broadcast_to(dzc[None, None, :], grid.N)
+    total0 = (V * orc.get(3))[~imm].sum()
+    orc.time_step(20.0, 3)
+    u, v, w = orc.get(0), orc.get(1), orc.get(2)
+    # faces between an immersed cell and its neighbour (x): u at face i touches cells i-1 and i
+    imm_u = imm | np.roll(imm, 1, axis=0)
+    imm_v = imm | np.roll(imm, 1, axis=1)
+    assert np.all(u[imm_u] == 0) and np.all(v[imm_v] == 0)
+    imm_w = np.zeros(w.shape, dtype=bool)
+    imm_w[:, :, 1:grid.Nz] = imm[:, :, 1:] | imm[:, :, :-1]
+    assert np.all(w[imm_w] == 0)
+    div = orc.divergence()
+    assert np.abs(div[~imm]).max() <= 1e-10 * np.abs(u).max() / (grid.extent[0] / grid.Nx)
+    total = (V * orc.get(3))[~imm].sum()
+    assert abs(total - total0) <= 1e-12 * abs(total0)
