@@ -95,6 +95,8 @@ extern "C" {
 #define OM_PRECOND_NONE                 1
 #define OM_PRECOND_DIAGONALLY_DOMINANT  2
 #define OM_PRECOND_FFT                  3  /* FFT-based solver of the underlying grid (Fourier–tridiagonal on stretched z) */
+#define OM_PRECOND_COLUMNWISE_TRIDIAGONAL 4 /* ColumnwiseTridiagonalPreconditioner: per-column vertical part of V∇²,
+                                               conjugate_gradient_poisson_solver.jl:298-397 */
 
 #define OM_MAX_TRACERS 8
 #define OM_MAX_FIELDS  (3 + OM_MAX_TRACERS)

@@ -51,6 +51,7 @@ template <class T> struct HostGrid {
     std::vector<T> dzc, dzf;   // index k + Hz (k from -Hz .. Nz + Hz), 0-based device k
     T Lx, Ly, Lz;
     bool stretched = false;
+    std::vector<double> zc;   // interior cell-centre heights, device k = 0 .. Nz-1
 };
 
 template <class T> static T spacing(double L, int N) { return (T)((long double)L / (long double)N); }
@@ -71,6 +72,9 @@ template <class T> static void make_grid(HostGrid<T> &g, const om_config_t &c) {
         for (auto &x : g.dzc) x = dz;
         for (auto &x : g.dzf) x = dz;
         g.Lz = (T)c.extent[2];
+        // node positions of a regular grid (range(z₀, step = Lz/Nz), rectilinear_grid.jl)
+        const long double d = (long double)c.extent[2] / Nz;
+        for (int kd = 0; kd < Nz; ++kd) g.zc.push_back((double)((long double)c.origin[2] + (kd + 0.5L) * d));
         return;
     }
     g.stretched = true;
@@ -95,6 +99,7 @@ template <class T> static void make_grid(HostGrid<T> &g, const om_config_t &c) {
     // device index kd = reference k - 1; array slot kd + Hz
     for (int kd = -Hz; kd < Nz + Hz; ++kd) g.dzc[kd + Hz] = F[kd + Hz + 1] - F[kd + Hz];
     for (int kd = -Hz + 1; kd < Nz + Hz; ++kd) g.dzf[kd + Hz] = C[kd + Hz] - C[kd + Hz - 1];
+    for (int kd = 0; kd < Nz; ++kd) g.zc.push_back((double)C[kd + Hz]);
     g.dzf[0] = g.dzf[1];
     g.dzc[n - 1] = g.dzc[n - 2];
     g.dzf[n - 1] = g.dzf[n - 2];
@@ -164,6 +169,7 @@ template <class T> struct ModelT : ModelBase {
     double *cg_r = nullptr, *cg_b = nullptr;       // residual and right-hand side (FP64, padded layout)
     double *cg_x = nullptr, *cg_p = nullptr, *cg_q = nullptr;   // iterate, search direction, A·p (FP64)
     T *cg_z = nullptr;                             // preconditioned residual (model precision)
+    T *cg_t = nullptr;                             // Thomas-sweep ratios of the columnwise preconditioner
     double *cg_partial = nullptr, *cg_out = nullptr, *cg_host = nullptr;
     int cg_iterations = 0;
     double cg_res_last = 0, cg_true_res_last = 0;
@@ -251,6 +257,21 @@ template <class T> struct ModelT : ModelBase {
         g.idzc = upload(idzc) + g.Hz;
         g.idzf = upload(idzf) + g.Hz;
         g.stretched = hg.stretched;
+        // GridFittedBottom: kbot = the number of cells of a column whose centre lies at or below
+        // the bottom height, compared in FT (grid_fitted_bottom.jl:118-147)
+        g.kbot = nullptr;
+        if (c.bottom_height) {
+            std::vector<int> kb((size_t)g.Nx * g.Ny, 0);
+            for (int j = 0; j < g.Ny; ++j)
+                for (int i = 0; i < g.Nx; ++i) {
+                    const T zb = (T)c.bottom_height[(size_t)i + (size_t)g.Nx * j];
+                    int n = 0;
+                    for (int kd = 0; kd < g.Nz; ++kd)
+                        if ((T)hg.zc[kd] <= zb) n = kd + 1;
+                    kb[(size_t)i + (size_t)g.Nx * j] = n;
+                }
+            g.kbot = upload(kb);
+        }
         // physics
         ph.advection = c.advection; ph.closure = c.closure; ph.buoyancy = c.buoyancy; ph.coriolis = c.coriolis;
         ph.n_tracers = nt; ph.b_tracer = c.buoyancy_tracer; ph.T_tracer = c.T_tracer; ph.S_tracer = c.S_tracer;
@@ -312,6 +333,7 @@ template <class T> struct ModelT : ModelBase {
             cg_q = field_origin(dalloc<double>(fsize)); cg_z = field_origin(dalloc<T>(fsize));
             cg_partial = dalloc<double>(2 * 1024);
             cg_out = dalloc<double>(4);
+            if (cg_precond() == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_t = dalloc<T>((size_t)g.Nx * g.Ny * g.Nz);
             HIPCHECK(hipHostMalloc((void **)&cg_host, 4 * sizeof(double), hipHostMallocDefault));
         }
         HIPCHECK(hipDeviceSynchronize());
@@ -373,6 +395,10 @@ template <class T> struct ModelT : ModelBase {
         else launch_tendencies(C, st);
     }
     void update_state() {
+        if (g.kbot) {   // mask immersed tracers (update_nonhydrostatic_model_state.jl:30-33)
+            PhaseScope ps(this, OM_PHASE_HALOS);
+            for (int n = 0; n < nt; ++n) launch_mask(g, s.U[3 + n], 0, st);
+        }
         halos_prognostic(all_mask(), false);
         if (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
             {
@@ -391,6 +417,10 @@ template <class T> struct ModelT : ModelBase {
         compute_tendencies();
     }
     void pressure_correction(T dtau) {
+        if (g.kbot) {   // mask immersed velocities (pressure_correction.jl:7)
+            PhaseScope ps(this, OM_PHASE_HALOS);
+            for (int f = 0; f < 3; ++f) launch_mask(g, s.U[f], f + 1, st);
+        }
         halos_prognostic(0x7u, true);
         const T eps = std::numeric_limits<T>::epsilon();
         const T dtp = dtau > eps ? dtau : eps;
@@ -428,7 +458,10 @@ template <class T> struct ModelT : ModelBase {
     }
     int cg_precond() const {
         const int pk = cfg.cg_preconditioner;
-        return pk == OM_PRECOND_DEFAULT ? OM_PRECOND_DIAGONALLY_DOMINANT : pk;
+        // the reference's default: the FFT solver of the underlying grid on an immersed grid
+        // (conjugate_gradient_poisson_solver.jl:141-147), the diagonally dominant one otherwise
+        if (pk == OM_PRECOND_DEFAULT) return g.kbot ? OM_PRECOND_FFT : OM_PRECOND_DIAGONALLY_DOMINANT;
+        return pk;
     }
     void cg_solve() {
         double *x = cg_x;
@@ -451,7 +484,8 @@ template <class T> struct ModelT : ModelBase {
             else if (pk == OM_PRECOND_FFT) {
                 cg_fft_src(g, cg_r, rhs_dense, st);
                 launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);
-            } else cg_copy_r(g, cg_r, cg_z, st);
+            } else if (pk == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_ct(g, cg_r, cg_z, cg_t, st);
+            else cg_copy_r(g, cg_r, cg_z, st);
             cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out, st);
             const double rho = cg_read(1);
             cg_update_p(g, cg_p, cg_z, it == 0 ? 0.0 : rho / rho_prev, it == 0 ? 1 : 0, st);
@@ -629,10 +663,15 @@ static std::string validate(const om_config_t *c) {
     if (c->pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
         for (int d = 0; d < 3; ++d)
             if (c->topology[d] == OM_FLAT) return "the conjugate-gradient solver needs a three-dimensional grid";
-        if (c->cg_preconditioner < OM_PRECOND_DEFAULT || c->cg_preconditioner > OM_PRECOND_FFT) return "unknown CG preconditioner";
+        if (c->cg_preconditioner < OM_PRECOND_DEFAULT || c->cg_preconditioner > OM_PRECOND_COLUMNWISE_TRIDIAGONAL) return "unknown CG preconditioner";
     }
     if (c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL && c->topology[2] != OM_BOUNDED)
         return "the Fourier-tridiagonal solver needs a Bounded z";
+    if (c->bottom_height) {
+        if (c->topology[2] != OM_BOUNDED) return "an immersed bottom needs a Bounded z";
+        if (c->advection != OM_ADV_WENO5 && c->advection != OM_ADV_NONE)
+            return "an immersed bottom supports WENO(order=5) advection or none";
+    }
     if (c->buoyancy == OM_BUOYANCY_TRACER && (c->buoyancy_tracer < 0 || c->buoyancy_tracer >= c->n_tracers))
         return "buoyancy tracer index out of range";
     if (c->buoyancy == OM_BUOYANCY_LINEAR_EOS &&

@@ -11,6 +11,8 @@
 // then one block summing the partials in a fixed order), so a solve is bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <limits>
+
 #include "om_kernels.h"
 
 namespace om {
@@ -42,15 +44,39 @@ template <class T> __device__ __forceinline__ void cell_of(const DevGrid<T> &g, 
 
 template <class T> __device__ __forceinline__ T vol(const DevGrid<T> &g, int k) { return g.Az * g.dzc[k]; }
 
-// V∇²ᶜᶜᶜ c (conjugate_gradient_poisson_solver.jl:62-66) evaluated in type E
+// 1 for an active cell, 0 for an immersed one
+template <class T> __device__ __forceinline__ double active(const DevGrid<T> &g, int i, int j, int k) {
+    return IB<T>(g).inactive_cell(i, j, k) ? 0.0 : 1.0;
+}
+
+// V∇²ᶜᶜᶜ c (conjugate_gradient_poisson_solver.jl:62-66) evaluated in type E.  On an immersed grid
+// the inner ∂ vanishes across a face next to an immersed cell and the outer δ next to an immersed
+// inactive face (conditional_differences.jl:15-44).
 template <class E, class T, class F>
-__device__ __forceinline__ E vlap(const DevGrid<T> &g, const F *c, int64_t o, int k) {
+__device__ __forceinline__ E vlap(const DevGrid<T> &g, const F *c, int64_t o, int i, int j, int k) {
     const E idx = E(1) / E(g.dx), idy = E(1) / E(g.dy);
     const E Ax = E(g.dy) * E(g.dzc[k]), Ay = E(g.dx) * E(g.dzc[k]), Az = E(g.dx) * E(g.dy);
     const E c0 = E(c[o]);
-    const E tx = Ax * ((E(c[o + 1]) - c0) * idx) - Ax * ((c0 - E(c[o - 1])) * idx);
-    const E ty = Ay * ((E(c[o + g.sy]) - c0) * idy) - Ay * ((c0 - E(c[o - g.sy])) * idy);
-    const E tz = Az * ((E(c[o + g.sz]) - c0) * (E(1) / E(g.dzf[k + 1]))) - Az * ((c0 - E(c[o - g.sz])) * (E(1) / E(g.dzf[k])));
+    E gxp = (E(c[o + 1]) - c0) * idx, gxm = (c0 - E(c[o - 1])) * idx;
+    E gyp = (E(c[o + g.sy]) - c0) * idy, gym = (c0 - E(c[o - g.sy])) * idy;
+    E gzp = (E(c[o + g.sz]) - c0) * (E(1) / E(g.dzf[k + 1])), gzm = (c0 - E(c[o - g.sz])) * (E(1) / E(g.dzf[k]));
+    bool ox = false, oy = false, oz = false;
+    if (g.kbot) {
+        const IB<T> ib(g);
+        const bool ic = ib.template iin<0, 0, 0>(i, j, k);
+        if (ic || ib.template iin<0, 0, 0>(i + 1, j, k)) gxp = E(0);
+        if (ic || ib.template iin<0, 0, 0>(i - 1, j, k)) gxm = E(0);
+        if (ic || ib.template iin<0, 0, 0>(i, j + 1, k)) gyp = E(0);
+        if (ic || ib.template iin<0, 0, 0>(i, j - 1, k)) gym = E(0);
+        if (ic || ib.template iin<0, 0, 0>(i, j, k + 1)) gzp = E(0);
+        if (ic || ib.template iin<0, 0, 0>(i, j, k - 1)) gzm = E(0);
+        ox = ib.template iin<1, 0, 0>(i, j, k) || ib.template iin<1, 0, 0>(i + 1, j, k);
+        oy = ib.template iin<0, 1, 0>(i, j, k) || ib.template iin<0, 1, 0>(i, j + 1, k);
+        oz = ib.template iin<0, 0, 1>(i, j, k) || ib.template iin<0, 0, 1>(i, j, k + 1);
+    }
+    const E tx = ox ? E(0) : Ax * gxp - Ax * gxm;
+    const E ty = oy ? E(0) : Ay * gyp - Ay * gym;
+    const E tz = oz ? E(0) : Az * gzp - Az * gzm;
     return (tx + ty) + tz;
 }
 
@@ -85,6 +111,7 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_xio(DevGrid<T> g,
 }
 
 // b = active · divᶜᶜᶜ(U) · V (solve_for_pressure.jl:44-50), or b = rhs · V for a given dense rhs
+// (solve!(x, ::ConjugateGradientPoissonSolver, b) takes the caller's right-hand side as it is)
 template <class T> __global__ void __launch_bounds__(256) k_cg_source(Ctx<T> C, const T *dense_rhs, double *b) {
     const DevGrid<T> &g = C.g;
     const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
@@ -93,15 +120,9 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_source(Ctx<T> C, 
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
         T d;
         if (dense_rhs) d = dense_rhs[t];
-        else {
-            const T *u = C.s.U[0], *v = C.s.U[1], *w = C.s.U[2];
-            const T Ax = g.dy * g.dzc[k], Ay = g.dx * g.dzc[k];
-            const T dxs = Ax * u[o + 1] - Ax * u[o];
-            const T dys = Ay * v[o + g.sy] - Ay * v[o];
-            const T dzs = g.Az * w[o + g.sz] - g.Az * w[o];
-            d = (T(1) / vol(g, k)) * ((dxs + dys) + dzs);
-        }
-        b[o] = (double)((T(1) * d) * vol(g, k));
+        else d = C.divU(i, j, k, i + 1, j + 1);
+        const T act = (!dense_rhs && IB<T>(g).inactive_cell(i, j, k)) ? T(0) : T(1);
+        b[o] = (double)((act * d) * vol(g, k));
     }
 }
 
@@ -113,7 +134,7 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_residual(DevGrid<
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k; cell_of(g, t, i, j, k);
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double q = vlap<double>(g, x, o, k);
+        const double q = vlap<double>(g, x, o, i, j, k);
         double rv = b[o] - q;
         const double rs = rv * (double)(T(1) / vol(g, k));
         acc += rs * rs;
@@ -148,7 +169,53 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_dd(DevGrid<T> g, 
         acc = acc - ay / ac * r[o + g.sy];
         acc = acc - azm / acm * r[o - g.sz];
         acc = acc - azp / acp * r[o + g.sz];
-        z[o] = (T)(1.0 / fabs(ac) * acc);
+        z[o] = (T)((1.0 / fabs(ac) * acc) * active(g, i, j, k));
+    }
+}
+
+// ColumnwiseTridiagonalPreconditioner (conjugate_gradient_poisson_solver.jl:298-397): one thread per
+// column solves the vertical part of V∇² (diagonal × (1 + 1/100), identity rows for inactive and
+// vertically isolated cells) by the Thomas sweep of batched_tridiagonal_solver.jl in the model's
+// precision; a pivot under 10·eps keeps z's previous value, as the reference's forward sweep does.
+// The sweep's ratios t live in a dense scratch array indexed k·(Nx·Ny) + column (coalesced).
+template <class T> __global__ void __launch_bounds__(256) k_cg_ct(DevGrid<T> g, const double *r, T *z, T *t) {
+    const int64_t ncol = (int64_t)g.Nx * g.Ny;
+    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int i = (int)(col % g.Nx), j = (int)(col / g.Nx);
+    const IB<T> ib(g);
+    auto azf = [&](int k) { return g.Az * g.idzf[k]; };
+    auto off = [&](int k) {
+        const T ap = ib.inactive_cell(i, j, k + 1) ? T(0) : azf(k + 1);
+        return ib.inactive_cell(i, j, k) ? T(0) : ap;
+    };
+    auto diag = [&](int k) {
+        const bool s = ib.inactive_cell(i, j, k), b = ib.inactive_cell(i, j, k - 1), a = ib.inactive_cell(i, j, k + 1);
+        const T am = b ? T(0) : azf(k), ap = a ? T(0) : azf(k + 1);
+        const T eps = T(1) / T(100);
+        return (s || (b && a)) ? T(1) : -(am + ap) * (T(1) + eps);
+    };
+    const T guard = T(10) * std::numeric_limits<T>::epsilon();
+    const int64_t o = idx3(i, j, 0, g.sy, g.sz);
+    T beta = diag(0);
+    T zp = T(r[o]) / beta;
+    z[o] = zp;
+    for (int k = 1; k < g.Nz; ++k) {
+        const T c = off(k - 1), bk = diag(k);
+        const T tk = c / beta;
+        t[(int64_t)k * ncol + col] = tk;
+        beta = bk - c * tk;
+        const int64_t ok = o + (int64_t)k * g.sz;
+        const T phis = (T(r[ok]) - c * zp) / beta;
+        const T zk = fabs(beta) > guard ? phis : z[ok];
+        z[ok] = zk;
+        zp = zk;
+    }
+    for (int k = g.Nz - 2; k >= 0; --k) {
+        const int64_t ok = o + (int64_t)k * g.sz;
+        const T zk = z[ok] - t[(int64_t)(k + 1) * ncol + col] * zp;
+        z[ok] = zk;
+        zp = zk;
     }
 }
 
@@ -200,7 +267,7 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_apply(DevGrid<T> 
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k; cell_of(g, t, i, j, k);
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double qq = vlap<double>(g, p, o, k);
+        const double qq = vlap<double>(g, p, o, i, j, k);
         q[o] = qq;
         acc += p[o] * qq;
     }
@@ -236,8 +303,9 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGri
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k; cell_of(g, t, i, j, k);
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        x[o] = (x[o] - mx) * 1.0;
-        const double rg = (r[o] - mr) * 1.0;
+        const double act = active(g, i, j, k);
+        x[o] = (x[o] - mx) * act;
+        const double rg = (r[o] - mr) * act;
         const double rs = rg * (double)(T(1) / vol(g, k));
         acc += rs * rs;
         r[o] = rs * (double)vol(g, k);
@@ -254,7 +322,7 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_true_residual(Dev
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k; cell_of(g, t, i, j, k);
         const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double v = b[o] - vlap<double>(g, x, o, k);
+        const double v = b[o] - vlap<double>(g, x, o, i, j, k);
         e[o] = v;
         acc += v;
     }
@@ -352,6 +420,10 @@ template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStr
 template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_fft_src<T>, red_grid(), red_blk(), 0, st, g, r, dense);
 }
+template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st) {
+    const int64_t ncol = (int64_t)g.Nx * g.Ny;
+    hipLaunchKernelGGL(k_cg_ct<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g, r, z, t);
+}
 template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_copy_r<T>, red_grid(), red_blk(), 0, st, g, r, z);
 }
@@ -398,6 +470,7 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void cg_dd<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                               \
     template void cg_fft_src<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                          \
     template void cg_copy_r<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                           \
+    template void cg_ct<T>(const DevGrid<T> &, const double *, T *, T *, hipStream_t);                          \
     template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
     template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, double, int, hipStream_t);            \
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \

@@ -223,7 +223,8 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
 template <class T> bool tiled_closure_supported(const Ctx<T> &C) {
     const DevGrid<T> &g = C.g;
     return (C.ph.closure == OM_CLOSURE_AMD || C.ph.closure == OM_CLOSURE_SMAGORINSKY_LILLY) && g.tx == OM_PERIODIC &&
-           g.ty == OM_PERIODIC && g.tz != OM_FLAT && g.Hx >= 1 && g.Hy >= 1 && g.Hz >= 1 && C.ph.n_tracers <= 2;
+           g.ty == OM_PERIODIC && g.tz != OM_FLAT && g.Hx >= 1 && g.Hy >= 1 && g.Hz >= 1 && C.ph.n_tracers <= 2 &&
+           !g.kbot;   // the immersed-boundary conditions live in the generic kernel
 }
 
 template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStream_t st) {

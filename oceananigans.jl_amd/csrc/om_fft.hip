@@ -92,11 +92,7 @@ template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const
     // divergence with periodic index wrap in x and y (halos of U* are not needed)
     const int ip = (i + 1 == C.g.Nx) ? 0 : i + 1;
     const int jp = (j + 1 == C.g.Ny) ? 0 : j + 1;
-    const T *u = C.s.U[0], *v = C.s.U[1], *w = C.s.U[2];
-    T sx = C.flx() ? T(0) : C.Axc(k) * C.ld(u, ip, j, k) - C.Axc(k) * C.ld(u, i, j, k);
-    T sy = C.fly() ? T(0) : C.Ayc(k) * C.ld(v, i, jp, k) - C.Ayc(k) * C.ld(v, i, j, k);
-    T sz = C.flz() ? T(0) : C.g.Az * C.ld(w, i, j, k + 1) - C.g.Az * C.ld(w, i, j, k);
-    return C.iVc(k) * ((sx + sy) + sz);
+    return C.divU(i, j, k, ip, jp);
 }
 
 template <class T> __global__ void __launch_bounds__(256) k_xfwd(XFwdArgs<T> A) {

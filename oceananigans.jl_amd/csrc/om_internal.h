@@ -31,6 +31,7 @@ template <class T> struct DevGrid {
     const T *dzc, *dzf;      // Δzᶜ[k], Δzᶠ[k] for k = -Hz .. Nz+Hz (pointer offset so [k] is valid)
     const T *idzc, *idzf;    // reciprocals
     int stretched;
+    const int *kbot;         // immersed bottom: immersed cells per interior column (x fastest), or nullptr
 };
 
 template <class T> struct DevField {

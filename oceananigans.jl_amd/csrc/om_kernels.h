@@ -128,6 +128,82 @@ template <class T> __device__ __forceinline__ T cen2(T m1, T z0) {
 }
 
 // ----------------------------------------------------------------------------------------
+// Immersed-bottom predicates (GridFittedBottom; src/Grids/inactive_node.jl:36-137,
+// immersed_boundary_interface.jl), device indices (0-based).  A node at location (LX, LY, LZ)
+// (0 Center, 1 Face) touches the cells i-a, j-b, k-c for a ≤ LX, b ≤ LY, c ≤ LZ: it is inactive
+// when all of them are and peripheral when any of them is (the reference's recursion, unrolled).
+// ----------------------------------------------------------------------------------------
+template <class T> struct IB {
+    const DevGrid<T> &g;
+    __device__ __forceinline__ explicit IB(const DevGrid<T> &gg) : g(gg) {}
+    __device__ __forceinline__ bool on() const { return g.kbot != nullptr; }
+    __device__ __forceinline__ bool outside(int k) const { return g.tz == OM_BOUNDED && (k < 0 || k >= g.Nz); }
+    __device__ __forceinline__ bool immersed(int i, int j, int k) const {
+        if (!g.kbot) return false;
+        const int ii = i < 0 ? i + g.Nx * ((-i + g.Nx - 1) / g.Nx) : i % g.Nx;
+        const int jj = j < 0 ? j + g.Ny * ((-j + g.Ny - 1) / g.Ny) : j % g.Ny;
+        return k < g.kbot[ii % g.Nx + g.Nx * (jj % g.Ny)];
+    }
+    __device__ __forceinline__ bool inactive_cell(int i, int j, int k) const { return immersed(i, j, k) || outside(k); }
+    template <int LX, int LY, int LZ> __device__ __forceinline__ bool all_inactive(int i, int j, int k, bool under) const {
+        bool r = true;
+#pragma unroll
+        for (int a = 0; a <= LX; ++a)
+#pragma unroll
+            for (int b = 0; b <= LY; ++b)
+#pragma unroll
+                for (int c = 0; c <= LZ; ++c) r = r && (under ? outside(k - c) : inactive_cell(i - a, j - b, k - c));
+        return r;
+    }
+    template <int LX, int LY, int LZ> __device__ __forceinline__ bool any_inactive(int i, int j, int k, bool under) const {
+        bool r = false;
+#pragma unroll
+        for (int a = 0; a <= LX; ++a)
+#pragma unroll
+            for (int b = 0; b <= LY; ++b)
+#pragma unroll
+                for (int c = 0; c <= LZ; ++c) r = r || (under ? outside(k - c) : inactive_cell(i - a, j - b, k - c));
+        return r;
+    }
+    // immersed_inactive_node / immersed_peripheral_node: on the immersed grid but not on the underlying one
+    template <int LX, int LY, int LZ> __device__ __forceinline__ bool iin(int i, int j, int k) const {
+        return on() && all_inactive<LX, LY, LZ>(i, j, k, false) && !all_inactive<LX, LY, LZ>(i, j, k, true);
+    }
+    template <int LX, int LY, int LZ> __device__ __forceinline__ bool iper(int i, int j, int k) const {
+        return on() && any_inactive<LX, LY, LZ>(i, j, k, false) && !any_inactive<LX, LY, LZ>(i, j, k, true);
+    }
+    // conditional δ between two points of one location (conditional_differences.jl:15-44)
+    template <int LX, int LY, int LZ> __device__ __forceinline__ bool zd(int i1, int j1, int k1, int i2, int j2, int k2) const {
+        return on() && (iin<LX, LY, LZ>(i1, j1, k1) || iin<LX, LY, LZ>(i2, j2, k2));
+    }
+    // near_{x,y,z}_immersed_boundary for `buffer` (immersed_advective_fluxes.jl:127-215): any inactive
+    // node among the 2·buffer stencil points; cell centres for a face interpolation point (cs = false),
+    // faces along d for a centre interpolation point (cs = true)
+    __device__ __forceinline__ bool near(int d, bool cs, int buffer, int i, int j, int k) const {
+        bool r = false;
+        for (int n = 1; n <= 2 * buffer; ++n) {
+            const int c = cs ? n - buffer : n - buffer - 1;
+            const int ii = i + (d == 0 ? c : 0), jj = j + (d == 1 ? c : 0), kk = k + (d == 2 ? c : 0);
+            bool in = inactive_cell(ii, jj, kk);
+            if (cs) in = in && inactive_cell(ii - (d == 0), jj - (d == 1), kk - (d == 2));
+            r = r || in;
+        }
+        return r;
+    }
+    // buffer chain WENO5 (0) → WENO3 (1) → Centered(2) (2); `topo` is the choice without an immersed boundary
+    __device__ __forceinline__ int bsch(int d, bool cs, int i, int j, int k, int topo) const {
+        if (!on()) return topo;
+        if (!near(d, cs, 3, i, j, k)) return 0;
+        return near(d, cs, 2, i, j, k) ? 2 : 1;
+    }
+    // advecting velocity: Centered(4) (true) or Centered(2) (false)
+    __device__ __forceinline__ bool s4(int d, bool cs, int i, int j, int k, bool topo) const {
+        if (!on()) return topo;
+        return !near(d, cs, 3, i, j, k);
+    }
+};
+
+// ----------------------------------------------------------------------------------------
 // Context: grid + state + physics with the reference's operators as member functions
 // ----------------------------------------------------------------------------------------
 template <class T> struct Ctx {
@@ -164,12 +240,17 @@ template <class T> struct Ctx {
     __device__ __forceinline__ bool zs4_face(int k) const { const int kr = k + 1; return !bz() || (kr >= 4 && kr <= g.Nz - 2); }
     __device__ __forceinline__ bool zs4_center(int k) const { const int kr = k + 1; return !bz() || (kr >= 3 && kr <= g.Nz - 2); }
 
-    // ---- biased reconstructions of field f at faces; x and y are never switched
-    __device__ __forceinline__ T bx(const T *f, int i, int j, int k, bool left) const {
-        return weno5<T>(ld(f, i - 3, j, k), ld(f, i - 2, j, k), ld(f, i - 1, j, k), ld(f, i, j, k), ld(f, i + 1, j, k), ld(f, i + 2, j, k), left);
+    // ---- biased reconstructions of field f at faces with scheme sch (0 WENO5, 1 WENO3, 2 Centered(2));
+    // x and y switch only next to an immersed bottom
+    __device__ __forceinline__ T bx_s(const T *f, int i, int j, int k, bool left, int sch) const {
+        if (sch == 0) return weno5<T>(ld(f, i - 3, j, k), ld(f, i - 2, j, k), ld(f, i - 1, j, k), ld(f, i, j, k), ld(f, i + 1, j, k), ld(f, i + 2, j, k), left);
+        if (sch == 1) return weno3<T>(ld(f, i - 2, j, k), ld(f, i - 1, j, k), ld(f, i, j, k), ld(f, i + 1, j, k), left);
+        return cen2<T>(ld(f, i - 1, j, k), ld(f, i, j, k));
     }
-    __device__ __forceinline__ T by(const T *f, int i, int j, int k, bool left) const {
-        return weno5<T>(ld(f, i, j - 3, k), ld(f, i, j - 2, k), ld(f, i, j - 1, k), ld(f, i, j, k), ld(f, i, j + 1, k), ld(f, i, j + 2, k), left);
+    __device__ __forceinline__ T by_s(const T *f, int i, int j, int k, bool left, int sch) const {
+        if (sch == 0) return weno5<T>(ld(f, i, j - 3, k), ld(f, i, j - 2, k), ld(f, i, j - 1, k), ld(f, i, j, k), ld(f, i, j + 1, k), ld(f, i, j + 2, k), left);
+        if (sch == 1) return weno3<T>(ld(f, i, j - 2, k), ld(f, i, j - 1, k), ld(f, i, j, k), ld(f, i, j + 1, k), left);
+        return cen2<T>(ld(f, i, j - 1, k), ld(f, i, j, k));
     }
     // z face k with scheme s (stencil centred on face k)
     __device__ __forceinline__ T bz_s(const T *f, int i, int j, int k, bool left, int sch) const {
@@ -178,13 +259,18 @@ template <class T> struct Ctx {
         return cen2<T>(ld(f, i, j, k - 1), ld(f, i, j, k));
     }
 
-    // ================= momentum advective fluxes (upwind_biased_advective_fluxes.jl:23-93)
+    // ================= momentum advective fluxes (upwind_biased_advective_fluxes.jl:23-93); on an
+    // immersed grid they vanish on the immersed periphery (immersed_advective_fluxes.jl:19-70) and the
+    // interpolation order drops next to immersed cells (immersed_advective_fluxes.jl:127-215)
     __device__ __forceinline__ T F_Uu(int i, int j, int k) const {  // ccc
         const T *u = s.U[0];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(u, i, j, k) + ld(u, i + 1, j, k)); return Axc(k) * a * a; }
+        const IB<T> ib(g);
+        if (ib.template iper<0, 0, 0>(i, j, k)) return T(0);
         const T A = Axc(k);
-        const T ut = cen4<T>(A * ld(u, i - 1, j, k), A * ld(u, i, j, k), A * ld(u, i + 1, j, k), A * ld(u, i + 2, j, k));
-        return ut * bx(u, i + 1, j, k, ut > T(0));
+        const T ut = ib.s4(0, true, i, j, k, true) ? cen4<T>(A * ld(u, i - 1, j, k), A * ld(u, i, j, k), A * ld(u, i + 1, j, k), A * ld(u, i + 2, j, k))
+                                                  : cen2<T>(A * ld(u, i, j, k), A * ld(u, i + 1, j, k));
+        return ut * bx_s(u, i + 1, j, k, ut > T(0), ib.bsch(0, true, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Vu(int i, int j, int k) const {  // ffc
         if (fly()) return T(0);
@@ -193,9 +279,13 @@ template <class T> struct Ctx {
             T vi = flx() ? ld(v, i, j, k) : T(0.5) * (ld(v, i - 1, j, k) + ld(v, i, j, k));
             return Ayc(k) * vi * (T(0.5) * (ld(u, i, j - 1, k) + ld(u, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<1, 1, 0>(i, j, k)) return T(0);
         const T A = Ayc(k);
-        const T vt = flx() ? A * ld(v, i, j, k) : cen4<T>(A * ld(v, i - 2, j, k), A * ld(v, i - 1, j, k), A * ld(v, i, j, k), A * ld(v, i + 1, j, k));
-        return vt * by(u, i, j, k, vt > T(0));
+        const T vt = flx() ? A * ld(v, i, j, k)
+                           : (ib.s4(0, false, i, j, k, true) ? cen4<T>(A * ld(v, i - 2, j, k), A * ld(v, i - 1, j, k), A * ld(v, i, j, k), A * ld(v, i + 1, j, k))
+                                                             : cen2<T>(A * ld(v, i - 1, j, k), A * ld(v, i, j, k)));
+        return vt * by_s(u, i, j, k, vt > T(0), ib.bsch(1, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Wu(int i, int j, int k) const {  // fcf
         if (flz()) return T(0);
@@ -204,9 +294,13 @@ template <class T> struct Ctx {
             T wi = flx() ? ld(w, i, j, k) : T(0.5) * (ld(w, i - 1, j, k) + ld(w, i, j, k));
             return g.Az * wi * (T(0.5) * (ld(u, i, j, k - 1) + ld(u, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<1, 0, 1>(i, j, k)) return T(0);
         const T A = g.Az;
-        const T wt = flx() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i - 2, j, k), A * ld(w, i - 1, j, k), A * ld(w, i, j, k), A * ld(w, i + 1, j, k));
-        return wt * bz_s(u, i, j, k, wt > T(0), zb_face(k));
+        const T wt = flx() ? A * ld(w, i, j, k)
+                           : (ib.s4(0, false, i, j, k, true) ? cen4<T>(A * ld(w, i - 2, j, k), A * ld(w, i - 1, j, k), A * ld(w, i, j, k), A * ld(w, i + 1, j, k))
+                                                             : cen2<T>(A * ld(w, i - 1, j, k), A * ld(w, i, j, k)));
+        return wt * bz_s(u, i, j, k, wt > T(0), ib.bsch(2, false, i, j, k, zb_face(k)));
     }
     __device__ __forceinline__ T F_Uv(int i, int j, int k) const {  // ffc
         if (flx()) return T(0);
@@ -215,16 +309,23 @@ template <class T> struct Ctx {
             T uj = fly() ? ld(u, i, j, k) : T(0.5) * (ld(u, i, j - 1, k) + ld(u, i, j, k));
             return Axc(k) * uj * (T(0.5) * (ld(v, i - 1, j, k) + ld(v, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<1, 1, 0>(i, j, k)) return T(0);
         const T A = Axc(k);
-        const T ut = fly() ? A * ld(u, i, j, k) : cen4<T>(A * ld(u, i, j - 2, k), A * ld(u, i, j - 1, k), A * ld(u, i, j, k), A * ld(u, i, j + 1, k));
-        return ut * bx(v, i, j, k, ut > T(0));
+        const T ut = fly() ? A * ld(u, i, j, k)
+                           : (ib.s4(1, false, i, j, k, true) ? cen4<T>(A * ld(u, i, j - 2, k), A * ld(u, i, j - 1, k), A * ld(u, i, j, k), A * ld(u, i, j + 1, k))
+                                                             : cen2<T>(A * ld(u, i, j - 1, k), A * ld(u, i, j, k)));
+        return ut * bx_s(v, i, j, k, ut > T(0), ib.bsch(0, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Vv(int i, int j, int k) const {  // ccc
         const T *v = s.U[1];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(v, i, j, k) + ld(v, i, j + 1, k)); return Ayc(k) * a * a; }
+        const IB<T> ib(g);
+        if (ib.template iper<0, 0, 0>(i, j, k)) return T(0);
         const T A = Ayc(k);
-        const T vt = cen4<T>(A * ld(v, i, j - 1, k), A * ld(v, i, j, k), A * ld(v, i, j + 1, k), A * ld(v, i, j + 2, k));
-        return vt * by(v, i, j + 1, k, vt > T(0));
+        const T vt = ib.s4(1, true, i, j, k, true) ? cen4<T>(A * ld(v, i, j - 1, k), A * ld(v, i, j, k), A * ld(v, i, j + 1, k), A * ld(v, i, j + 2, k))
+                                                  : cen2<T>(A * ld(v, i, j, k), A * ld(v, i, j + 1, k));
+        return vt * by_s(v, i, j + 1, k, vt > T(0), ib.bsch(1, true, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Wv(int i, int j, int k) const {  // cff
         if (flz()) return T(0);
@@ -233,9 +334,13 @@ template <class T> struct Ctx {
             T wj = fly() ? ld(w, i, j, k) : T(0.5) * (ld(w, i, j - 1, k) + ld(w, i, j, k));
             return g.Az * wj * (T(0.5) * (ld(v, i, j, k - 1) + ld(v, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<0, 1, 1>(i, j, k)) return T(0);
         const T A = g.Az;
-        const T wt = fly() ? A * ld(w, i, j, k) : cen4<T>(A * ld(w, i, j - 2, k), A * ld(w, i, j - 1, k), A * ld(w, i, j, k), A * ld(w, i, j + 1, k));
-        return wt * bz_s(v, i, j, k, wt > T(0), zb_face(k));
+        const T wt = fly() ? A * ld(w, i, j, k)
+                           : (ib.s4(1, false, i, j, k, true) ? cen4<T>(A * ld(w, i, j - 2, k), A * ld(w, i, j - 1, k), A * ld(w, i, j, k), A * ld(w, i, j + 1, k))
+                                                             : cen2<T>(A * ld(w, i, j - 1, k), A * ld(w, i, j, k)));
+        return wt * bz_s(v, i, j, k, wt > T(0), ib.bsch(2, false, i, j, k, zb_face(k)));
     }
     __device__ __forceinline__ T F_Uw(int i, int j, int k) const {  // fcf
         if (flx()) return T(0);
@@ -244,12 +349,14 @@ template <class T> struct Ctx {
             T uk = flz() ? ld(u, i, j, k) : T(0.5) * (ld(u, i, j, k - 1) + ld(u, i, j, k));
             return Axf(k) * uk * (T(0.5) * (ld(w, i - 1, j, k) + ld(w, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<1, 0, 1>(i, j, k)) return T(0);
         T ut;
         if (flz()) ut = Axc(k) * ld(u, i, j, k);
-        else if (zs4_face(k))
+        else if (ib.s4(2, false, i, j, k, zs4_face(k)))
             ut = cen4<T>(Axc(k - 2) * ld(u, i, j, k - 2), Axc(k - 1) * ld(u, i, j, k - 1), Axc(k) * ld(u, i, j, k), Axc(k + 1) * ld(u, i, j, k + 1));
         else ut = cen2<T>(Axc(k - 1) * ld(u, i, j, k - 1), Axc(k) * ld(u, i, j, k));
-        return ut * bx(w, i, j, k, ut > T(0));
+        return ut * bx_s(w, i, j, k, ut > T(0), ib.bsch(0, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Vw(int i, int j, int k) const {  // cff
         if (fly()) return T(0);
@@ -258,52 +365,63 @@ template <class T> struct Ctx {
             T vk = flz() ? ld(v, i, j, k) : T(0.5) * (ld(v, i, j, k - 1) + ld(v, i, j, k));
             return Ayf(k) * vk * (T(0.5) * (ld(w, i, j - 1, k) + ld(w, i, j, k)));
         }
+        const IB<T> ib(g);
+        if (ib.template iper<0, 1, 1>(i, j, k)) return T(0);
         T vt;
         if (flz()) vt = Ayc(k) * ld(v, i, j, k);
-        else if (zs4_face(k))
+        else if (ib.s4(2, false, i, j, k, zs4_face(k)))
             vt = cen4<T>(Ayc(k - 2) * ld(v, i, j, k - 2), Ayc(k - 1) * ld(v, i, j, k - 1), Ayc(k) * ld(v, i, j, k), Ayc(k + 1) * ld(v, i, j, k + 1));
         else vt = cen2<T>(Ayc(k - 1) * ld(v, i, j, k - 1), Ayc(k) * ld(v, i, j, k));
-        return vt * by(w, i, j, k, vt > T(0));
+        return vt * by_s(w, i, j, k, vt > T(0), ib.bsch(1, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_Ww(int i, int j, int k) const {  // ccc (centre k uses the face k+1 stencils)
         if (flz()) return T(0);
         const T *w = s.U[2];
         if (ph.advection == OM_ADV_CENTERED2) { T a = T(0.5) * (ld(w, i, j, k) + ld(w, i, j, k + 1)); return g.Az * a * a; }
+        const IB<T> ib(g);
+        if (ib.template iper<0, 0, 0>(i, j, k)) return T(0);
         const T A = g.Az;
-        const T wt = zs4_center(k) ? cen4<T>(A * ld(w, i, j, k - 1), A * ld(w, i, j, k), A * ld(w, i, j, k + 1), A * ld(w, i, j, k + 2))
-                                   : cen2<T>(A * ld(w, i, j, k), A * ld(w, i, j, k + 1));
-        return wt * bz_s(w, i, j, k + 1, wt > T(0), zb_center(k));
+        const T wt = ib.s4(2, true, i, j, k, zs4_center(k)) ? cen4<T>(A * ld(w, i, j, k - 1), A * ld(w, i, j, k), A * ld(w, i, j, k + 1), A * ld(w, i, j, k + 2))
+                                                           : cen2<T>(A * ld(w, i, j, k), A * ld(w, i, j, k + 1));
+        return wt * bz_s(w, i, j, k + 1, wt > T(0), ib.bsch(2, true, i, j, k, zb_center(k)));
     }
     // ================= tracer advective fluxes (upwind_biased_advective_fluxes.jl:99-121)
     __device__ __forceinline__ T F_cx(const T *c, int i, int j, int k) const {
         const T ui = ld(s.U[0], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (Axc(k) * ui) * cen2<T>(ld(c, i - 1, j, k), ld(c, i, j, k));
-        return Axc(k) * ui * bx(c, i, j, k, ui > T(0));
+        const IB<T> ib(g);
+        if (ib.template iper<1, 0, 0>(i, j, k)) return T(0);
+        return Axc(k) * ui * bx_s(c, i, j, k, ui > T(0), ib.bsch(0, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_cy(const T *c, int i, int j, int k) const {
         const T vj = ld(s.U[1], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (Ayc(k) * vj) * cen2<T>(ld(c, i, j - 1, k), ld(c, i, j, k));
-        return Ayc(k) * vj * by(c, i, j, k, vj > T(0));
+        const IB<T> ib(g);
+        if (ib.template iper<0, 1, 0>(i, j, k)) return T(0);
+        return Ayc(k) * vj * by_s(c, i, j, k, vj > T(0), ib.bsch(1, false, i, j, k, 0));
     }
     __device__ __forceinline__ T F_cz(const T *c, int i, int j, int k) const {
         const T wk = ld(s.U[2], i, j, k);
         if (ph.advection == OM_ADV_CENTERED2) return (g.Az * wk) * cen2<T>(ld(c, i, j, k - 1), ld(c, i, j, k));
-        return g.Az * wk * bz_s(c, i, j, k, wk > T(0), zb_face(k));
+        const IB<T> ib(g);
+        if (ib.template iper<0, 0, 1>(i, j, k)) return T(0);
+        return g.Az * wk * bz_s(c, i, j, k, wk > T(0), ib.bsch(2, false, i, j, k, zb_face(k)));
     }
 
     // ================= gradients (velocity_tracer_gradients.jl:6-18)
-    __device__ __forceinline__ T dx_u(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[0], i + 1, j, k) - ld(s.U[0], i, j, k)) * g.idx; }
-    __device__ __forceinline__ T dy_v(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[1], i, j + 1, k) - ld(s.U[1], i, j, k)) * g.idy; }
-    __device__ __forceinline__ T dz_w(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[2], i, j, k + 1) - ld(s.U[2], i, j, k)) * g.idzc[k]; }
-    __device__ __forceinline__ T dx_v(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i - 1, j, k)) * g.idx; }
-    __device__ __forceinline__ T dx_w(int i, int j, int k) const { return flx() ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i - 1, j, k)) * g.idx; }
-    __device__ __forceinline__ T dy_u(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j - 1, k)) * g.idy; }
-    __device__ __forceinline__ T dy_w(int i, int j, int k) const { return fly() ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i, j - 1, k)) * g.idy; }
-    __device__ __forceinline__ T dz_u(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j, k - 1)) * g.idzf[k]; }
-    __device__ __forceinline__ T dz_v(int i, int j, int k) const { return flz() ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i, j, k - 1)) * g.idzf[k]; }
-    __device__ __forceinline__ T dxc(const T *c, int i, int j, int k) const { return flx() ? T(0) : (ld(c, i, j, k) - ld(c, i - 1, j, k)) * g.idx; }
-    __device__ __forceinline__ T dyc(const T *c, int i, int j, int k) const { return fly() ? T(0) : (ld(c, i, j, k) - ld(c, i, j - 1, k)) * g.idy; }
-    __device__ __forceinline__ T dzc(const T *c, int i, int j, int k) const { return flz() ? T(0) : (ld(c, i, j, k) - ld(c, i, j, k - 1)) * g.idzf[k]; }
+    // conditional δ next to an immersed bottom (conditional_differences.jl:15-44)
+    __device__ __forceinline__ T dx_u(int i, int j, int k) const { return flx() || IB<T>(g).template zd<1, 0, 0>(i, j, k, i + 1, j, k) ? T(0) : (ld(s.U[0], i + 1, j, k) - ld(s.U[0], i, j, k)) * g.idx; }
+    __device__ __forceinline__ T dy_v(int i, int j, int k) const { return fly() || IB<T>(g).template zd<0, 1, 0>(i, j, k, i, j + 1, k) ? T(0) : (ld(s.U[1], i, j + 1, k) - ld(s.U[1], i, j, k)) * g.idy; }
+    __device__ __forceinline__ T dz_w(int i, int j, int k) const { return flz() || IB<T>(g).template zd<0, 0, 1>(i, j, k, i, j, k + 1) ? T(0) : (ld(s.U[2], i, j, k + 1) - ld(s.U[2], i, j, k)) * g.idzc[k]; }
+    __device__ __forceinline__ T dx_v(int i, int j, int k) const { return flx() || IB<T>(g).template zd<0, 1, 0>(i, j, k, i - 1, j, k) ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dx_w(int i, int j, int k) const { return flx() || IB<T>(g).template zd<0, 0, 1>(i, j, k, i - 1, j, k) ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dy_u(int i, int j, int k) const { return fly() || IB<T>(g).template zd<1, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dy_w(int i, int j, int k) const { return fly() || IB<T>(g).template zd<0, 0, 1>(i, j, k, i, j - 1, k) ? T(0) : (ld(s.U[2], i, j, k) - ld(s.U[2], i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dz_u(int i, int j, int k) const { return flz() || IB<T>(g).template zd<1, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : (ld(s.U[0], i, j, k) - ld(s.U[0], i, j, k - 1)) * g.idzf[k]; }
+    __device__ __forceinline__ T dz_v(int i, int j, int k) const { return flz() || IB<T>(g).template zd<0, 1, 0>(i, j, k, i, j, k - 1) ? T(0) : (ld(s.U[1], i, j, k) - ld(s.U[1], i, j, k - 1)) * g.idzf[k]; }
+    __device__ __forceinline__ T dxc(const T *c, int i, int j, int k) const { return flx() || IB<T>(g).template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : (ld(c, i, j, k) - ld(c, i - 1, j, k)) * g.idx; }
+    __device__ __forceinline__ T dyc(const T *c, int i, int j, int k) const { return fly() || IB<T>(g).template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : (ld(c, i, j, k) - ld(c, i, j - 1, k)) * g.idy; }
+    __device__ __forceinline__ T dzc(const T *c, int i, int j, int k) const { return flz() || IB<T>(g).template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : (ld(c, i, j, k) - ld(c, i, j, k - 1)) * g.idzf[k]; }
     __device__ __forceinline__ T S12(int i, int j, int k) const { return T(0.5) * (dy_u(i, j, k) + dx_v(i, j, k)); }
     __device__ __forceinline__ T S13(int i, int j, int k) const { return T(0.5) * (dz_u(i, j, k) + dx_w(i, j, k)); }
     __device__ __forceinline__ T S23(int i, int j, int k) const { return T(0.5) * (dz_v(i, j, k) + dy_w(i, j, k)); }
@@ -343,6 +461,29 @@ template <class T> struct Ctx {
         if (ph.closure == OM_CLOSURE_AMD) return T(0.5) * (ld(s.kap[n], i - di, j - dj, k - dk) + ld(s.kap[n], i, j, k));
         return (T(0.5) * (ld(s.nu, i - di, j - dj, k - dk) + ld(s.nu, i, j, k))) / ph.c2;
     }
+
+    // ∇·U at a cell centre (divergence_operators.jl:16-19), conditional δ on an immersed grid;
+    // ip, jp address the i + 1, j + 1 neighbours (wrapped by a caller whose halos are stale)
+    __device__ __forceinline__ T divU(int i, int j, int k, int ip, int jp) const {
+        const IB<T> ib(g);
+        const T *u = s.U[0], *v = s.U[1], *w = s.U[2];
+        T sx = flx() || ib.template zd<1, 0, 0>(i, j, k, i + 1, j, k) ? T(0) : Axc(k) * ld(u, ip, j, k) - Axc(k) * ld(u, i, j, k);
+        T sy = fly() || ib.template zd<0, 1, 0>(i, j, k, i, j + 1, k) ? T(0) : Ayc(k) * ld(v, i, jp, k) - Ayc(k) * ld(v, i, j, k);
+        T sz = flz() || ib.template zd<0, 0, 1>(i, j, k, i, j, k + 1) ? T(0) : g.Az * ld(w, i, j, k + 1) - g.Az * ld(w, i, j, k);
+        return iVc(k) * ((sx + sy) + sz);
+    }
+
+    // viscous and diffusive fluxes (abstract_scalar_diffusivity_closure.jl:214-262), zero on the
+    // immersed periphery of their location (immersed_diffusive_fluxes.jl:25-42)
+    __device__ __forceinline__ T f_ux(int i, int j, int k) const { return IB<T>(g).template iper<0, 0, 0>(i, j, k) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dx_u(i, j, k)); }
+    __device__ __forceinline__ T f_uy(int i, int j, int k) const { return IB<T>(g).template iper<1, 1, 0>(i, j, k) ? T(0) : T(-2) * (nu_ffc(i, j, k) * S12(i, j, k)); }
+    __device__ __forceinline__ T f_uz(int i, int j, int k) const { return IB<T>(g).template iper<1, 0, 1>(i, j, k) ? T(0) : T(-2) * (nu_fcf(i, j, k) * S13(i, j, k)); }
+    __device__ __forceinline__ T f_vy(int i, int j, int k) const { return IB<T>(g).template iper<0, 0, 0>(i, j, k) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dy_v(i, j, k)); }
+    __device__ __forceinline__ T f_vz(int i, int j, int k) const { return IB<T>(g).template iper<0, 1, 1>(i, j, k) ? T(0) : T(-2) * (nu_cff(i, j, k) * S23(i, j, k)); }
+    __device__ __forceinline__ T f_wz(int i, int j, int k) const { return IB<T>(g).template iper<0, 0, 0>(i, j, k) ? T(0) : T(-2) * (nu_ccc(i, j, k) * dz_w(i, j, k)); }
+    __device__ __forceinline__ T q_x(int n, const T *c, int i, int j, int k) const { return IB<T>(g).template iper<1, 0, 0>(i, j, k) ? T(0) : -kap_f(n, i, j, k, 1, 0, 0) * dxc(c, i, j, k); }
+    __device__ __forceinline__ T q_y(int n, const T *c, int i, int j, int k) const { return IB<T>(g).template iper<0, 1, 0>(i, j, k) ? T(0) : -kap_f(n, i, j, k, 0, 1, 0) * dyc(c, i, j, k); }
+    __device__ __forceinline__ T q_z(int n, const T *c, int i, int j, int k) const { return IB<T>(g).template iper<0, 0, 1>(i, j, k) ? T(0) : -kap_f(n, i, j, k, 0, 0, 1) * dzc(c, i, j, k); }
 
     __device__ __forceinline__ T buoy(int i, int j, int k) const {
         if (ph.buoyancy == OM_BUOYANCY_TRACER) return ld(s.btr, i, j, k);
@@ -396,6 +537,8 @@ template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);
 // U −= dtp·∇p (phi = 0: p is the pressure) or U −= ∇ϕ (phi = 1: ϕ = p·Δt, as the reference stores it)
 template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st);
+// mask_immersed_field! of a field at location loc (0 ccc, 1 fcc, 2 cfc, 3 ccf) on an immersed grid
+template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st);
 // conjugate-gradient Poisson solver kernels (om_cg.hip)
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st);
 template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st);
@@ -407,6 +550,8 @@ template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st
 template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
 template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st);
 template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
+// ColumnwiseTridiagonalPreconditioner: z = Lz⁻¹ r per column; t is an Nx·Ny·Nz scratch array
+template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st);
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);
 template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st);
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st);

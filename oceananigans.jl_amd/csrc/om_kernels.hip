@@ -24,29 +24,35 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
     const bool clo = C.ph.closure != OM_CLOSURE_NONE;
     const int64_t o = C.at(i, j, k);
 
+    // On an immersed grid every difference is a conditional δ, fluxes vanish on the immersed
+    // periphery and the Coriolis transports are masked (immersed_boundary_grid.jl conditional
+    // operators, coriolis_schemes.jl:56-59); without one every predicate below is false.
+    const IB<T> ib(C.g);
     // ---------------- u (fcc)
     {
         T G = T(0);
         if (adv) {
-            T sx = C.flx() ? T(0) : C.F_Uu(i, j, k) - C.F_Uu(i - 1, j, k);
-            T sy = C.F_Vu(i, j + 1, k) - C.F_Vu(i, j, k);
-            T sz = C.F_Wu(i, j, k + 1) - C.F_Wu(i, j, k);
+            T sx = C.flx() || ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : C.F_Uu(i, j, k) - C.F_Uu(i - 1, j, k);
+            T sy = ib.template zd<1, 1, 0>(i, j + 1, k, i, j, k) ? T(0) : C.F_Vu(i, j + 1, k) - C.F_Vu(i, j, k);
+            T sz = ib.template zd<1, 0, 1>(i, j, k + 1, i, j, k) ? T(0) : C.F_Wu(i, j, k + 1) - C.F_Wu(i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         if (C.ph.coriolis) {
             const T *v = C.s.U[1];
             const T A = C.Ayc(k);
-            T a = C.flx() ? A * C.ld(v, i, j, k) : T(0.5) * (A * C.ld(v, i - 1, j, k) + A * C.ld(v, i, j, k));
-            T b = C.flx() ? A * C.ld(v, i, j + 1, k) : T(0.5) * (A * C.ld(v, i - 1, j + 1, k) + A * C.ld(v, i, j + 1, k));
+            auto q = [&](int a, int b) { return ib.template iper<0, 1, 0>(a, b, k) ? T(0) : A * C.ld(v, a, b, k); };
+            T a = C.flx() ? q(i, j) : T(0.5) * (q(i - 1, j) + q(i, j));
+            T b = C.flx() ? q(i, j + 1) : T(0.5) * (q(i - 1, j + 1) + q(i, j + 1));
             T I = C.fly() ? a : T(0.5) * (a + b);
             G += (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
         }
-        if (C.ph.hydrostatic) G -= (C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k)) * C.g.idx;
+        if (C.ph.hydrostatic)
+            G -= (ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k)) * C.g.idx;
         if (clo) {
             const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
-            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_ccc(i, j, k) * C.dx_u(i, j, k))) - Ax * (T(-2) * (C.nu_ccc(i - 1, j, k) * C.dx_u(i - 1, j, k)));
-            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_ffc(i, j + 1, k) * C.S12(i, j + 1, k))) - Ay * (T(-2) * (C.nu_ffc(i, j, k) * C.S12(i, j, k)));
-            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_fcf(i, j, k + 1) * C.S13(i, j, k + 1))) - Az * (T(-2) * (C.nu_fcf(i, j, k) * C.S13(i, j, k)));
+            T sx = C.flx() || ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : Ax * C.f_ux(i, j, k) - Ax * C.f_ux(i - 1, j, k);
+            T sy = C.fly() || ib.template zd<1, 1, 0>(i, j + 1, k, i, j, k) ? T(0) : Ay * C.f_uy(i, j + 1, k) - Ay * C.f_uy(i, j, k);
+            T sz = C.flz() || ib.template zd<1, 0, 1>(i, j, k + 1, i, j, k) ? T(0) : Az * C.f_uz(i, j, k + 1) - Az * C.f_uz(i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         C.s.Gn[0][o] = G;
@@ -55,25 +61,27 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
     {
         T G = T(0);
         if (adv) {
-            T sx = C.F_Uv(i + 1, j, k) - C.F_Uv(i, j, k);
-            T sy = C.fly() ? T(0) : C.F_Vv(i, j, k) - C.F_Vv(i, j - 1, k);
-            T sz = C.F_Wv(i, j, k + 1) - C.F_Wv(i, j, k);
+            T sx = ib.template zd<1, 1, 0>(i + 1, j, k, i, j, k) ? T(0) : C.F_Uv(i + 1, j, k) - C.F_Uv(i, j, k);
+            T sy = C.fly() || ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : C.F_Vv(i, j, k) - C.F_Vv(i, j - 1, k);
+            T sz = ib.template zd<0, 1, 1>(i, j, k + 1, i, j, k) ? T(0) : C.F_Wv(i, j, k + 1) - C.F_Wv(i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         if (C.ph.coriolis) {
             const T *u = C.s.U[0];
             const T A = C.Axc(k);
-            T a = C.flx() ? A * C.ld(u, i, j - 1, k) : T(0.5) * (A * C.ld(u, i, j - 1, k) + A * C.ld(u, i + 1, j - 1, k));
-            T b = C.flx() ? A * C.ld(u, i, j, k) : T(0.5) * (A * C.ld(u, i, j, k) + A * C.ld(u, i + 1, j, k));
+            auto q = [&](int a, int b) { return ib.template iper<1, 0, 0>(a, b, k) ? T(0) : A * C.ld(u, a, b, k); };
+            T a = C.flx() ? q(i, j - 1) : T(0.5) * (q(i, j - 1) + q(i + 1, j - 1));
+            T b = C.flx() ? q(i, j) : T(0.5) * (q(i, j) + q(i + 1, j));
             T I = C.fly() ? b : T(0.5) * (a + b);
             G -= (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
         }
-        if (C.ph.hydrostatic) G -= (C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k)) * C.g.idy;
+        if (C.ph.hydrostatic)
+            G -= (ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k)) * C.g.idy;
         if (clo) {
             const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
-            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_ffc(i + 1, j, k) * C.S12(i + 1, j, k))) - Ax * (T(-2) * (C.nu_ffc(i, j, k) * C.S12(i, j, k)));
-            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_ccc(i, j, k) * C.dy_v(i, j, k))) - Ay * (T(-2) * (C.nu_ccc(i, j - 1, k) * C.dy_v(i, j - 1, k)));
-            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_cff(i, j, k + 1) * C.S23(i, j, k + 1))) - Az * (T(-2) * (C.nu_cff(i, j, k) * C.S23(i, j, k)));
+            T sx = C.flx() || ib.template zd<1, 1, 0>(i + 1, j, k, i, j, k) ? T(0) : Ax * C.f_uy(i + 1, j, k) - Ax * C.f_uy(i, j, k);
+            T sy = C.fly() || ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : Ay * C.f_vy(i, j, k) - Ay * C.f_vy(i, j - 1, k);
+            T sz = C.flz() || ib.template zd<0, 1, 1>(i, j, k + 1, i, j, k) ? T(0) : Az * C.f_vz(i, j, k + 1) - Az * C.f_vz(i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         C.s.Gn[1][o] = G;
@@ -82,18 +90,18 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
     {
         T G = T(0);
         if (adv) {
-            T sx = C.F_Uw(i + 1, j, k) - C.F_Uw(i, j, k);
-            T sy = C.F_Vw(i, j + 1, k) - C.F_Vw(i, j, k);
-            T sz = C.flz() ? T(0) : C.F_Ww(i, j, k) - C.F_Ww(i, j, k - 1);
+            T sx = ib.template zd<1, 0, 1>(i + 1, j, k, i, j, k) ? T(0) : C.F_Uw(i + 1, j, k) - C.F_Uw(i, j, k);
+            T sy = ib.template zd<0, 1, 1>(i, j + 1, k, i, j, k) ? T(0) : C.F_Vw(i, j + 1, k) - C.F_Vw(i, j, k);
+            T sz = C.flz() || ib.template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : C.F_Ww(i, j, k) - C.F_Ww(i, j, k - 1);
             G -= C.iVf(k) * ((sx + sy) + sz);
         }
         if (!C.ph.hydrostatic && C.ph.buoyancy != OM_BUOYANCY_NONE && !C.flz())
             G += T(0.5) * (C.buoy(i, j, k - 1) + C.buoy(i, j, k));
         if (clo) {
             const T Ax = C.Axf(k), Ay = C.Ayf(k), Az = C.g.Az;
-            T sx = C.flx() ? T(0) : Ax * (T(-2) * (C.nu_fcf(i + 1, j, k) * C.S13(i + 1, j, k))) - Ax * (T(-2) * (C.nu_fcf(i, j, k) * C.S13(i, j, k)));
-            T sy = C.fly() ? T(0) : Ay * (T(-2) * (C.nu_cff(i, j + 1, k) * C.S23(i, j + 1, k))) - Ay * (T(-2) * (C.nu_cff(i, j, k) * C.S23(i, j, k)));
-            T sz = C.flz() ? T(0) : Az * (T(-2) * (C.nu_ccc(i, j, k) * C.dz_w(i, j, k))) - Az * (T(-2) * (C.nu_ccc(i, j, k - 1) * C.dz_w(i, j, k - 1)));
+            T sx = C.flx() || ib.template zd<1, 0, 1>(i + 1, j, k, i, j, k) ? T(0) : Ax * C.f_uz(i + 1, j, k) - Ax * C.f_uz(i, j, k);
+            T sy = C.fly() || ib.template zd<0, 1, 1>(i, j + 1, k, i, j, k) ? T(0) : Ay * C.f_vz(i, j + 1, k) - Ay * C.f_vz(i, j, k);
+            T sz = C.flz() || ib.template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : Az * C.f_wz(i, j, k) - Az * C.f_wz(i, j, k - 1);
             G -= C.iVf(k) * ((sx + sy) + sz);
         }
         C.s.Gn[2][o] = G;
@@ -107,16 +115,16 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
         const T *c = C.s.U[3 + n];
         T G = T(0);
         if (adv) {
-            T sx = C.flx() ? T(0) : C.F_cx(c, i + 1, j, k) - C.F_cx(c, i, j, k);
-            T sy = C.fly() ? T(0) : C.F_cy(c, i, j + 1, k) - C.F_cy(c, i, j, k);
-            T sz = C.flz() ? T(0) : C.F_cz(c, i, j, k + 1) - C.F_cz(c, i, j, k);
+            T sx = C.flx() || ib.template zd<1, 0, 0>(i + 1, j, k, i, j, k) ? T(0) : C.F_cx(c, i + 1, j, k) - C.F_cx(c, i, j, k);
+            T sy = C.fly() || ib.template zd<0, 1, 0>(i, j + 1, k, i, j, k) ? T(0) : C.F_cy(c, i, j + 1, k) - C.F_cy(c, i, j, k);
+            T sz = C.flz() || ib.template zd<0, 0, 1>(i, j, k + 1, i, j, k) ? T(0) : C.F_cz(c, i, j, k + 1) - C.F_cz(c, i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         if (clo) {
             const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
-            T sx = C.flx() ? T(0) : Ax * (-C.kap_f(n, i + 1, j, k, 1, 0, 0) * C.dxc(c, i + 1, j, k)) - Ax * (-C.kap_f(n, i, j, k, 1, 0, 0) * C.dxc(c, i, j, k));
-            T sy = C.fly() ? T(0) : Ay * (-C.kap_f(n, i, j + 1, k, 0, 1, 0) * C.dyc(c, i, j + 1, k)) - Ay * (-C.kap_f(n, i, j, k, 0, 1, 0) * C.dyc(c, i, j, k));
-            T sz = C.flz() ? T(0) : Az * (-C.kap_f(n, i, j, k + 1, 0, 0, 1) * C.dzc(c, i, j, k + 1)) - Az * (-C.kap_f(n, i, j, k, 0, 0, 1) * C.dzc(c, i, j, k));
+            T sx = C.flx() || ib.template zd<1, 0, 0>(i + 1, j, k, i, j, k) ? T(0) : Ax * C.q_x(n, c, i + 1, j, k) - Ax * C.q_x(n, c, i, j, k);
+            T sy = C.fly() || ib.template zd<0, 1, 0>(i, j + 1, k, i, j, k) ? T(0) : Ay * C.q_y(n, c, i, j + 1, k) - Ay * C.q_y(n, c, i, j, k);
+            T sz = C.flz() || ib.template zd<0, 0, 1>(i, j, k + 1, i, j, k) ? T(0) : Az * C.q_z(n, c, i, j, k + 1) - Az * C.q_z(n, c, i, j, k);
             G -= C.iVc(k) * ((sx + sy) + sz);
         }
         C.s.Gn[3 + n][o] = G;
@@ -474,15 +482,36 @@ template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T 
     const int64_t o = C.at(i, j, k);
     const T *p = C.s.pNHS;
     const T pc = p[o];
+    // on an immersed grid the gradient is a conditional δ of cell-centred pressure
+    const IB<T> ib(C.g);
+    const T gx = ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : pc - p[C.at(i - 1, j, k)];
+    const T gy = ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : pc - p[C.at(i, j - 1, k)];
+    const T gz = ib.template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : pc - p[C.at(i, j, k - 1)];
     if (phi) {   // ∂ϕ (pressure_correction.jl:82-89)
-        if (!C.flx()) C.s.U[0][o] -= (pc - p[C.at(i - 1, j, k)]) * C.g.idx;
-        if (!C.fly()) C.s.U[1][o] -= (pc - p[C.at(i, j - 1, k)]) * C.g.idy;
-        if (!C.flz()) C.s.U[2][o] -= (pc - p[C.at(i, j, k - 1)]) * C.g.idzf[k];
+        if (!C.flx()) C.s.U[0][o] -= gx * C.g.idx;
+        if (!C.fly()) C.s.U[1][o] -= gy * C.g.idy;
+        if (!C.flz()) C.s.U[2][o] -= gz * C.g.idzf[k];
         return;
     }
-    if (!C.flx()) C.s.U[0][o] -= (dtp * (pc - p[C.at(i - 1, j, k)])) * C.g.idx;
-    if (!C.fly()) C.s.U[1][o] -= (dtp * (pc - p[C.at(i, j - 1, k)])) * C.g.idy;
-    if (!C.flz()) C.s.U[2][o] -= (dtp * (pc - p[C.at(i, j, k - 1)])) * C.g.idzf[k];
+    if (!C.flx()) C.s.U[0][o] -= (dtp * gx) * C.g.idx;
+    if (!C.fly()) C.s.U[1][o] -= (dtp * gy) * C.g.idy;
+    if (!C.flz()) C.s.U[2][o] -= (dtp * gz) * C.g.idzf[k];
+}
+
+// mask_immersed_field! (mask_immersed_field.jl:65-80): zero on the immersed periphery of the
+// field's location over its interior indices (k = 0 .. Nz for w on a Bounded z)
+template <class T> __global__ void __launch_bounds__(256) k_mask(DevGrid<T> g, T *f, int loc, int nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.Nx || j >= g.Ny || k >= nk) return;
+    const IB<T> ib(g);
+    bool z;
+    if (loc == 1) z = ib.template iper<1, 0, 0>(i, j, k);
+    else if (loc == 2) z = ib.template iper<0, 1, 0>(i, j, k);
+    else if (loc == 3) z = ib.template iper<0, 0, 1>(i, j, k);
+    else z = ib.template iper<0, 0, 0>(i, j, k);
+    if (z) f[idx3(i, j, k, g.sy, g.sz)] = T(0);
 }
 
 // ∇·U at cell centres (divergence_operators.jl:16-19)
@@ -491,11 +520,7 @@ template <class T> __global__ void __launch_bounds__(256) k_divergence(Ctx<T> C,
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
-    const T *u = C.s.U[0], *v = C.s.U[1], *w = C.s.U[2];
-    T sx = C.flx() ? T(0) : C.Axc(k) * C.ld(u, i + 1, j, k) - C.Axc(k) * C.ld(u, i, j, k);
-    T sy = C.fly() ? T(0) : C.Ayc(k) * C.ld(v, i, j + 1, k) - C.Ayc(k) * C.ld(v, i, j, k);
-    T sz = C.flz() ? T(0) : C.g.Az * C.ld(w, i, j, k + 1) - C.g.Az * C.ld(w, i, j, k);
-    out[(int64_t)i + (int64_t)C.g.Nx * ((int64_t)j + (int64_t)C.g.Ny * k)] = C.iVc(k) * ((sx + sy) + sz);
+    out[(int64_t)i + (int64_t)C.g.Nx * ((int64_t)j + (int64_t)C.g.Ny * k)] = C.divU(i, j, k, i + 1, j + 1);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -525,6 +550,12 @@ template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int f
 template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
     dim3 blk(64, 4, 1);
     hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, dtp, phi);
+}
+template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st) {
+    if (!g.kbot) return;
+    const int nk = (loc == 3 && g.tz == OM_BOUNDED) ? g.Nz + 1 : g.Nz;
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_mask<T>, cell_grid(g.Nx, g.Ny, nk, blk), blk, 0, st, g, f, loc, nk);
 }
 template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t st) {
     dim3 blk(64, 4, 1);
@@ -562,6 +593,7 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
     template void launch_hydrostatic<T>(const Ctx<T> &, hipStream_t);                              \
     template void launch_rk3<T>(const Ctx<T> &, T, T, T, int, int, hipStream_t);                    \
     template void launch_correct<T>(const Ctx<T> &, T, int, hipStream_t);                               \
+    template void launch_mask<T>(const DevGrid<T> &, T *, int, hipStream_t);                           \
     template void launch_divergence<T>(const Ctx<T> &, T *, hipStream_t);                          \
     template void launch_halos<T>(const DevGrid<T> &, const HaloJob<T> &, hipStream_t);
 // Each precision is compiled in its own object: FP64 without FMA contraction (the reference's

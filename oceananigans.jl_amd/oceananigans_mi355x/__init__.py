@@ -13,6 +13,7 @@ from .physics import (  # noqa: F401
     AnisotropicMinimumDissipation,
     BuoyancyTracer,
     Centered,
+    ColumnwiseTridiagonalPreconditioner,
     ConjugateGradientPoissonSolver,
     DiagonallyDominantPreconditioner,
     FFTBasedPoissonSolver,

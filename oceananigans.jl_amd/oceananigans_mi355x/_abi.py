@@ -21,6 +21,7 @@ OM_FIELD_GN, OM_FIELD_GM = 100, 200
 OM_FIELD_PNHS, OM_FIELD_PHY, OM_FIELD_NU, OM_FIELD_KAPPA0 = 300, 301, 302, 303
 OM_SOLVER_FFT, OM_SOLVER_FOURIER_TRIDIAGONAL, OM_SOLVER_CONJUGATE_GRADIENT = 0, 1, 2
 OM_PRECOND_DEFAULT, OM_PRECOND_NONE, OM_PRECOND_DIAGONALLY_DOMINANT, OM_PRECOND_FFT = 0, 1, 2, 3
+OM_PRECOND_COLUMNWISE_TRIDIAGONAL = 4
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")

@@ -133,10 +133,19 @@ class DiagonallyDominantPreconditioner:
     """Pointwise truncated-Neumann-series preconditioner (conjugate_gradient_poisson_solver.jl:256-296)."""
 
 
+class ColumnwiseTridiagonalPreconditioner:
+    """ColumnwiseTridiagonalPreconditioner(grid): per horizontal column, the vertical part of V∇²
+    (diagonal regularised by 1 + 1/100, inactive cells as identity rows) solved exactly by one
+    Thomas sweep (conjugate_gradient_poisson_solver.jl:298-397)."""
+
+    def __init__(self, grid=None):
+        self.grid = grid
+
+
 class ConjugateGradientPoissonSolver:
     """ConjugateGradientPoissonSolver(grid; preconditioner, reltol, abstol, maxiter)
     (conjugate_gradient_poisson_solver.jl:134-163).  preconditioner: "default" (the reference's
-    choice), None, DiagonallyDominantPreconditioner(), or an FFTBasedPoissonSolver /
+    choice), None, DiagonallyDominantPreconditioner(), ColumnwiseTridiagonalPreconditioner(), or an FFTBasedPoissonSolver /
     FourierTridiagonalPoissonSolver of the grid.  reltol and abstol default to sqrt(eps(FT)),
     maxiter to Nx·Ny·Nz."""
 
@@ -156,4 +165,6 @@ class ConjugateGradientPoissonSolver:
             return _abi.OM_PRECOND_DIAGONALLY_DOMINANT
         if isinstance(p, (FFTBasedPoissonSolver, FourierTridiagonalPoissonSolver)):
             return _abi.OM_PRECOND_FFT
+        if isinstance(p, ColumnwiseTridiagonalPreconditioner):
+            return _abi.OM_PRECOND_COLUMNWISE_TRIDIAGONAL
         raise ValueError(f"unsupported preconditioner {p!r}")

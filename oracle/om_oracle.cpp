@@ -1363,6 +1363,43 @@ template <class T> static void cg_precondition_fft(Model<T> &m, Fld<T> &z, Fld<T
         z(i, j, k) = (T)a[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))].real();
 }
 
+// ColumnwiseTridiagonalPreconditioner (conjugate_gradient_poisson_solver.jl:298-397): per column the
+// vertical part of V∇² with its diagonal scaled by 1 + 1/100 (identity rows for inactive and
+// vertically isolated cells), solved by the batched Thomas sweep of batched_tridiagonal_solver.jl
+// (solve_batched_tridiagonal_system_z!: forward sweep with the 10·eps pivot guard that keeps the
+// output's previous value, then back substitution).
+template <class T> static void cg_precondition_ct(Model<T> &m, Fld<T> &z, Fld<T> &r) {
+    IBP<T> P(m);
+    const int Nx = m.Nx(), Ny = m.Ny(), Nz = m.Nz();
+    auto inact = [&](int i, int j, int k) { return P.inactive_cell(i, j, k, false); };
+    auto azf = [&](int k) { return m.Az() * (T(1) / m.g.dzF(k)); };
+    auto off = [&](int i, int j, int k) {
+        const T ap = inact(i, j, k + 1) ? T(0) : azf(k + 1);
+        return inact(i, j, k) ? T(0) : ap;
+    };
+    auto diag = [&](int i, int j, int k) {
+        const bool s = inact(i, j, k), b = inact(i, j, k - 1), a = inact(i, j, k + 1);
+        const T am = b ? T(0) : azf(k), ap = a ? T(0) : azf(k + 1);
+        const T eps = T(1) / T(100);
+        return (s || (b && a)) ? T(1) : -(am + ap) * (T(1) + eps);
+    };
+    const T guard = T(10) * std::numeric_limits<T>::epsilon();
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+        std::vector<T> t(Nz + 1);
+        T beta = diag(i, j, 1);
+        z(i, j, 1) = r(i, j, 1) / beta;
+        for (int k = 2; k <= Nz; ++k) {
+            const T c = off(i, j, k - 1), bk = diag(i, j, k), a = off(i, j, k - 1);
+            t[k] = c / beta;
+            beta = bk - a * t[k];
+            const T phis = (r(i, j, k) - a * z(i, j, k - 1)) / beta;
+            if (std::abs(beta) > guard) z(i, j, k) = phis;
+        }
+        for (int k = Nz - 1; k >= 1; --k) z(i, j, k) -= t[k + 1] * z(i, j, k + 1);
+    }
+}
+
 // the reference's default: the FFT-based solver of the underlying grid on an immersed grid, the
 // diagonally dominant preconditioner otherwise (conjugate_gradient_poisson_solver.jl:141-147)
 template <class T> static int cg_preconditioner_kind(const Model<T> &m) {
@@ -1394,6 +1431,7 @@ template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
     while (iteration < maxiter && !(res <= tol)) {
         if (pk == OM_PRECOND_DIAGONALLY_DOMINANT) cg_precondition_dd(m, z, r);
         else if (pk == OM_PRECOND_FFT) cg_precondition_fft(m, z, r);
+        else if (pk == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_precondition_ct(m, z, r);
         else interior([&](int i, int j, int k) { z(i, j, k) = r(i, j, k); });
         const T rho = cg_sum<T>(m, [&](int i, int j, int k) { return z(i, j, k) * r(i, j, k); });
         if (iteration == 0) interior([&](int i, int j, int k) { p(i, j, k) = z(i, j, k); });
@@ -1428,11 +1466,21 @@ template <class T> static void cg_solve(Model<T> &m, Fld<T> &x, Fld<T> &b) {
     interior([&](int i, int j, int k) {
         const double idx = 1.0 / (double)m.g.dx, idy = 1.0 / (double)m.g.dy;
         auto X = [&](int a, int b2, int c) { return (double)x(a, b2, c); };
+        // the same conditional differences as cg_laplacian on an immersed grid
+        IBP<T> P(m);
+        auto zc = [&](int a, int b2, int c) { return m.ib && (P.imm_inactive(i, j, k, 0, 0, 0) || P.imm_inactive(a, b2, c, 0, 0, 0)); };
+        auto zfp = [&](int a, int b2, int c, int lx, int ly, int lz) {
+            return m.ib && (P.imm_inactive(i, j, k, lx, ly, lz) || P.imm_inactive(a, b2, c, lx, ly, lz));
+        };
+        auto G = [&](bool z, double v) { return z ? 0.0 : v; };
         double lap = 0;
-        if (m.g.topo[0] != F) lap += (double)m.Ax_c(k) * ((X(i + 1, j, k) - X(i, j, k)) * idx - (X(i, j, k) - X(i - 1, j, k)) * idx);
-        if (m.g.topo[1] != F) lap += (double)m.Ay_c(k) * ((X(i, j + 1, k) - X(i, j, k)) * idy - (X(i, j, k) - X(i, j - 1, k)) * idy);
-        if (m.g.topo[2] != F)
-            lap += (double)m.Az() * ((X(i, j, k + 1) - X(i, j, k)) / (double)m.g.dzF(k + 1) - (X(i, j, k) - X(i, j, k - 1)) / (double)m.g.dzF(k));
+        if (m.g.topo[0] != F && !zfp(i + 1, j, k, 1, 0, 0))
+            lap += (double)m.Ax_c(k) * (G(zc(i + 1, j, k), (X(i + 1, j, k) - X(i, j, k)) * idx) - G(zc(i - 1, j, k), (X(i, j, k) - X(i - 1, j, k)) * idx));
+        if (m.g.topo[1] != F && !zfp(i, j + 1, k, 0, 1, 0))
+            lap += (double)m.Ay_c(k) * (G(zc(i, j + 1, k), (X(i, j + 1, k) - X(i, j, k)) * idy) - G(zc(i, j - 1, k), (X(i, j, k) - X(i, j - 1, k)) * idy));
+        if (m.g.topo[2] != F && !zfp(i, j, k + 1, 0, 0, 1))
+            lap += (double)m.Az() * (G(zc(i, j, k + 1), (X(i, j, k + 1) - X(i, j, k)) / (double)m.g.dzF(k + 1)) -
+                                     G(zc(i, j, k - 1), (X(i, j, k) - X(i, j, k - 1)) / (double)m.g.dzF(k)));
         const double e = (double)b(i, j, k) - lap;
         ev[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] = e;
         esum += e;
@@ -1727,6 +1775,26 @@ void orc_set_finalize(void *hp, int enforce_incompressibility) {
 void orc_last_solve_info(void *hp, int *iterations, double *residual, double *true_residual) {
     auto *h = (orc_model *)hp;
     DISPATCH(h, { *iterations = m->cg_iterations; *residual = m->cg_residual; *true_residual = m->cg_true_residual; });
+}
+
+// one application of the CG preconditioner `kind` (OM_PRECOND_*) to an interior residual r
+void orc_cg_precondition(void *hp, int kind, const double *r, double *z) {
+    auto *h = (orc_model *)hp;
+    DISPATCH(h, {
+        using FT = decltype(m->g.dx);
+        int N[3] = {m->Nx(), m->Ny(), m->Nz()}, H[3] = {m->g.H[0], m->g.H[1], m->g.H[2]};
+        Fld<FT> rr, zz;
+        rr.init(N, H);
+        zz.init(N, H);
+        size_t t = 0;
+        for (int k = 1; k <= N[2]; ++k) for (int j = 1; j <= N[1]; ++j) for (int i = 1; i <= N[0]; ++i) rr(i, j, k) = (FT)r[t++];
+        if (kind == OM_PRECOND_DIAGONALLY_DOMINANT) cg_precondition_dd(*m, zz, rr);
+        else if (kind == OM_PRECOND_FFT) cg_precondition_fft(*m, zz, rr);
+        else if (kind == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_precondition_ct(*m, zz, rr);
+        else zz = rr;
+        t = 0;
+        for (int k = 1; k <= N[2]; ++k) for (int j = 1; j <= N[1]; ++j) for (int i = 1; i <= N[0]; ++i) z[t++] = (double)zz(i, j, k);
+    });
 }
 
 void orc_get_clock(void *hp, double *time, int64_t *it, int *stage) {

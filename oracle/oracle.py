@@ -54,6 +54,7 @@ def lib():
         "orc_divergence": (None, [vp, pd]),
         "orc_poisson_solve": (None, [vp, pd, pd]),
         "orc_last_solve_info": (None, [vp, P(i), pd, pd]),
+        "orc_cg_precondition": (None, [vp, i, pd, pd]),
         "orc_grid_z": (None, [vp, pd, pd, pd, pd]),
         "orc_weno5": (d, [pd, i, i]),
         "orc_weno3": (d, [pd, i, i]),
@@ -195,6 +196,14 @@ class OracleModel:
         it, r, t = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         self.L.orc_last_solve_info(self.h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t))
         return it.value, r.value, t.value
+
+    def cg_precondition(self, kind, r):
+        """z = M⁻¹ r for the CG preconditioner `kind` (OM_PRECOND_*), r and z interior (Nx, Ny, Nz)."""
+        g = self.grid
+        rr = _f64(r)
+        out = np.empty_like(rr)
+        self.L.orc_cg_precondition(self.h, int(kind), _p(rr), _p(out))
+        return out.reshape((g.Nx, g.Ny, g.Nz), order="F")
 
     def grid_z(self):
         g = self.grid

@@ -25,11 +25,12 @@ def _grid(name):
 def _precond(kind, grid):
     if kind == "fft":
         return om.FourierTridiagonalPoissonSolver() if grid.stretched else om.FFTBasedPoissonSolver()
-    return {"default": "default", "none": None, "dd": om.DiagonallyDominantPreconditioner()}[kind]
+    return {"default": "default", "none": None, "dd": om.DiagonallyDominantPreconditioner(),
+            "ct": om.ColumnwiseTridiagonalPreconditioner()}[kind]
 
 
 @pytest.mark.parametrize("gname", ["PPB", "PPP", "stretched"])
-@pytest.mark.parametrize("pkind", ["default", "none", "fft"])
+@pytest.mark.parametrize("pkind", ["default", "none", "fft", "ct"])
 def test_cg_poisson_solve_matches_oracle(gname, pkind):
     from oceananigans_mi355x.synthetic import uniform
     grid = _grid(gname)
