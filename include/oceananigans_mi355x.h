@@ -196,6 +196,20 @@ int om_poisson_solve(om_model_t *m, const double *rhs_interior, double *phi_inte
 int om_last_solve_info(const om_model_t *m, int32_t *iterations, double *residual_norm, double *true_residual_norm);
 int om_divergence(om_model_t *m, double *div_interior);
 
+/* diagnostics reductions on the device
+ * om_cell_advection_timescale: cell_advection_timescale(model)
+ *   (src/Models/NonhydrostaticModels/NonhydrostaticModels.jl:93-97, src/Advection/cell_advection_timescale.jl:14-35):
+ *   the minimum over interior cells of 1 / (|u|/Δxᶠᶜᶜ + |v|/Δyᶜᶠᶜ + |w|/Δzᶜᶜᶠ) in the model's precision;
+ *   +Inf for a fluid at rest, NaN when a velocity is NaN.
+ * om_field_statistics: maximum, minimum and the number of NaNs of a field's data, over its parent
+ *   array (with_halos = 1: maximum(field.data.parent) in
+ *   src/TurbulenceClosures/turbulence_closure_diagnostics.jl:50-70; any(isnan, parent(field)) in
+ *   src/Diagnostics/nan_checker.jl:37) or its interior (with_halos = 0); NaNs are excluded from the
+ *   maximum and minimum. */
+int om_cell_advection_timescale(om_model_t *m, double *tau);
+int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
+                        int64_t *nan_count);
+
 /* timing of the last om_time_step call (HIP events on the handle's stream).
  * ms_total: the whole call.  With phase timing enabled, ms_per_phase[p] and launches_per_phase[p]
  * (p < nphases ≤ OM_NPHASES) give the summed device time and the number of timed launches of

@@ -170,6 +170,7 @@ template <class T> struct ModelT : ModelBase {
     double *cg_x = nullptr, *cg_p = nullptr, *cg_q = nullptr;   // iterate, search direction, A·p (FP64)
     T *cg_z = nullptr;                             // preconditioned residual (model precision)
     T *cg_t = nullptr;                             // Thomas-sweep ratios of the columnwise preconditioner
+    double *diag_partial = nullptr, *diag_out = nullptr, *diag_host = nullptr;   // diagnostics reductions
     double *cg_partial = nullptr, *cg_out = nullptr, *cg_host = nullptr;
     int cg_iterations = 0;
     double cg_res_last = 0, cg_true_res_last = 0;
@@ -212,6 +213,7 @@ template <class T> struct ModelT : ModelBase {
         if (st) (void)hipStreamSynchronize(st);
         for (void *p : allocs) (void)hipFree(p);
         if (cg_host) (void)hipHostFree(cg_host);
+        if (diag_host) (void)hipHostFree(diag_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -326,6 +328,9 @@ template <class T> struct ModelT : ModelBase {
         P.ly = upload(eigen<T>(g.Ny, hg.Ly, g.ty, 1));
         P.lz = upload(eigen<T>(g.Nz, hg.Lz, g.tz, g.tz == OM_BOUNDED ? 2 : 1));
         P.eps10 = T(10) * std::numeric_limits<T>::epsilon();
+        diag_partial = dalloc<double>(3 * 1024);
+        diag_out = dalloc<double>(4);
+        HIPCHECK(hipHostMalloc((void **)&diag_host, 4 * sizeof(double), hipHostMallocDefault));
         if (c.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
             cg = true;
             cg_r = field_origin(dalloc<double>(fsize)); cg_b = field_origin(dalloc<double>(fsize));
@@ -871,6 +876,34 @@ int om_last_solve_info(const om_model_t *mc, int32_t *iterations, double *residu
         if (iterations) *iterations = M->cg_iterations;
         if (residual_norm) *residual_norm = M->cg_res_last;
         if (true_residual_norm) *true_residual_norm = M->cg_true_res_last;
+    });
+}
+
+int om_cell_advection_timescale(om_model_t *m, double *tau) {
+    if (!m || !tau) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        launch_adv_timescale(M->ctx(), M->diag_partial, M->diag_out, M->st);
+        HIPCHECK(hipMemcpyAsync(M->diag_host, M->diag_out, 3 * sizeof(double), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+        if (M->diag_host[2] > 0) *tau = std::numeric_limits<double>::quiet_NaN();
+        else *tau = (double)(TT(1) / (TT)std::max(M->diag_host[0], 0.0));
+    });
+}
+
+int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
+                        int64_t *nan_count) {
+    if (!m) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        bool is_w;
+        auto *o = M->field_ptr(field, is_w);
+        if (!o) throw ArgError{"unknown field id"};
+        launch_field_stats(M->g, o, with_halos ? 1 : 0, (is_w && M->g.tz == OM_BOUNDED) ? 1 : 0, M->diag_partial, M->diag_out, M->st);
+        HIPCHECK(hipMemcpyAsync(M->diag_host, M->diag_out, 3 * sizeof(double), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+        if (maximum) *maximum = M->diag_host[0];
+        if (minimum) *minimum = M->diag_host[1];
+        if (nan_count) *nan_count = (int64_t)M->diag_host[2];
     });
 }
 

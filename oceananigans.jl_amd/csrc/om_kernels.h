@@ -570,4 +570,10 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
 template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
                                        T divisor, hipStream_t st);
 
+// diagnostics reductions (om_diagnostics.hip); partial holds 3·1024 doubles, out 3 doubles
+// (max, min, NaN count); the advection timescale reduction leaves max = max inverse timescale
+template <class T> void launch_adv_timescale(const Ctx<T> &C, double *partial, double *out, hipStream_t st);
+template <class T> void launch_field_stats(const DevGrid<T> &g, const T *origin, int with_halos, int face_z, double *partial,
+                                           double *out, hipStream_t st);
+
 }  // namespace om

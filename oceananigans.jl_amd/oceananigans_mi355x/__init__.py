@@ -30,4 +30,20 @@ from .physics import (  # noqa: F401
     WENO,
 )
 from .models import NonhydrostaticModel, build_config, set_, time_step_  # noqa: F401
-from .simulations import Simulation, run_  # noqa: F401
+from .simulations import (  # noqa: F401
+    Callback,
+    Checkpointer,
+    IterationInterval,
+    NaNChecker,
+    Simulation,
+    TimeInterval,
+    TimeStepWizard,
+    cell_advection_timescale,
+    cell_diffusion_timescale,
+    conjure_time_step_wizard_,
+    load_checkpoint_state,
+    prognostic_state,
+    restore_prognostic_state_,
+    run_,
+    set_simulation_,
+)

@@ -202,6 +202,22 @@ class Field:
         _abi.check(self.model._lib.om_get_field(self.model._h, self.id, _abi.dptr(out), 1), self.model._h)
         return out.reshape(shape, order="F")
 
+    def maximum(self):
+        """maximum(field.data.parent) (halos included), reduced on the device."""
+        return self.model.field_statistics(self.id, True)[0]
+
+    def minimum(self):
+        return self.model.field_statistics(self.id, True)[1]
+
+    def hasnan(self):
+        """any(isnan, parent(field)) (src/Diagnostics/nan_checker.jl:37)."""
+        return self.model.field_statistics(self.id, True)[2] > 0
+
+    def set_parent(self, values):
+        """copyto!(parent(field), values): the whole padded array, halos included."""
+        buf = _abi.as_fortran_f64(np.asarray(values, dtype=np.float64))
+        _abi.check(self.model._lib.om_set_field(self.model._h, self.id, _abi.dptr(buf), 1), self.model._h)
+
     def info(self):
         info = _abi.OmFieldInfo()
         _abi.check(self.model._lib.om_field_info(self.model._h, self.id, ctypes.byref(info)), self.model._h)
@@ -352,6 +368,20 @@ class NonhydrostaticModel:
         cnt = (ctypes.c_int32 * n)()
         _abi.check(self._lib.om_last_step_timing(self._h, ctypes.byref(t), ms, cnt, n), self._h)
         return {name: (ms[p], cnt[p]) for p, name in enumerate(_abi.PHASES)}
+
+    def cell_advection_timescale(self):
+        """cell_advection_timescale(model) (NonhydrostaticModels.jl:93-97): min over cells of
+        1 / (|u|/Δx + |v|/Δy + |w|/Δz), reduced on the device."""
+        t = ctypes.c_double()
+        _abi.check(self._lib.om_cell_advection_timescale(self._h, ctypes.byref(t)), self._h)
+        return t.value
+
+    def field_statistics(self, field_id, with_halos=True):
+        """(maximum, minimum, number of NaNs) of a field's parent data (or interior), on the device."""
+        mx, mn, nn = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _abi.check(self._lib.om_field_statistics(self._h, int(field_id), 1 if with_halos else 0, ctypes.byref(mx),
+                                                 ctypes.byref(mn), ctypes.byref(nn)), self._h)
+        return mx.value, mn.value, nn.value
 
     def divergence(self):
         n = self.grid.Nx * self.grid.Ny * self.grid.Nz

@@ -1700,7 +1700,7 @@ int orc_set_field(void *hp, int id, const double *src) {
         int lo[3], hi[3]; interior_extent(*m, id, lo, hi);
         size_t t = 0;
         for (int k = lo[2]; k <= hi[2]; ++k) for (int j = lo[1]; j <= hi[1]; ++j) for (int i = lo[0]; i <= hi[0]; ++i)
-            (*f)(i, j, k) = (decltype(f->d[0]))src[t++];
+            (*f)(i, j, k) = (std::decay_t<decltype(f->d[0])>)src[t++];
     });
     return rc;
 }
@@ -1740,7 +1740,7 @@ int orc_set_field_range(void *hp, int id, const int lo[3], const int hi[3], cons
         if (!f) { rc = -1; break; }
         size_t t = 0;
         for (int k = lo[2]; k <= hi[2]; ++k) for (int j = lo[1]; j <= hi[1]; ++j) for (int i = lo[0]; i <= hi[0]; ++i)
-            (*f)(i, j, k) = (decltype(f->d[0]))src[t++];
+            (*f)(i, j, k) = (std::decay_t<decltype(f->d[0])>)src[t++];
     });
     return rc;
 }
@@ -1775,6 +1775,28 @@ void orc_set_finalize(void *hp, int enforce_incompressibility) {
 void orc_last_solve_info(void *hp, int *iterations, double *residual, double *true_residual) {
     auto *h = (orc_model *)hp;
     DISPATCH(h, { *iterations = m->cg_iterations; *residual = m->cg_residual; *true_residual = m->cg_true_residual; });
+}
+
+// cell_advection_timescale(model) (cell_advection_timescale.jl:14-35): minimum over interior cells
+// of 1 / (|u|Δx⁻¹ + |v|Δy⁻¹ + |w|Δz⁻¹ᶜᶜᶠ) in FT, a Flat direction contributing 0
+double orc_cell_advection_timescale(void *hp) {
+    auto *h = (orc_model *)hp;
+    double out = 0;
+    DISPATCH(h, {
+        using FT = decltype(m->g.dx);
+        FT tau = std::numeric_limits<FT>::infinity();
+        bool nan = false;
+        for (int k = 1; k <= m->Nz(); ++k) for (int j = 1; j <= m->Ny(); ++j) for (int i = 1; i <= m->Nx(); ++i) {
+            const FT ix = m->g.topo[0] == F ? FT(0) : std::abs(m->U[0](i, j, k)) * (FT(1) / m->g.dx);
+            const FT iy = m->g.topo[1] == F ? FT(0) : std::abs(m->U[1](i, j, k)) * (FT(1) / m->g.dy);
+            const FT iz = m->g.topo[2] == F ? FT(0) : std::abs(m->U[2](i, j, k)) * (FT(1) / m->g.dzF(k));
+            const FT t = FT(1) / ((ix + iy) + iz);
+            if (t != t) nan = true;
+            else tau = std::min(tau, t);
+        }
+        out = nan ? std::numeric_limits<double>::quiet_NaN() : (double)tau;
+    });
+    return out;
 }
 
 // one application of the CG preconditioner `kind` (OM_PRECOND_*) to an interior residual r

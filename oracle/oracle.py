@@ -55,6 +55,7 @@ def lib():
         "orc_poisson_solve": (None, [vp, pd, pd]),
         "orc_last_solve_info": (None, [vp, P(i), pd, pd]),
         "orc_cg_precondition": (None, [vp, i, pd, pd]),
+        "orc_cell_advection_timescale": (d, [vp]),
         "orc_grid_z": (None, [vp, pd, pd, pd, pd]),
         "orc_weno5": (d, [pd, i, i]),
         "orc_weno3": (d, [pd, i, i]),
@@ -196,6 +197,9 @@ class OracleModel:
         it, r, t = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         self.L.orc_last_solve_info(self.h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t))
         return it.value, r.value, t.value
+
+    def cell_advection_timescale(self):
+        return self.L.orc_cell_advection_timescale(self.h)
 
     def cg_precondition(self, kind, r):
         """z = M⁻¹ r for the CG preconditioner `kind` (OM_PRECOND_*), r and z interior (Nx, Ny, Nz)."""

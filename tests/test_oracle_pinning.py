@@ -718,3 +718,39 @@ def test_immersed_projection_and_tracer_conservation():
     assert np.abs(div[~imm]).max() <= 1e-10 * np.abs(u).max() / (grid.extent[0] / grid.Nx)
     total = (V * orc.get(3))[~imm].sum()
     assert abs(total - total0) <= 1e-12 * abs(total0)
+
+
+# ---------------------------------------------------------------------------------------------
+# cell_advection_timescale (src/Advection/cell_advection_timescale.jl:14-35)
+# ---------------------------------------------------------------------------------------------
+def test_cell_advection_timescale_closed_forms():
+    """Uniform (U, V, W): τ = 1/(|U|/Δx + |V|/Δy + |W|/Δz) (w vanishes on the bottom wall, which
+    only removes one layer of cells from the minimum); on a stretched grid the thinnest face
+    spacing sets τ; a fluid at rest gives +Inf; Flat directions contribute nothing."""
+    grid = om.RectilinearGrid(size=(4, 5, 6), x=(0, 2), y=(0, 5), z=(-3, 0))
+    U, V, W = 0.3, -0.2, 0.05
+    orc = oracle_for(grid, {}, {})
+    for q, val in enumerate((U, V, W)):
+        orc.set(q, np.full(om.models.interior_shape(grid, q), val))
+    expect = 1.0 / (abs(U) / 0.5 + abs(V) / 1.0 + abs(W) / 0.5)
+    assert orc.cell_advection_timescale() == pytest.approx(expect, rel=1e-15)
+    rest = oracle_for(grid, {}, {})
+    assert rest.cell_advection_timescale() == np.inf
+    sgrid = om.RectilinearGrid(size=(4, 4, 4), x=(0, 1), y=(0, 1), z=[-4, -2, -1, -0.25, 0])
+    so = oracle_for(sgrid, {}, {})
+    so.set(2, np.full(om.models.interior_shape(sgrid, 2), 1.0))
+    dzf = so.grid_z()[1][sgrid.H[2]:sgrid.H[2] + 5]            # face spacings, faces 1..5
+    assert so.cell_advection_timescale() == pytest.approx(dzf[0:4].min(), rel=1e-15)
+    g2 = om.RectilinearGrid(size=(8, 8), x=(0, 1), y=(0, 2), topology=(om.Periodic, om.Periodic, om.Flat))
+    o2 = oracle_for(g2, {"advection": om.WENO()}, {"u": 2.0, "v": 1.0})
+    assert o2.cell_advection_timescale() == pytest.approx(1.0 / (2.0 / 0.125 + 1.0 / 0.25), rel=1e-15)
+
+
+def test_fp32_oracle_field_round_trip():
+    """An FP32 oracle stores set! values rounded to FP32 and returns them exactly."""
+    grid = om.RectilinearGrid(float_type=np.float32, size=(4, 3, 5), extent=(1, 1, 1))
+    cfg, keep = om.build_config(grid)
+    o = O.OracleModel(cfg, grid)
+    a = np.random.default_rng(4).normal(size=(4, 3, 5))
+    o.set(3 - 3, a)
+    np.testing.assert_array_equal(o.get(0), a.astype(np.float32).astype(np.float64))
