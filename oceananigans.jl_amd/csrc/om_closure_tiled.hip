@@ -40,9 +40,13 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
     const DevPhysics<T> &ph = C.ph;
     const int tid = threadIdx.x;
     const int a = tid & (CT - 1), b = tid >> 4;
-    const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+    // work item → (z chunk, tile): the chunks of one tile are consecutive, tiles in row order
+    const int item = xcd_work_item((int)blockIdx.x, (int)gridDim.x);
+    const int nch = (g.Nz + kchunk - 1) / kchunk;
+    const int tile = item / nch, chunk = item - tile * nch;
+    const int tx = tile % ntx, ty = tile / ntx;
     const int i0 = tx * CT, j0 = ty * CT;
-    const int k0 = blockIdx.y * kchunk;
+    const int k0 = chunk * kchunk;
     const int k1 = min(k0 + kchunk, g.Nz);
     const int i = i0 + a, j = j0 + b;
     const bool colok = i <= g.Nx + g.Hx - 1 && j <= g.Ny + g.Hy - 1;
@@ -232,7 +236,7 @@ template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStr
     const int ntx = (g.Nx + CT - 1) / CT, nty = (g.Ny + CT - 1) / CT;
     if (kchunk < 1) kchunk = 1;
     const int nch = (g.Nz + kchunk - 1) / kchunk;
-    const dim3 grid(ntx * nty, nch), blk(CT * CT);
+    const dim3 grid(ntx * nty * nch), blk(CT * CT);
     switch (C.ph.n_tracers) {
         case 0: hipLaunchKernelGGL((k_closure_tiled<T, 0>), grid, blk, 0, st, C, kchunk, ntx); break;
         case 1: hipLaunchKernelGGL((k_closure_tiled<T, 1>), grid, blk, 0, st, C, kchunk, ntx); break;

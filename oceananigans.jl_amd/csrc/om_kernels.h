@@ -89,6 +89,37 @@ template <class T> __device__ __forceinline__ T weno5(T s0, T s1, T s2, T s3, T 
     return (al0 * sinv) * pa + (al1 * sinv) * pb + (al2 * sinv) * pc;
 }
 
+// FP32 WENO(order=5): the same stencils, smoothness indicators on central-value-shifted values
+// and Z weights, evaluated with fused multiply-adds and hardware reciprocals (≤ 1 ulp), and the
+// weighted sum normalised once: Σ αₖ pₖ / Σ αₖ.  The reference's FP32 path rounds differently at
+// the level of single ulps; FP32 results are held to the FP64 oracle within a stated tolerance,
+// never bitwise, so the kernel spends its VALU budget on fewer instructions instead.
+template <> __device__ __forceinline__ float weno5<float>(float s0, float s1, float s2, float s3, float s4, float s5, bool left) {
+    typedef RC<float> K;
+    const float a0 = left ? s2 : s3, a1 = left ? s3 : s2, a2 = left ? s4 : s1;
+    const float b0 = left ? s1 : s4, b1 = left ? s2 : s3, b2 = left ? s3 : s2;
+    const float c0 = left ? s0 : s5, c1 = left ? s1 : s4, c2 = left ? s2 : s3;
+    // β with the middle value subtracted (weno_interpolants.jl:276-281): x(k₀x + k₂z) + k₅z²
+    const float xa = a0 - a1, za = a2 - a1, xb = b0 - b1, zb = b2 - b1, xc = c0 - c1, zc = c2 - c1;
+    const float ba = fmaf(xa, fmaf(10.0f, xa, 11.0f * za), 4.0f * za * za);
+    const float bb = fmaf(xb, fmaf(4.0f, xb, 5.0f * zb), 4.0f * zb * zb);
+    const float bc = fmaf(xc, fmaf(4.0f, xc, 11.0f * zc), 10.0f * zc * zc);
+    const float tau = fabsf(ba - bc);
+    const float qa = tau * __builtin_amdgcn_rcpf(ba + K::eps);
+    const float qb = tau * __builtin_amdgcn_rcpf(bb + K::eps);
+    const float qc = tau * __builtin_amdgcn_rcpf(bc + K::eps);
+    const float al0 = K::C50 * fmaf(qa, qa, 1.0f), al1 = K::C51 * fmaf(qb, qb, 1.0f), al2 = K::C52 * fmaf(qc, qc, 1.0f);
+    const float pa = fmaf(K::a0, a0, fmaf(K::a1, a1, K::a2 * a2));
+    const float pb = fmaf(K::b0, b0, fmaf(K::b1, b1, K::b2 * b2));
+    const float pc = fmaf(K::c0, c0, fmaf(K::c1, c1, K::c2 * c2));
+    const float num = fmaf(al0, pa, fmaf(al1, pb, al2 * pc));
+    return num * __builtin_amdgcn_rcpf((al0 + al1) + al2);
+}
+
+// 1 / x: IEEE division in FP64 (the reference's rounding); the hardware reciprocal in FP32
+template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <class T> __device__ __forceinline__ T beta2_q(T a, T b) {
 #pragma clang fp contract(off)
     return a * (T(1) * a + T(-2) * b) + b * (T(1) * b);
@@ -111,7 +142,7 @@ template <class T> __device__ __forceinline__ T weno3(T s0, T s1, T s2, T s3, bo
     const T tau = fabs(ba - bb);
     const T qa = wdiv<T>(tau, ba + K::eps), qb = wdiv<T>(tau, bb + K::eps);
     const T al0 = K::C30 * (T(1) + qa * qa), al1 = K::C31 * (T(1) + qb * qb);
-    const T sinv = T(1) / (al0 + al1);
+    const T sinv = recip<T>(al0 + al1);
     return (al0 * sinv) * (K::d0 * a0 + K::d1 * a1) + (al1 * sinv) * (K::e0 * b0 + K::e1 * b1);
 }
 
@@ -125,6 +156,16 @@ template <class T> __device__ __forceinline__ T cen4(T m2, T m1, T z0, T p1) {
 template <class T> __device__ __forceinline__ T cen2(T m1, T z0) {
     typedef RC<T> K;
     return K::h_m1 * m1 + K::h_0 * z0;
+}
+
+// XCD-aware work order for a 1-D grid of nb blocks.  The dispatcher hands consecutive block ids to
+// the 8 XCDs in turn, and each XCD has its own L2; this maps the blocks of one XCD onto one
+// contiguous run of work items, so that tiles sharing halo planes share an L2.  A bijection on
+// [0, nb): the first 8·⌊nb/8⌋ ids are redistributed, the remainder keeps its own id.
+__device__ __forceinline__ int xcd_work_item(int id, int nb) {
+    const int per = nb >> 3;
+    if (id >= (per << 3)) return id;
+    return (id & 7) * per + (id >> 3);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -223,8 +264,8 @@ template <class T> struct Ctx {
     __device__ __forceinline__ T Axf(int k) const { return g.dy * g.dzf[k]; }
     __device__ __forceinline__ T Ayc(int k) const { return g.dx * g.dzc[k]; }
     __device__ __forceinline__ T Ayf(int k) const { return g.dx * g.dzf[k]; }
-    __device__ __forceinline__ T iVc(int k) const { return T(1) / (g.Az * g.dzc[k]); }
-    __device__ __forceinline__ T iVf(int k) const { return T(1) / (g.Az * g.dzf[k]); }
+    __device__ __forceinline__ T iVc(int k) const { return recip<T>(g.Az * g.dzc[k]); }
+    __device__ __forceinline__ T iVf(int k) const { return recip<T>(g.Az * g.dzf[k]); }
 
     // z boundary switching: 0 = 5th-order (or Centered4), 1 = 3rd-order, 2 = Centered2
     __device__ __forceinline__ int zb_face(int k) const {

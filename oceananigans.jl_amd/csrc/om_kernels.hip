@@ -44,7 +44,7 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
             T a = C.flx() ? q(i, j) : T(0.5) * (q(i - 1, j) + q(i, j));
             T b = C.flx() ? q(i, j + 1) : T(0.5) * (q(i - 1, j + 1) + q(i, j + 1));
             T I = C.fly() ? a : T(0.5) * (a + b);
-            G += (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
+            G += (T(0.5) * (C.ph.f + C.ph.f)) * I * recip<T>(A);
         }
         if (C.ph.hydrostatic)
             G -= (ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k)) * C.g.idx;
@@ -73,7 +73,7 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
             T a = C.flx() ? q(i, j - 1) : T(0.5) * (q(i, j - 1) + q(i + 1, j - 1));
             T b = C.flx() ? q(i, j) : T(0.5) * (q(i, j) + q(i + 1, j));
             T I = C.fly() ? b : T(0.5) * (a + b);
-            G -= (T(0.5) * (C.ph.f + C.ph.f)) * I * (T(1) / A);
+            G -= (T(0.5) * (C.ph.f + C.ph.f)) * I * recip<T>(A);
         }
         if (C.ph.hydrostatic)
             G -= (ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k)) * C.g.idy;

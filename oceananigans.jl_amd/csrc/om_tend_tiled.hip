@@ -66,16 +66,24 @@ template <class T> __device__ __forceinline__ void stage_small(T *dst, const T *
     }
 }
 
+// Occupancy: with one tracer the FP32 kernel fits 96 VGPRs without spilling, i.e. 5 waves per
+// SIMD (5 blocks per CU, 149 KiB of LDS); the hint asks the compiler for that budget.
+template <class T, int NT> constexpr int tend_min_blocks() { return (sizeof(T) == 4 && NT <= 1) ? 5 : 1; }
+
 template <class T, int NT>
-__global__ void __launch_bounds__(256) k_tend_tiled(Ctx<T> C, int kchunk, int ntx) {
+__global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(Ctx<T> C, int kchunk, int ntx) {
     __shared__ TileSmem<T, NT> S;
     const DevGrid<T> &g = C.g;
     const DevPhysics<T> &ph = C.ph;
     const int tid = threadIdx.x;
     const int a = tid & (TT - 1), b = tid >> 4;
-    const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+    // work item → (z chunk, tile): the chunks of one tile are consecutive, tiles in row order
+    const int item = xcd_work_item((int)blockIdx.x, (int)gridDim.x);
+    const int nch = (g.Nz + kchunk - 1) / kchunk;
+    const int tile = item / nch, chunk = item - tile * nch;
+    const int tx = tile % ntx, ty = tile / ntx;
     const int i0 = tx * TC, j0 = ty * TC;
-    const int k0 = blockIdx.y * kchunk;
+    const int k0 = chunk * kchunk;
     const int k1 = min(k0 + kchunk, g.Nz);
     const int i = i0 + a, j = j0 + b;
     const bool colok = i <= g.Nx + g.Hx - 1 && j <= g.Ny + g.Hy - 1;
@@ -304,7 +312,7 @@ __global__ void __launch_bounds__(256) k_tend_tiled(Ctx<T> C, int kchunk, int nt
 
         if (cell) {
             const int64_t o = col + (int64_t)kk * g.sz;
-            const T iVc = T(1) / (g.Az * g.dzc[kk]), iVf = T(1) / (g.Az * g.dzf[kk]);
+            const T iVc = recip<T>(g.Az * g.dzc[kk]), iVf = recip<T>(g.Az * g.dzf[kk]);
             const int sx0 = b * TT + a, sx1 = sx0 + 1, sy1 = sx0 + TT;
             auto diffs = [&](int q, T &sx, T &sy, T &sz, int part) {
                 sx = S.fx[2 * q + part][sx1] - S.fx[2 * q + part][sx0];
@@ -321,7 +329,7 @@ __global__ void __launch_bounds__(256) k_tend_tiled(Ctx<T> C, int kchunk, int nt
                     const T ca = half * (A * P(V, -1, 0) + A * P(V, 0, 0));
                     const T cb = half * (A * P(V, -1, 1) + A * P(V, 0, 1));
                     const T I = half * (ca + cb);
-                    G += (half * (ph.f + ph.f)) * I * (T(1) / A);
+                    G += (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
                 if (ph.hydrostatic) G -= (C.s.pHY[o] - C.s.pHY[o - 1]) * g.idx;
                 if (has_clo) { diffs(0, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
@@ -336,7 +344,7 @@ __global__ void __launch_bounds__(256) k_tend_tiled(Ctx<T> C, int kchunk, int nt
                     const T ca = half * (A * P(U, 0, -1) + A * P(U, 1, -1));
                     const T cb = half * (A * P(U, 0, 0) + A * P(U, 1, 0));
                     const T I = half * (ca + cb);
-                    G -= (half * (ph.f + ph.f)) * I * (T(1) / A);
+                    G -= (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
                 if (ph.hydrostatic) G -= (C.s.pHY[o] - C.s.pHY[o - g.sy]) * g.idy;
                 if (has_clo) { diffs(1, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
@@ -391,7 +399,7 @@ template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream
     const int ntx = (g.Nx + TC - 1) / TC, nty = (g.Ny + TC - 1) / TC;
     if (kchunk < 1) kchunk = 1;
     const int nch = (g.Nz + kchunk - 1) / kchunk;
-    const dim3 grid(ntx * nty, nch), blk(256);
+    const dim3 grid(ntx * nty * nch), blk(256);
     switch (C.ph.n_tracers) {
         case 0: hipLaunchKernelGGL((k_tend_tiled<T, 0>), grid, blk, 0, st, C, kchunk, ntx); break;
         case 1: hipLaunchKernelGGL((k_tend_tiled<T, 1>), grid, blk, 0, st, C, kchunk, ntx); break;
