@@ -97,6 +97,11 @@ extern "C" {
 #define OM_PRECOND_FFT                  3  /* FFT-based solver of the underlying grid (Fourier–tridiagonal on stretched z) */
 #define OM_PRECOND_COLUMNWISE_TRIDIAGONAL 4 /* ColumnwiseTridiagonalPreconditioner: per-column vertical part of V∇²,
                                                conjugate_gradient_poisson_solver.jl:298-397 */
+#define OM_PRECOND_LEARNED              5  /* the FFT preconditioner plus a learned convolutional correction,
+                                              in flexible CG with an FFT fallback (no reference counterpart;
+                                              it replaces the preconditioner call of
+                                              conjugate_gradient_poisson_solver.jl:169-211); Nx a multiple
+                                              of 16, Bounded z */
 
 #define OM_MAX_TRACERS 8
 #define OM_MAX_FIELDS  (3 + OM_MAX_TRACERS)
@@ -207,6 +212,35 @@ int om_divergence(om_model_t *m, double *div_interior);
  *   src/Diagnostics/nan_checker.jl:37) or its interior (with_halos = 0); NaNs are excluded from the
  *   maximum and minimum. */
 int om_cell_advection_timescale(om_model_t *m, double *tau);
+
+/* learned preconditioner (no reference counterpart; see OM_PRECOND_LEARNED).  The network: four
+ * 3×3×3 convolutions of 16 channels (ReLU after the first three), BF16 weights and activations,
+ * FP32 accumulation on the matrix cores.  Input channels per cell: r V⁻¹ / rms, z_FFT / rms,
+ * active-cell mask; output: the correction to z_FFT in units of rms(z_FFT).
+ * Weights: OM_LEARNED_NWEIGHTS FP32 values, layout [layer][tap][cin][cout], tap = (dz+1)·9 +
+ * (dy+1)·3 + (dx+1).  Host arrays of the test hooks are dense, x fastest, channels innermost.
+ *   om_learned_set_weights / get_weights   the FP32 master weights (setting them resets Adam)
+ *   om_learned_log       log (input, target) pairs from the first `iterations_per_solve`
+ *                        iterations of every later FFT- or learned-preconditioned CG solve, into a
+ *                        ring of `capacity` pairs on the device; target = (x_final − x_k)/rms − ẑ_k
+ *   om_learned_pairs     number of pairs held
+ *   om_learned_train     `steps` Adam steps (lr) over the held pairs, cycling; loss of the first
+ *                        and of the last step (mean square over active cells)
+ *   om_learned_forward   network output for a given 3-channel input (test hook)
+ *   om_learned_backward  loss, weight gradients and input gradient (3 channels) for a given
+ *                        input, target and mask (test hook)
+ *   om_learned_last_solve  iterations that used the network in the last solve, and whether the
+ *                        solve handed over to the FFT preconditioner */
+#define OM_LEARNED_NWEIGHTS (4 * 27 * 16 * 16)
+int om_learned_set_weights(om_model_t *m, const float *weights);
+int om_learned_get_weights(om_model_t *m, float *weights);
+int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity);
+int om_learned_pairs(om_model_t *m, int32_t *count);
+int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first, double *loss_last);
+int om_learned_forward(om_model_t *m, const float *input3, float *output);
+int om_learned_backward(om_model_t *m, const float *input3, const float *target, const float *mask, double *loss,
+                        float *dweights, float *dinput3);
+int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *fell_back);
 int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
                         int64_t *nan_count);
 

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liboceananigans_mi355x.so")
 SOURCES = ["om_api.cpp", "om_kernels.hip", "om_tend_tiled.hip", "om_fft.hip", "om_cg.hip", "om_closure_tiled.hip",
-           "om_diagnostics.hip"]
+           "om_diagnostics.hip", "om_learned.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

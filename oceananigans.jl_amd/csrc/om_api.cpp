@@ -20,6 +20,7 @@
 
 #include "om_internal.h"
 #include "om_kernels.h"
+#include "om_learned.h"
 
 using namespace om;
 
@@ -174,6 +175,17 @@ template <class T> struct ModelT : ModelBase {
     double *cg_partial = nullptr, *cg_out = nullptr, *cg_host = nullptr;
     int cg_iterations = 0;
     double cg_res_last = 0, cg_true_res_last = 0;
+    // learned correction to the FFT preconditioner (OM_PRECOND_LEARNED) and its training data
+    NetState net;
+    float *ln_in3 = nullptr, *ln_maskd = nullptr;  // current network input (3 channels, dense), active mask
+    double ln_nactive = 0;
+    double *cg_rprev = nullptr;                    // r of the previous iteration (flexible CG)
+    int ln_learned_its = 0, ln_fell_back = 0;
+    int log_iters = 0, log_cap = 0, log_count = 0, log_next = 0;   // pair logging (om_learned_log)
+    std::vector<float *> pair_in3, pair_tgt;       // logged pairs (device, dense)
+    std::vector<double *> snap_x;                  // iterates of the logged iterations of the current solve
+    std::vector<float *> snap_in3;
+    std::vector<double> snap_isz;
     // launch-shape choices (om_set_tuning)
     int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
     int tune_kchunk = 16;              // z levels per block of the tiled kernel
@@ -226,6 +238,10 @@ template <class T> struct ModelT : ModelBase {
         HIPCHECK(hipMemset(p, 0, n * sizeof(U) + 256));
         allocs.push_back(p);
         return reinterpret_cast<U *>(p);
+    }
+    void dfree(void *p) {
+        for (size_t q = 0; q < allocs.size(); ++q)
+            if (allocs[q] == p) { HIPCHECK(hipFree(p)); allocs.erase(allocs.begin() + q); return; }
     }
     template <class U> const U *upload(const std::vector<U> &h) {
         U *d = dalloc<U>(h.size());
@@ -336,9 +352,13 @@ template <class T> struct ModelT : ModelBase {
             cg_r = field_origin(dalloc<double>(fsize)); cg_b = field_origin(dalloc<double>(fsize));
             cg_x = field_origin(dalloc<double>(fsize)); cg_p = field_origin(dalloc<double>(fsize));
             cg_q = field_origin(dalloc<double>(fsize)); cg_z = field_origin(dalloc<T>(fsize));
-            cg_partial = dalloc<double>(2 * 1024);
+            cg_partial = dalloc<double>(4 * 1024);
             cg_out = dalloc<double>(4);
             if (cg_precond() == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_t = dalloc<T>((size_t)g.Nx * g.Ny * g.Nz);
+            if (cg_precond() == OM_PRECOND_LEARNED) {
+                cg_rprev = field_origin(dalloc<double>(fsize));
+                net_alloc();
+            }
             HIPCHECK(hipHostMalloc((void **)&cg_host, 4 * sizeof(double), hipHostMallocDefault));
         }
         HIPCHECK(hipDeviceSynchronize());
@@ -468,32 +488,149 @@ template <class T> struct ModelT : ModelBase {
         if (pk == OM_PRECOND_DEFAULT) return g.kbot ? OM_PRECOND_FFT : OM_PRECOND_DIAGONALLY_DOMINANT;
         return pk;
     }
+    // ---- learned preconditioner: network buffers (allocated on first use; zero weights give a
+    // zero correction, i.e. the FFT preconditioner)
+    void net_alloc() {
+        if (net.w) return;
+        if (g.Nx % 16 != 0) throw ArgError{"the learned preconditioner needs Nx to be a multiple of 16"};
+        if (g.tz != OM_BOUNDED) throw ArgError{"the learned preconditioner needs a Bounded z"};
+        net.d = NetDims{g.Nx, g.Ny, g.Nz};
+        const size_t pv = (size_t)(g.Nx + 2) * (g.Ny + 2) * (g.Nz + 2) * 16;
+        for (int l = 0; l < NET_LAYERS; ++l) net.act[l] = dalloc<uint16_t>(pv);
+        net.g[0] = dalloc<uint16_t>(pv);
+        net.g[1] = dalloc<uint16_t>(pv);
+        net.w = dalloc<float>(NET_LAYERS * NET_WPL);
+        net.m = dalloc<float>(NET_LAYERS * NET_WPL);
+        net.v = dalloc<float>(NET_LAYERS * NET_WPL);
+        net.dw = dalloc<float>(NET_LAYERS * NET_WPL);
+        net.wf = dalloc<uint16_t>(NET_LAYERS * 14 * 64 * 8);
+        net.wb = dalloc<uint16_t>(NET_LAYERS * 14 * 64 * 8);
+        const size_t nc = (size_t)g.Nx * g.Ny * g.Nz;
+        net.out = dalloc<float>(nc);
+        net.wpartial = dalloc<float>((size_t)net.nchunk * 27 * 256);
+        net.lpartial = dalloc<double>(NET_RED_BLOCKS);
+        net.lsum = dalloc<double>(1);
+        ln_in3 = dalloc<float>(3 * nc);
+        ln_maskd = dalloc<float>(nc);
+        ln_mask(g, ln_maskd, st);
+        net_pack(net, st);
+        std::vector<float> hm(nc);
+        HIPCHECK(hipMemcpyAsync(hm.data(), ln_maskd, nc * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        ln_nactive = 0;
+        for (float v : hm) ln_nactive += v;
+    }
+    // scales s_r = rms(r V⁻¹), s_z = rms(z) over the active cells and the network input for (r, z)
+    bool ln_prepare(double &isz) {
+        ln_scales(g, cg_r, cg_z, cg_partial, cg_out, st);
+        cg_read(3);
+        const double na = std::max(cg_host[2], 1.0);
+        const double sr = std::sqrt(cg_host[0] / na), sz = std::sqrt(cg_host[1] / na);
+        if (!(sr > 0) || !(sz > 0)) return false;
+        isz = 1.0 / sz;
+        ln_input(g, cg_r, cg_z, 1.0 / sr, isz, ln_in3, st);
+        return true;
+    }
+    void ln_log_snapshot(int k, double isz) {
+        const size_t nc = (size_t)g.Nx * g.Ny * g.Nz;
+        while ((int)snap_x.size() <= k) {
+            snap_x.push_back(field_origin(dalloc<double>(fsize)));
+            snap_in3.push_back(dalloc<float>(3 * nc));
+            snap_isz.push_back(0);
+        }
+        cg_copy_d(g, cg_x, snap_x[k], st);
+        HIPCHECK(hipMemcpyAsync(snap_in3[k], ln_in3, 3 * nc * sizeof(float), hipMemcpyDeviceToDevice, st));
+        snap_isz[k] = isz;
+    }
+    // after a converged solve: pairs (input of iteration k, (x_final − x_k)/s_z − ẑ_k with its mean removed)
+    void ln_log_pairs(int nlogged) {
+        const size_t nc = (size_t)g.Nx * g.Ny * g.Nz;
+        for (int k = 0; k < nlogged; ++k) {
+            if ((int)pair_in3.size() < log_cap) {
+                pair_in3.push_back(dalloc<float>(3 * nc));
+                pair_tgt.push_back(dalloc<float>(nc));
+            }
+            const int slot = log_next;
+            log_next = (log_next + 1) % log_cap;
+            log_count = std::min(log_count + 1, log_cap);
+            HIPCHECK(hipMemcpyAsync(pair_in3[slot], snap_in3[k], 3 * nc * sizeof(float), hipMemcpyDeviceToDevice, st));
+            ln_target(g, cg_x, snap_x[k], snap_in3[k], snap_isz[k], pair_tgt[slot], cg_partial, cg_out, st);
+            cg_read(2);
+            const float mean = (float)(cg_host[0] / std::max(cg_host[1], 1.0));
+            ln_center(g, pair_tgt[slot], mean, st);
+        }
+    }
+
     void cg_solve() {
+        const int pk = cg_precond();
+        cg_solve_with(pk);
+        // the recomputed FP64 residual judges the learned preconditioner: a solve it leaves above
+        // ten times the tolerance is finished by FFT-preconditioned CG from where it stopped
+        if (pk == OM_PRECOND_LEARNED && cg_true_res_last > 10.0 * std::max(cg_tol_last, 1e-300)) {
+            const int it0 = cg_iterations;
+            ln_fell_back = 1;
+            cg_solve_with(OM_PRECOND_FFT);
+            cg_iterations += it0;
+        }
+    }
+    double cg_tol_last = 0;
+    void cg_solve_with(const int pk) {
         double *x = cg_x;
         const double seps = std::sqrt((double)std::numeric_limits<T>::epsilon());
         const double reltol = cfg.cg_reltol < 0 ? (double)(T)seps : cfg.cg_reltol;
         const double abstol = cfg.cg_abstol < 0 ? (double)(T)seps : cfg.cg_abstol;
         const int64_t maxiter = cfg.cg_maxiter > 0 ? cfg.cg_maxiter : (int64_t)g.Nx * g.Ny * g.Nz;
         const double ncell = (double)g.Nx * (double)g.Ny * (double)g.Nz;
-        const int pk = cg_precond();
         cg_halo_r(g, x, st);
         cg_residual(g, x, cg_b, cg_r, cg_partial, cg_out, st);
         const double res0 = std::sqrt(cg_read(1));
         const double tol = std::max(reltol * res0, abstol);
+        cg_tol_last = tol;
         cg_gauge_norm(g, x, cg_r, 0.0, 0.0, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
         double res = std::sqrt(cg_read(1));
         int64_t it = 0;
         double rho_prev = 0;
+        // flexible CG for the learned preconditioner (a nonlinear map): β = zₖ·(rₖ − rₖ₋₁) / (zₖ₋₁·rₖ₋₁);
+        // three iterations in a row that do not bring ‖V⁻¹r‖ below 0.999 of its best value so far hand
+        // over to the FFT preconditioner for the rest of the solve
+        const bool flex = pk == OM_PRECOND_LEARNED;
+        bool fallback = false;
+        int stall = 0, nlogged = 0;
+        double res_best = res;
+        const bool logging = log_iters > 0 && (pk == OM_PRECOND_FFT || pk == OM_PRECOND_LEARNED) && net.w;
+        if (flex) { ln_learned_its = 0; ln_fell_back = 0; }
         while (it < maxiter && !(res <= tol)) {
             if (pk == OM_PRECOND_DIAGONALLY_DOMINANT) { cg_halo_r(g, cg_r, st); cg_dd(g, cg_r, cg_z, st); }
-            else if (pk == OM_PRECOND_FFT) {
+            else if (pk == OM_PRECOND_FFT || pk == OM_PRECOND_LEARNED) {
                 cg_fft_src(g, cg_r, rhs_dense, st);
                 launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);
+                const bool log_this = logging && it < log_iters;
+                const bool learn_this = pk == OM_PRECOND_LEARNED && !fallback;
+                double isz = 0;
+                if ((log_this || learn_this) && ln_prepare(isz)) {
+                    if (log_this) { ln_log_snapshot((int)it, isz); nlogged = (int)it + 1; }
+                    if (learn_this) {
+                        net_input_dense(net, ln_in3, st);
+                        net_forward(net, net.out, st);
+                        ln_correct(g, cg_z, net.out, 1.0 / isz, st);
+                        ++ln_learned_its;
+                    }
+                }
             } else if (pk == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_ct(g, cg_r, cg_z, cg_t, st);
             else cg_copy_r(g, cg_r, cg_z, st);
-            cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out, st);
-            const double rho = cg_read(1);
-            cg_update_p(g, cg_p, cg_z, it == 0 ? 0.0 : rho / rho_prev, it == 0 ? 1 : 0, st);
+            double rho, beta;
+            if (flex && it > 0) {
+                cg_dot2(g, cg_z, cg_r, cg_rprev, cg_partial, cg_out, st);
+                cg_read(2);
+                rho = cg_host[0];
+                beta = (cg_host[0] - cg_host[1]) / rho_prev;
+            } else {
+                cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out, st);
+                rho = cg_read(1);
+                beta = it == 0 ? 0.0 : rho / rho_prev;
+            }
+            cg_update_p(g, cg_p, cg_z, beta, it == 0 ? 1 : 0, st);
+            if (flex) cg_copy_d(g, cg_r, cg_rprev, st);
             cg_halo_r(g, cg_p, st);
             cg_apply(g, cg_p, cg_q, cg_partial, cg_out, st);
             const double alpha = rho / cg_read(1);
@@ -504,9 +641,15 @@ template <class T> struct ModelT : ModelBase {
             res = std::sqrt(cg_read(1));
             it += 1;
             rho_prev = rho;
+            if (flex && !fallback) {
+                stall = res < 0.999 * res_best ? 0 : stall + 1;
+                if (stall >= 3) { fallback = true; ln_fell_back = 1; }
+            }
+            res_best = std::min(res_best, res);
         }
         cg_iterations = (int)it;
         cg_res_last = res;
+        if (logging && nlogged > 0 && res <= tol) ln_log_pairs(nlogged);
         cg_halo_r(g, x, st);
         cg_true_residual(g, x, cg_b, cg_q, cg_partial, cg_out, st);
         const double emean = cg_read(1) / ncell;
@@ -668,7 +811,9 @@ static std::string validate(const om_config_t *c) {
     if (c->pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT) {
         for (int d = 0; d < 3; ++d)
             if (c->topology[d] == OM_FLAT) return "the conjugate-gradient solver needs a three-dimensional grid";
-        if (c->cg_preconditioner < OM_PRECOND_DEFAULT || c->cg_preconditioner > OM_PRECOND_COLUMNWISE_TRIDIAGONAL) return "unknown CG preconditioner";
+        if (c->cg_preconditioner < OM_PRECOND_DEFAULT || c->cg_preconditioner > OM_PRECOND_LEARNED) return "unknown CG preconditioner";
+        if (c->cg_preconditioner == OM_PRECOND_LEARNED && (c->N[0] % 16 != 0 || c->topology[2] != OM_BOUNDED))
+            return "the learned preconditioner needs Nx to be a multiple of 16 and a Bounded z";
     }
     if (c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL && c->topology[2] != OM_BOUNDED)
         return "the Fourier-tridiagonal solver needs a Bounded z";
@@ -904,6 +1049,119 @@ int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double
         if (maximum) *maximum = M->diag_host[0];
         if (minimum) *minimum = M->diag_host[1];
         if (nan_count) *nan_count = (int64_t)M->diag_host[2];
+    });
+}
+
+int om_learned_set_weights(om_model_t *m, const float *weights) {
+    if (!m || !weights) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        M->net_alloc();
+        const size_t nb = (size_t)NET_LAYERS * NET_WPL * sizeof(float);
+        HIPCHECK(hipMemcpyAsync(M->net.w, weights, nb, hipMemcpyHostToDevice, M->st));
+        HIPCHECK(hipMemsetAsync(M->net.m, 0, nb, M->st));
+        HIPCHECK(hipMemsetAsync(M->net.v, 0, nb, M->st));
+        M->net.adam_step = 0;
+        net_pack(M->net, M->st);
+        HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_learned_get_weights(om_model_t *m, float *weights) {
+    if (!m || !weights) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        M->net_alloc();
+        HIPCHECK(hipMemcpyAsync(weights, M->net.w, (size_t)NET_LAYERS * NET_WPL * sizeof(float), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity) {
+    if (!m || iterations_per_solve < 0 || capacity < 0) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (!M->cg) throw ArgError{"pairs are logged from conjugate-gradient solves"};
+        M->net_alloc();
+        if (capacity < M->log_cap) throw ArgError{"the pair capacity cannot shrink"};
+        M->log_iters = capacity > 0 ? iterations_per_solve : 0;
+        M->log_cap = capacity;
+    });
+}
+
+int om_learned_pairs(om_model_t *m, int32_t *count) {
+    if (!m || !count) return OM_EINVAL;
+    return WITH_MODEL(m, { *count = M->log_count; });
+}
+
+int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first, double *loss_last) {
+    if (!m || steps < 0 || !(lr > 0)) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (M->log_count == 0) throw ArgError{"no logged pairs to train on"};
+        NetState &S = M->net;
+        const float inv_n = (float)(1.0 / std::max(M->ln_nactive, 1.0));
+        double h = 0;
+        for (int s = 0; s < steps; ++s) {
+            const int pr = s % M->log_count;
+            net_input_dense(S, M->pair_in3[pr], M->st);
+            net_forward(S, S.out, M->st);
+            net_loss(S, S.out, M->pair_tgt[pr], M->ln_maskd, inv_n, M->st);
+            if (s == 0 || s == steps - 1) {
+                HIPCHECK(hipMemcpyAsync(&h, S.lsum, sizeof(double), hipMemcpyDeviceToHost, M->st));
+                HIPCHECK(hipStreamSynchronize(M->st));
+                if (s == 0 && loss_first) *loss_first = h * (double)inv_n;
+                if (s == steps - 1 && loss_last) *loss_last = h * (double)inv_n;
+            }
+            net_backward(S, nullptr, M->st);
+            net_adam(S, (float)lr, M->st);
+        }
+        HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_learned_forward(om_model_t *m, const float *input3, float *output) {
+    if (!m || !input3 || !output) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        M->net_alloc();
+        const size_t nc = (size_t)M->g.Nx * M->g.Ny * M->g.Nz;
+        HIPCHECK(hipMemcpyAsync(M->ln_in3, input3, 3 * nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        net_input_dense(M->net, M->ln_in3, M->st);
+        net_forward(M->net, M->net.out, M->st);
+        HIPCHECK(hipMemcpyAsync(output, M->net.out, nc * sizeof(float), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_learned_backward(om_model_t *m, const float *input3, const float *target, const float *mask, double *loss, float *dweights,
+                        float *dinput3) {
+    if (!m || !input3 || !target || !mask) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        M->net_alloc();
+        const size_t nc = (size_t)M->g.Nx * M->g.Ny * M->g.Nz;
+        float *dev = M->template dalloc<float>(5 * nc);   // target, mask, input gradient
+        double n = 0;
+        for (size_t q = 0; q < nc; ++q) n += mask[q];
+        const float inv_n = (float)(1.0 / std::max(n, 1.0));
+        HIPCHECK(hipMemcpyAsync(M->ln_in3, input3, 3 * nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        HIPCHECK(hipMemcpyAsync(dev, target, nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        HIPCHECK(hipMemcpyAsync(dev + nc, mask, nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        net_input_dense(M->net, M->ln_in3, M->st);
+        net_forward(M->net, M->net.out, M->st);
+        net_loss(M->net, M->net.out, dev, dev + nc, inv_n, M->st);
+        net_backward(M->net, dev + 2 * nc, M->st);
+        double h = 0;
+        HIPCHECK(hipMemcpyAsync(&h, M->net.lsum, sizeof(double), hipMemcpyDeviceToHost, M->st));
+        if (dweights)
+            HIPCHECK(hipMemcpyAsync(dweights, M->net.dw, (size_t)NET_LAYERS * NET_WPL * sizeof(float), hipMemcpyDeviceToHost, M->st));
+        if (dinput3) HIPCHECK(hipMemcpyAsync(dinput3, dev + 2 * nc, 3 * nc * sizeof(float), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+        if (loss) *loss = h * (double)inv_n;
+        M->dfree(dev);
+    });
+}
+
+int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *fell_back) {
+    if (!m) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (learned_iterations) *learned_iterations = M->ln_learned_its;
+        if (fell_back) *fell_back = M->ln_fell_back;
     });
 }
 

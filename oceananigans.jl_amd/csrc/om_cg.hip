@@ -345,6 +345,104 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_gauged_norm(DevGr
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Learned correction to the FFT preconditioner (the network is in om_learned.hip) and flexible CG
+// ---------------------------------------------------------------------------------------------
+// partial sums over active cells of (r V⁻¹)², z² and 1 (the input scales)
+template <class T> __global__ void __launch_bounds__(256) k_ln_scales(DevGrid<T> g, const double *r, const T *z, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double act = active(g, i, j, k);
+        const double rv = r[o] * (double)(T(1) / vol(g, k)), zv = (double)z[o];
+        a0 += act * rv * rv;
+        a1 += act * zv * zv;
+        a2 += act;
+    }
+    const double s0 = block_sum(a0), s1 = block_sum(a1), s2 = block_sum(a2);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; partial[2 * kRedBlocks + blockIdx.x] = s2; }
+}
+// network input channels, dense FP32 in cell order: r V⁻¹ / s_r, z / s_z, active
+template <class T> __global__ void __launch_bounds__(256) k_ln_input(DevGrid<T> g, const double *r, const T *z, double isr, double isz,
+                                                                     float *in3) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        in3[3 * t] = (float)(r[o] * (double)(T(1) / vol(g, k)) * isr);
+        in3[3 * t + 1] = (float)((double)z[o] * isz);
+        in3[3 * t + 2] = (float)active(g, i, j, k);
+    }
+}
+// z += s_z · active · (network output)
+template <class T> __global__ void __launch_bounds__(256) k_ln_correct(DevGrid<T> g, T *z, const float *out, double sz) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        z[o] = (T)((double)z[o] + sz * active(g, i, j, k) * (double)out[t]);
+    }
+}
+// interior copy of an FP64 cell field
+template <class T> __global__ void __launch_bounds__(256) k_cg_copy_d(DevGrid<T> g, const double *src, double *dst) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        dst[o] = src[o];
+    }
+}
+// z·r and z·r_prev (the flexible-CG β of Polak–Ribière form)
+template <class T> __global__ void __launch_bounds__(256) k_cg_dot2(DevGrid<T> g, const T *z, const double *r, const double *rp,
+                                                                    double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double a0 = 0, a1 = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double zv = (double)z[o];
+        a0 += zv * r[o];
+        a1 += zv * rp[o];
+    }
+    const double s0 = block_sum(a0), s1 = block_sum(a1);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; }
+}
+// target of a logged pair: (x_final − x_k) / s_z − ẑ_k on active cells (0 elsewhere), with partial
+// sums of the target and of the active count (its mean is removed afterwards: the pressure's
+// constant is free)
+template <class T> __global__ void __launch_bounds__(256) k_ln_target(DevGrid<T> g, const double *xf, const double *xk, const float *in3k,
+                                                                      double isz, float *tgt, double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double a0 = 0, a1 = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double act = active(g, i, j, k);
+        const double v = act * ((xf[o] - xk[o]) * isz - (double)in3k[3 * t + 1]);
+        tgt[t] = (float)v;
+        a0 += v;
+        a1 += act;
+    }
+    const double s0 = block_sum(a0), s1 = block_sum(a1);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; }
+}
+template <class T> __global__ void __launch_bounds__(256) k_ln_center(DevGrid<T> g, float *tgt, float mean) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        tgt[t] = active(g, i, j, k) > 0 ? tgt[t] - mean : 0.0f;
+    }
+}
+template <class T> __global__ void __launch_bounds__(256) k_ln_mask(DevGrid<T> g, float *mask) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        mask[t] = (float)active(g, i, j, k);
+    }
+}
+
 // out[v] = Σ partial[v·kRedBlocks + b] over b, in a fixed order
 static __global__ void __launch_bounds__(kRedThreads) k_cg_finish(const double *partial, int nvals, double *out) {
     for (int v = 0; v < nvals; ++v) {
@@ -424,6 +522,35 @@ template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, 
     const int64_t ncol = (int64_t)g.Nx * g.Ny;
     hipLaunchKernelGGL(k_cg_ct<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g, r, z, t);
 }
+template <class T> void ln_scales(const DevGrid<T> &g, const double *r, const T *z, double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_scales<T>, red_grid(), red_blk(), 0, st, g, r, z, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 3, out);
+}
+template <class T> void ln_input(const DevGrid<T> &g, const double *r, const T *z, double isr, double isz, float *in3, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_input<T>, red_grid(), red_blk(), 0, st, g, r, z, isr, isz, in3);
+}
+template <class T> void ln_correct(const DevGrid<T> &g, T *z, const float *out, double sz, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_correct<T>, red_grid(), red_blk(), 0, st, g, z, out, sz);
+}
+template <class T> void cg_copy_d(const DevGrid<T> &g, const double *src, double *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_copy_d<T>, red_grid(), red_blk(), 0, st, g, src, dst);
+}
+template <class T> void cg_dot2(const DevGrid<T> &g, const T *z, const double *r, const double *rp, double *partial, double *out,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_dot2<T>, red_grid(), red_blk(), 0, st, g, z, r, rp, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+}
+template <class T> void ln_target(const DevGrid<T> &g, const double *xf, const double *xk, const float *in3k, double isz, float *tgt,
+                                  double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_target<T>, red_grid(), red_blk(), 0, st, g, xf, xk, in3k, isz, tgt, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+}
+template <class T> void ln_center(const DevGrid<T> &g, float *tgt, float mean, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_center<T>, red_grid(), red_blk(), 0, st, g, tgt, mean);
+}
+template <class T> void ln_mask(const DevGrid<T> &g, float *mask, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_mask<T>, red_grid(), red_blk(), 0, st, g, mask);
+}
 template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_copy_r<T>, red_grid(), red_blk(), 0, st, g, r, z);
 }
@@ -471,6 +598,15 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void cg_fft_src<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                          \
     template void cg_copy_r<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                           \
     template void cg_ct<T>(const DevGrid<T> &, const double *, T *, T *, hipStream_t);                          \
+    template void ln_scales<T>(const DevGrid<T> &, const double *, const T *, double *, double *, hipStream_t);    \
+    template void ln_input<T>(const DevGrid<T> &, const double *, const T *, double, double, float *, hipStream_t); \
+    template void ln_correct<T>(const DevGrid<T> &, T *, const float *, double, hipStream_t);                    \
+    template void cg_copy_d<T>(const DevGrid<T> &, const double *, double *, hipStream_t);                       \
+    template void cg_dot2<T>(const DevGrid<T> &, const T *, const double *, const double *, double *, double *, hipStream_t); \
+    template void ln_target<T>(const DevGrid<T> &, const double *, const double *, const float *, double, float *, double *, \
+                               double *, hipStream_t);                                                            \
+    template void ln_center<T>(const DevGrid<T> &, float *, float, hipStream_t);                                 \
+    template void ln_mask<T>(const DevGrid<T> &, float *, hipStream_t);                                          \
     template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
     template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, double, int, hipStream_t);            \
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \

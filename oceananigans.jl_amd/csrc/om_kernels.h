@@ -591,6 +591,17 @@ template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st
 template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
 template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st);
 template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
+// learned correction to the FFT preconditioner and flexible CG (om_cg.hip)
+template <class T> void ln_scales(const DevGrid<T> &g, const double *r, const T *z, double *partial, double *out, hipStream_t st);
+template <class T> void ln_input(const DevGrid<T> &g, const double *r, const T *z, double isr, double isz, float *in3, hipStream_t st);
+template <class T> void ln_correct(const DevGrid<T> &g, T *z, const float *out, double sz, hipStream_t st);
+template <class T> void cg_copy_d(const DevGrid<T> &g, const double *src, double *dst, hipStream_t st);
+template <class T> void cg_dot2(const DevGrid<T> &g, const T *z, const double *r, const double *rp, double *partial, double *out,
+                                hipStream_t st);
+template <class T> void ln_target(const DevGrid<T> &g, const double *xf, const double *xk, const float *in3k, double isz, float *tgt,
+                                  double *partial, double *out, hipStream_t st);
+template <class T> void ln_center(const DevGrid<T> &g, float *tgt, float mean, hipStream_t st);
+template <class T> void ln_mask(const DevGrid<T> &g, float *mask, hipStream_t st);
 // ColumnwiseTridiagonalPreconditioner: z = Lz⁻¹ r per column; t is an Nx·Ny·Nz scratch array
 template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st);
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);

@@ -22,6 +22,7 @@ from .physics import (  # noqa: F401
     FluxBoundaryCondition,
     FPlane,
     GradientBoundaryCondition,
+    LearnedPreconditioner,
     LinearEquationOfState,
     ScalarDiffusivity,
     SeawaterBuoyancy,

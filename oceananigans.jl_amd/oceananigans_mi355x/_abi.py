@@ -22,6 +22,8 @@ OM_FIELD_PNHS, OM_FIELD_PHY, OM_FIELD_NU, OM_FIELD_KAPPA0 = 300, 301, 302, 303
 OM_SOLVER_FFT, OM_SOLVER_FOURIER_TRIDIAGONAL, OM_SOLVER_CONJUGATE_GRADIENT = 0, 1, 2
 OM_PRECOND_DEFAULT, OM_PRECOND_NONE, OM_PRECOND_DIAGONALLY_DOMINANT, OM_PRECOND_FFT = 0, 1, 2, 3
 OM_PRECOND_COLUMNWISE_TRIDIAGONAL = 4
+OM_PRECOND_LEARNED = 5
+OM_LEARNED_NWEIGHTS = 4 * 27 * 16 * 16
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
@@ -110,6 +112,17 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_double)]),
     "om_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "om_cell_advection_timescale": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "om_learned_set_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_get_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "om_learned_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "om_learned_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]),
+    "om_learned_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_last_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "om_field_statistics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
 }

@@ -270,6 +270,9 @@ class NonhydrostaticModel:
         self.Gn = _NamedFields((n, Field(self, _abi.OM_FIELD_GN + i, "G" + n)) for i, n in enumerate(names))
         self.Gm = _NamedFields((n, Field(self, _abi.OM_FIELD_GM + i, "G⁻" + n)) for i, n in enumerate(names))
         self.clock = Clock(self)
+        pc = getattr(pressure_solver, "preconditioner", None)
+        if getattr(pc, "weights", None) is not None:
+            self.learned_set_weights(pc.weights)
         # the constructor ends with update_state!(model) (nonhydrostatic_model.jl:309)
         self.update_state()
 
@@ -382,6 +385,60 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_field_statistics(self._h, int(field_id), 1 if with_halos else 0, ctypes.byref(mx),
                                                  ctypes.byref(mn), ctypes.byref(nn)), self._h)
         return mx.value, mn.value, nn.value
+
+    # ---------------------------------------------------------------- learned preconditioner
+    def learned_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        if w.size != _abi.OM_LEARNED_NWEIGHTS:
+            raise ValueError(f"expected {_abi.OM_LEARNED_NWEIGHTS} weights")
+        _abi.check(self._lib.om_learned_set_weights(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), self._h)
+
+    def learned_get_weights(self):
+        w = np.empty(_abi.OM_LEARNED_NWEIGHTS, dtype=np.float32)
+        _abi.check(self._lib.om_learned_get_weights(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), self._h)
+        return w.reshape(4, 27, 16, 16)
+
+    def learned_log(self, iterations_per_solve=4, capacity=64):
+        _abi.check(self._lib.om_learned_log(self._h, int(iterations_per_solve), int(capacity)), self._h)
+
+    def learned_pairs(self):
+        n = ctypes.c_int32()
+        _abi.check(self._lib.om_learned_pairs(self._h, ctypes.byref(n)), self._h)
+        return n.value
+
+    def learned_train(self, steps, lr=1e-3):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _abi.check(self._lib.om_learned_train(self._h, int(steps), float(lr), ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, b.value
+
+    @staticmethod
+    def _fp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def learned_forward(self, input3):
+        """Network output (Nx, Ny, Nz) for a (Nx, Ny, Nz, 3) input."""
+        g = self.grid
+        x = np.ascontiguousarray(np.transpose(np.asarray(input3, dtype=np.float32), (2, 1, 0, 3)))
+        out = np.empty((g.Nz, g.Ny, g.Nx), dtype=np.float32)
+        _abi.check(self._lib.om_learned_forward(self._h, self._fp(x), self._fp(out)), self._h)
+        return np.transpose(out, (2, 1, 0))
+
+    def learned_backward(self, input3, target, mask):
+        """(loss, dL/dW (4, 27, 16, 16), dL/d(input) (Nx, Ny, Nz, 3)) of the mean-square loss."""
+        g = self.grid
+        tr = lambda a: np.ascontiguousarray(np.transpose(np.asarray(a, dtype=np.float32), (2, 1, 0) + tuple(range(3, np.ndim(a)))))
+        x, t, m = tr(input3), tr(target), tr(mask)
+        dw = np.empty(_abi.OM_LEARNED_NWEIGHTS, dtype=np.float32)
+        din = np.empty((g.Nz, g.Ny, g.Nx, 3), dtype=np.float32)
+        loss = ctypes.c_double()
+        _abi.check(self._lib.om_learned_backward(self._h, self._fp(x), self._fp(t), self._fp(m), ctypes.byref(loss),
+                                                 self._fp(dw), self._fp(din)), self._h)
+        return loss.value, dw.reshape(4, 27, 16, 16), np.transpose(din, (2, 1, 0, 3))
+
+    def learned_last_solve(self):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _abi.check(self._lib.om_learned_last_solve(self._h, ctypes.byref(a), ctypes.byref(b)), self._h)
+        return a.value, bool(b.value)
 
     def divergence(self):
         n = self.grid.Nx * self.grid.Ny * self.grid.Nz

@@ -142,6 +142,18 @@ class ColumnwiseTridiagonalPreconditioner:
         self.grid = grid
 
 
+class LearnedPreconditioner:
+    """The FFT preconditioner of the underlying grid plus a learned convolutional correction
+    (four 3×3×3 convolutions of 16 channels, BF16 on the matrix cores), applied inside flexible CG
+    with a fallback to the FFT preconditioner when convergence stalls.  No reference counterpart:
+    it takes the place of the preconditioner call of conjugate_gradient_poisson_solver.jl:169-211.
+    `weights`: optional (4, 27, 16, 16) FP32 array [layer][tap][cin][cout]; zero weights give the
+    FFT preconditioner."""
+
+    def __init__(self, weights=None):
+        self.weights = weights
+
+
 class ConjugateGradientPoissonSolver:
     """ConjugateGradientPoissonSolver(grid; preconditioner, reltol, abstol, maxiter)
     (conjugate_gradient_poisson_solver.jl:134-163).  preconditioner: "default" (the reference's
@@ -167,4 +179,6 @@ class ConjugateGradientPoissonSolver:
             return _abi.OM_PRECOND_FFT
         if isinstance(p, ColumnwiseTridiagonalPreconditioner):
             return _abi.OM_PRECOND_COLUMNWISE_TRIDIAGONAL
+        if isinstance(p, LearnedPreconditioner):
+            return _abi.OM_PRECOND_LEARNED
         raise ValueError(f"unsupported preconditioner {p!r}")

@@ -1,0 +1,116 @@
+"""The learned preconditioner on the GPU, through the C ABI: the MFMA forward pass and the
+hand-written backward pass against the oracle (oracle/learned.py), training on pairs logged from
+the model's own CG solves, and flexible CG with the trained network against FFT-preconditioned CG.
+
+Forward / backward bars: relative 2-norm difference ≤ 2e-3 per output.  Both sides round the
+same quantities to BF16 and accumulate in FP32, in different orders; a sum that lands within
+FP32 rounding of a BF16 rounding boundary rounds the other way (one BF16 ulp, 0.4 %), which is
+what the bar allows for."""
+import numpy as np
+import pytest
+
+import oceananigans_mi355x as om
+from cases import product_for, rel_err, seamount
+from oracle import learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _net_model(N=(32, 8, 6)):
+    grid = om.RectilinearGrid(size=N, extent=(1.0, 0.5, 0.4))
+    return om.NonhydrostaticModel(grid, advection=om.WENO(),
+                                  pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=om.LearnedPreconditioner()))
+
+
+def _inputs(N, seed):
+    rng = np.random.default_rng(seed)
+    x3 = rng.normal(size=N + (3,)).astype(np.float32)
+    x3[..., 2] = (rng.random(N) > 0.15).astype(np.float32)
+    return x3
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_matches_oracle():
+    model = _net_model()
+    N = model.grid.N
+    w = L.init_weights(5, scale=1.0)
+    model.learned_set_weights(w)
+    np.testing.assert_array_equal(model.learned_get_weights(), w)
+    x3 = _inputs(N, 1)
+    y = model.learned_forward(x3)
+    yo, _ = L.forward(x3, w)
+    e = _rel(y, yo)
+    print("forward rel err", e, "max|y|", np.abs(yo).max(), flush=True)
+    assert e <= 2e-3
+
+
+def test_backward_matches_oracle():
+    model = _net_model()
+    N = model.grid.N
+    w = L.init_weights(6, scale=1.0)
+    model.learned_set_weights(w)
+    x3 = _inputs(N, 2)
+    t = np.random.default_rng(3).normal(size=N).astype(np.float32)
+    m = x3[..., 2]
+    loss, dw, din = model.learned_backward(x3, t, m)
+    out, acts = L.forward(x3, w)
+    loss_o, dout = L.loss_and_grad_output(out, t, m)
+    dwo, gino = L.backward(acts, w, dout.astype(np.float32))
+    errs = [_rel(dw[l], dwo[l]) for l in range(4)]
+    ein = _rel(din, gino[..., :3])
+    print("loss", loss, loss_o, "dW rel err per layer", errs, "input grad", ein, flush=True)
+    assert loss == pytest.approx(loss_o, rel=2e-3)
+    assert max(errs) <= 2e-3 and ein <= 2e-3
+
+
+def _seamount_models(pc_a, pc_b, N=(32, 32, 16)):
+    grid, kw, init = seamount(N=N)
+    ka = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=pc_a, reltol=1e-10, abstol=0))
+    kb = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=pc_b, reltol=1e-10, abstol=0))
+    return product_for(grid, ka, init), product_for(grid, kb, init)
+
+
+def test_train_on_logged_pairs_then_flexible_cg_matches_fft_cg():
+    """Pairs logged from the learned model's own solves (the network starts at zero weights, i.e.
+    at the FFT preconditioner); Adam lowers the loss; the trained network inside flexible CG
+    reaches the same states as FFT-preconditioned CG, and the recomputed FP64 residual stays
+    within ten times the tolerance."""
+    learned, fft = _seamount_models(om.LearnedPreconditioner(), om.FourierTridiagonalPoissonSolver())
+    w0 = L.init_weights(7)
+    w0[3] = 0.0                               # zero output layer: starts as the FFT preconditioner
+    learned.learned_set_weights(w0)
+    learned.learned_log(iterations_per_solve=4, capacity=48)
+    for _ in range(3):
+        om.time_step_(learned, 20.0)
+        om.time_step_(fft, 20.0)
+    npairs = learned.learned_pairs()
+    assert npairs > 0
+    l0, l1 = learned.learned_train(300, lr=2e-3)
+    print("pairs", npairs, "loss first / last", l0, l1, flush=True)
+    assert l1 < l0
+    its = []
+    for _ in range(2):
+        om.time_step_(learned, 20.0)
+        om.time_step_(fft, 20.0)
+        its.append((learned.last_solve_info()[0], fft.last_solve_info()[0], learned.learned_last_solve()))
+    print("iterations learned / fft / (network iterations, fell back)", its, flush=True)
+    for n in ("u", "v", "w"):
+        assert rel_err(learned.velocities[n].interior(), fft.velocities[n].interior()) <= 1e-8
+    it, res, true_res = learned.last_solve_info()
+    assert true_res <= 10 * res + 1e-30 or true_res <= 1e-8
+
+
+def test_untrained_large_weights_still_converge():
+    """A network far from useful (large random weights): flexible CG either converges or hands
+    over to the FFT preconditioner; either way the state matches FFT-preconditioned CG."""
+    learned, fft = _seamount_models(om.LearnedPreconditioner(weights=L.init_weights(11, scale=5.0)),
+                                    om.FourierTridiagonalPoissonSolver(), N=(16, 16, 12))
+    for _ in range(2):
+        om.time_step_(learned, 20.0)
+        om.time_step_(fft, 20.0)
+    print("learned", learned.last_solve_info(), learned.learned_last_solve(), "fft", fft.last_solve_info(), flush=True)
+    for n in ("u", "v", "w"):
+        assert rel_err(learned.velocities[n].interior(), fft.velocities[n].interior()) <= 1e-8

@@ -1,0 +1,77 @@
+"""Pins of the learned-preconditioner oracle (oracle/learned.py): BF16 rounding, the convolution
+and its adjoint, the gradients, and the Adam step.  The reference has no such network, so every
+pin is a closed form, a brute-force evaluation or a finite difference."""
+import numpy as np
+import pytest
+
+from oracle import learned as L
+
+
+def test_bf16_rounding_matches_round_to_nearest_even():
+    torch = pytest.importorskip("torch")
+    x = np.random.default_rng(0).normal(size=10000).astype(np.float32) * np.float32(3.7)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(L.bf16(x), ref)
+    one = np.float32(1.0)
+    ties = np.array([one + np.float32(2.0 ** -8), one + np.float32(3 * 2.0 ** -8), one + np.float32(2.0 ** -9)], dtype=np.float32)
+    np.testing.assert_array_equal(L.bf16(ties), np.array([1.0, 1.0 + 2.0 ** -6, 1.0], dtype=np.float32))
+
+
+def test_convolution_against_brute_force():
+    """y[i,j,k,o] = Σ x[(i+dx) mod Nx, (j+dy) mod Ny, k+dz, c] W[t,c,o], zero outside 0 ≤ k+dz < Nz."""
+    rng = np.random.default_rng(1)
+    Nx, Ny, Nz = 3, 4, 3
+    x = rng.normal(size=(Nx, Ny, Nz, 16))
+    w = rng.normal(size=(27, 16, 16))
+    y = L.conv(x, w, np.float64)
+    ref = np.zeros_like(y)
+    for i in range(Nx):
+        for j in range(Ny):
+            for k in range(Nz):
+                for t, (dz, dy, dx) in enumerate(L.taps()):
+                    kk = k + dz
+                    if 0 <= kk < Nz:
+                        ref[i, j, k] += x[(i + dx) % Nx, (j + dy) % Ny, kk] @ w[t]
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+    # the adjoint: <conv(x), g> = <x, conv_transpose(g)>
+    g = rng.normal(size=y.shape)
+    assert np.sum(y * g) == pytest.approx(np.sum(x * L.conv_transpose(g, w, np.float64)), rel=1e-12)
+
+
+def test_gradients_against_finite_differences():
+    rng = np.random.default_rng(2)
+    N = (6, 5, 4)
+    x3 = rng.normal(size=N + (3,))
+    x3[..., 2] = (rng.random(N) > 0.2)
+    w = L.init_weights(3, scale=1.0).astype(np.float64)
+    t = rng.normal(size=N)
+    m = x3[..., 2].copy()
+    out, acts = L.forward(x3, w, precise=True)
+    loss, dout = L.loss_and_grad_output(out, t, m)
+    dw, gin = L.backward(acts, w, dout, precise=True)
+
+    def lossof(ww, xx):
+        return L.loss_and_grad_output(L.forward(xx, ww, True)[0], t, m)[0]
+
+    e = 1e-6
+    for idx in [(0, 13, 0, 5), (0, 2, 2, 9), (1, 4, 7, 2), (2, 26, 15, 15), (3, 0, 3, 0), (3, 13, 9, 0)]:
+        wp, wm = w.copy(), w.copy()
+        wp[idx] += e
+        wm[idx] -= e
+        fd = (lossof(wp, x3) - lossof(wm, x3)) / (2 * e)
+        assert dw[idx] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+    for v in [(1, 2, 3, 0), (0, 0, 0, 1), (5, 4, 0, 2)]:
+        xp, xm = x3.copy(), x3.copy()
+        xp[v] += e
+        xm[v] -= e
+        fd = (lossof(w, xp) - lossof(w, xm)) / (2 * e)
+        assert gin[v] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+
+
+def test_first_adam_step_closed_form():
+    """Step 1: m̂ = g, v̂ = g², so p ← p − lr · g / (|g| + ε)."""
+    p = np.array([1.0, -2.0, 0.5], dtype=np.float32)
+    g = np.array([0.3, -4.0, 1e-3], dtype=np.float32)
+    z = np.zeros(3, dtype=np.float32)
+    new, m, v = L.adam(p, g, z, z, 1, 0.01)
+    np.testing.assert_allclose(new, p - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-6)
