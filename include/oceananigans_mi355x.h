@@ -222,13 +222,17 @@ int om_cell_advection_timescale(om_model_t *m, double *tau);
  *   om_learned_set_weights / get_weights   the FP32 master weights (setting them resets Adam)
  *   om_learned_log       log (input, target) pairs from the first `iterations_per_solve`
  *                        iterations of every later FFT- or learned-preconditioned CG solve, into a
- *                        ring of `capacity` pairs on the device; target = (x_final − x_k)/rms − ẑ_k
+ *                        ring of `capacity` pairs on the device; target = (x_final − x_k)/rms − ẑ_k;
+ *                        each call starts a new set
  *   om_learned_pairs     number of pairs held
  *   om_learned_train     `steps` Adam steps (lr) over the held pairs, cycling; loss of the first
- *                        and of the last step (mean square over active cells)
+ *                        and of the last step: the A-energy of the remaining error relative to
+ *                        the target's, (δ − t)ᵀ(−V∇²)(δ − t) / tᵀ(−V∇²)t, with the solver's operator
+ *   om_learned_evaluate  that loss averaged over the held pairs, without training (1 = the FFT
+ *                        preconditioner alone, 0 = the exact inverse)
  *   om_learned_forward   network output for a given 3-channel input (test hook)
- *   om_learned_backward  loss, weight gradients and input gradient (3 channels) for a given
- *                        input, target and mask (test hook)
+ *   om_learned_backward  mean-square loss over the mask, weight gradients and input gradient
+ *                        (3 channels) for a given input, target and mask (test hook)
  *   om_learned_last_solve  iterations that used the network in the last solve, and whether the
  *                        solve handed over to the FFT preconditioner */
 #define OM_LEARNED_NWEIGHTS (4 * 27 * 16 * 16)
@@ -237,6 +241,7 @@ int om_learned_get_weights(om_model_t *m, float *weights);
 int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity);
 int om_learned_pairs(om_model_t *m, int32_t *count);
 int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first, double *loss_last);
+int om_learned_evaluate(om_model_t *m, double *mean_loss);
 int om_learned_forward(om_model_t *m, const float *input3, float *output);
 int om_learned_backward(om_model_t *m, const float *input3, const float *target, const float *mask, double *loss,
                         float *dweights, float *dinput3);
@@ -266,6 +271,9 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
 #define OM_TUNE_TENDENCY_KERNEL 0   /* 0: z-marching LDS-tiled tendency and closure kernels where supported
                                        (default); 1: thread-per-cell kernels */
 #define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernels (default 16) */
+#define OM_TUNE_LEARNED_ITERATIONS 2 /* leading iterations of each CG solve that apply the learned
+                                        correction (0: all); later iterations use the FFT
+                                        preconditioner.  Solutions agree to the solver tolerance. */
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 

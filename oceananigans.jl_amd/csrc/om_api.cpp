@@ -183,12 +183,15 @@ template <class T> struct ModelT : ModelBase {
     int ln_learned_its = 0, ln_fell_back = 0;
     int log_iters = 0, log_cap = 0, log_count = 0, log_next = 0;   // pair logging (om_learned_log)
     std::vector<float *> pair_in3, pair_tgt;       // logged pairs (device, dense)
+    std::vector<double> pair_energy;               // tᵀ(−V∇²)t of each pair's target
+    float *ln_gout = nullptr;                      // dense output gradient of the energy loss
     std::vector<double *> snap_x;                  // iterates of the logged iterations of the current solve
     std::vector<float *> snap_in3;
     std::vector<double> snap_isz;
     // launch-shape choices (om_set_tuning)
     int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
     int tune_kchunk = 16;              // z levels per block of the tiled kernel
+    int tune_learned_iterations = 0;   // leading CG iterations that apply the learned correction (0: all)
 
     struct PhaseScope {
         ModelT *M;
@@ -549,6 +552,7 @@ template <class T> struct ModelT : ModelBase {
             if ((int)pair_in3.size() < log_cap) {
                 pair_in3.push_back(dalloc<float>(3 * nc));
                 pair_tgt.push_back(dalloc<float>(nc));
+                pair_energy.push_back(0);
             }
             const int slot = log_next;
             log_next = (log_next + 1) % log_cap;
@@ -558,6 +562,8 @@ template <class T> struct ModelT : ModelBase {
             cg_read(2);
             const float mean = (float)(cg_host[0] / std::max(cg_host[1], 1.0));
             ln_center(g, pair_tgt[slot], mean, st);
+            ln_energy(g, (const float *)nullptr, pair_tgt[slot], cg_q, 0.0, (float *)nullptr, cg_partial, cg_out, st);
+            pair_energy[slot] = cg_read(1);
         }
     }
 
@@ -605,7 +611,8 @@ template <class T> struct ModelT : ModelBase {
                 cg_fft_src(g, cg_r, rhs_dense, st);
                 launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);
                 const bool log_this = logging && it < log_iters;
-                const bool learn_this = pk == OM_PRECOND_LEARNED && !fallback;
+                const bool learn_this = pk == OM_PRECOND_LEARNED && !fallback &&
+                                        (tune_learned_iterations == 0 || it < tune_learned_iterations);
                 double isz = 0;
                 if ((log_this || learn_this) && ln_prepare(isz)) {
                     if (log_this) { ln_log_snapshot((int)it, isz); nlogged = (int)it + 1; }
@@ -1080,9 +1087,18 @@ int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity
     return WITH_MODEL(m, {
         if (!M->cg) throw ArgError{"pairs are logged from conjugate-gradient solves"};
         M->net_alloc();
-        if (capacity < M->log_cap) throw ArgError{"the pair capacity cannot shrink"};
+        // a new call starts a new set of pairs (buffers are reused)
         M->log_iters = capacity > 0 ? iterations_per_solve : 0;
         M->log_cap = capacity;
+        M->log_count = 0;
+        M->log_next = 0;
+        while ((int)M->pair_in3.size() > capacity) {
+            M->dfree(M->pair_in3.back());
+            M->dfree(M->pair_tgt.back());
+            M->pair_in3.pop_back();
+            M->pair_tgt.pop_back();
+            M->pair_energy.pop_back();
+        }
     });
 }
 
@@ -1096,23 +1112,41 @@ int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first
     return WITH_MODEL(m, {
         if (M->log_count == 0) throw ArgError{"no logged pairs to train on"};
         NetState &S = M->net;
-        const float inv_n = (float)(1.0 / std::max(M->ln_nactive, 1.0));
-        double h = 0;
+        if (!M->ln_gout) M->ln_gout = M->template dalloc<float>((size_t)M->g.Nx * M->g.Ny * M->g.Nz);
+        // loss: the A-energy of the remaining error relative to that of the target,
+        // (δ − t)ᵀ(−V∇²)(δ − t) / tᵀ(−V∇²)t, the quantity a CG preconditioner should make small
         for (int s = 0; s < steps; ++s) {
             const int pr = s % M->log_count;
+            const double et = std::max(M->pair_energy[pr], 1e-300);
             net_input_dense(S, M->pair_in3[pr], M->st);
             net_forward(S, S.out, M->st);
-            net_loss(S, S.out, M->pair_tgt[pr], M->ln_maskd, inv_n, M->st);
+            ln_energy(M->g, S.out, M->pair_tgt[pr], M->cg_q, 1.0 / et, M->ln_gout, M->cg_partial, M->cg_out, M->st);
             if (s == 0 || s == steps - 1) {
-                HIPCHECK(hipMemcpyAsync(&h, S.lsum, sizeof(double), hipMemcpyDeviceToHost, M->st));
-                HIPCHECK(hipStreamSynchronize(M->st));
-                if (s == 0 && loss_first) *loss_first = h * (double)inv_n;
-                if (s == steps - 1 && loss_last) *loss_last = h * (double)inv_n;
+                const double e = M->cg_read(1) / et;
+                if (s == 0 && loss_first) *loss_first = e;
+                if (s == steps - 1 && loss_last) *loss_last = e;
             }
+            net_output_grad(S, M->ln_gout, M->st);
             net_backward(S, nullptr, M->st);
             net_adam(S, (float)lr, M->st);
         }
         HIPCHECK(hipStreamSynchronize(M->st));
+    });
+}
+
+int om_learned_evaluate(om_model_t *m, double *mean_loss) {
+    if (!m || !mean_loss) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (M->log_count == 0) throw ArgError{"no logged pairs to evaluate on"};
+        NetState &S = M->net;
+        double sum = 0;
+        for (int pr = 0; pr < M->log_count; ++pr) {
+            net_input_dense(S, M->pair_in3[pr], M->st);
+            net_forward(S, S.out, M->st);
+            ln_energy(M->g, S.out, M->pair_tgt[pr], M->cg_q, 0.0, (float *)nullptr, M->cg_partial, M->cg_out, M->st);
+            sum += M->cg_read(1) / std::max(M->pair_energy[pr], 1e-300);
+        }
+        *mean_loss = sum / M->log_count;
     });
 }
 
@@ -1169,18 +1203,24 @@ int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (!m) return OM_EINVAL;
     if (key == OM_TUNE_TENDENCY_KERNEL && (value < 0 || value > 1)) return OM_EINVAL;
     if (key == OM_TUNE_KCHUNK && (value < 1 || value > 1 << 20)) return OM_EINVAL;
-    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK) return OM_EINVAL;
+    if (key == OM_TUNE_LEARNED_ITERATIONS && value < 0) return OM_EINVAL;
+    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK && key != OM_TUNE_LEARNED_ITERATIONS) return OM_EINVAL;
     return WITH_MODEL(m, {
         if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
-        else M->tune_kchunk = value;
+        else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = value;
+        else M->tune_learned_iterations = value;
     });
 }
 
 int om_get_tuning(const om_model_t *mc, int32_t key, int32_t *value) {
     om_model_t *m = const_cast<om_model_t *>(mc);
     if (!m || !value) return OM_EINVAL;
-    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK) return OM_EINVAL;
-    return WITH_MODEL(m, { *value = key == OM_TUNE_TENDENCY_KERNEL ? M->tune_tendency_kernel : M->tune_kchunk; });
+    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK && key != OM_TUNE_LEARNED_ITERATIONS) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        *value = key == OM_TUNE_TENDENCY_KERNEL ? M->tune_tendency_kernel
+               : key == OM_TUNE_KCHUNK          ? M->tune_kchunk
+                                                : M->tune_learned_iterations;
+    });
 }
 
 int om_last_step_timing(const om_model_t *mc, double *ms_total, double *ms_per_phase, int32_t *launches_per_phase,

@@ -443,6 +443,31 @@ template <class T> __global__ void __launch_bounds__(256) k_ln_mask(DevGrid<T> g
     }
 }
 
+// A-energy training loss of the learned correction.  d = act · (y − t) (dimensionless, in units
+// of rms(z_FFT)) goes into a padded FP64 field; after its halos, the loss is dᵀ(−V∇²)d with the
+// solver's own (conditional) operator, and its gradient with respect to y is −2·act·V∇²d.
+template <class T> __global__ void __launch_bounds__(256) k_ln_err(DevGrid<T> g, const float *y, const float *tgt, double *d) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        d[idx3(i, j, k, g.sy, g.sz)] = active(g, i, j, k) * ((y ? (double)y[t] : 0.0) - (double)tgt[t]);
+    }
+}
+template <class T> __global__ void __launch_bounds__(256) k_ln_energy(DevGrid<T> g, const double *d, double gscale, float *gout,
+                                                                      double *partial) {
+    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+    double acc = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k; cell_of(g, t, i, j, k);
+        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+        const double ad = vlap<double>(g, d, o, i, j, k);
+        acc -= d[o] * ad;
+        if (gout) gout[t] = (float)(-2.0 * gscale * active(g, i, j, k) * ad);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // out[v] = Σ partial[v·kRedBlocks + b] over b, in a fixed order
 static __global__ void __launch_bounds__(kRedThreads) k_cg_finish(const double *partial, int nvals, double *out) {
     for (int v = 0; v < nvals; ++v) {
@@ -548,6 +573,13 @@ template <class T> void ln_target(const DevGrid<T> &g, const double *xf, const d
 template <class T> void ln_center(const DevGrid<T> &g, float *tgt, float mean, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_center<T>, red_grid(), red_blk(), 0, st, g, tgt, mean);
 }
+template <class T> void ln_energy(const DevGrid<T> &g, const float *y, const float *tgt, double *dscratch, double gscale, float *gout,
+                                  double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_err<T>, red_grid(), red_blk(), 0, st, g, y, tgt, dscratch);
+    cg_halo_r(g, dscratch, st);
+    hipLaunchKernelGGL(k_ln_energy<T>, red_grid(), red_blk(), 0, st, g, (const double *)dscratch, gscale, gout, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+}
 template <class T> void ln_mask(const DevGrid<T> &g, float *mask, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_mask<T>, red_grid(), red_blk(), 0, st, g, mask);
 }
@@ -607,6 +639,8 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
                                double *, hipStream_t);                                                            \
     template void ln_center<T>(const DevGrid<T> &, float *, float, hipStream_t);                                 \
     template void ln_mask<T>(const DevGrid<T> &, float *, hipStream_t);                                          \
+    template void ln_energy<T>(const DevGrid<T> &, const float *, const float *, double *, double, float *, double *, double *, \
+                               hipStream_t);                                                                      \
     template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
     template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, double, int, hipStream_t);            \
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \

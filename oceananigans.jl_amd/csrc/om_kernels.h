@@ -602,6 +602,10 @@ template <class T> void ln_target(const DevGrid<T> &g, const double *xf, const d
                                   double *partial, double *out, hipStream_t st);
 template <class T> void ln_center(const DevGrid<T> &g, float *tgt, float mean, hipStream_t st);
 template <class T> void ln_mask(const DevGrid<T> &g, float *mask, hipStream_t st);
+// A-energy loss dᵀ(−V∇²)d of d = act·(y − t) (y may be null: d = −t) and, when gout is given,
+// gout = −2·gscale·act·V∇²d; dscratch is a padded FP64 field
+template <class T> void ln_energy(const DevGrid<T> &g, const float *y, const float *tgt, double *dscratch, double gscale, float *gout,
+                                  double *partial, double *out, hipStream_t st);
 // ColumnwiseTridiagonalPreconditioner: z = Lz⁻¹ r per column; t is an Nx·Ny·Nz scratch array
 template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st);
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);

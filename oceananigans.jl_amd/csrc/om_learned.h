@@ -35,5 +35,6 @@ void net_forward(NetState &S, float *out, hipStream_t st);
 void net_backward(NetState &S, float *din, hipStream_t st);
 void net_loss(NetState &S, const float *out, const float *target, const float *mask, float inv_n, hipStream_t st);
 void net_adam(NetState &S, float lr, hipStream_t st);
+void net_output_grad(NetState &S, const float *gdense, hipStream_t st);
 
 }  // namespace om

@@ -213,6 +213,22 @@ __global__ void __launch_bounds__(256) k_loss(NetDims d, const float *y, const f
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// the back-propagated gradient at the network output, from a dense FP32 array (channel 0)
+__global__ void __launch_bounds__(256) k_out_grad(NetDims d, const float *gd, bf16 *g) {
+    const int64_t nv = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int i = (int)(v % d.nx);
+    const int64_t r = v / d.nx;
+    const int j = (int)(r % d.ny), k = (int)(r / d.ny);
+    bf16x8 a, b;
+    for (int e = 0; e < 8; ++e) { a[e] = (bf16)0.0f; b[e] = (bf16)0.0f; }
+    a[0] = (bf16)gd[v];
+    bf16 *p = g + pidx(d, i, j, k) * 16;
+    *reinterpret_cast<bf16x8 *>(p) = a;
+    *reinterpret_cast<bf16x8 *>(p + 8) = b;
+}
+
 // Σ partial[b] in a fixed order (one block)
 __global__ void __launch_bounds__(256) k_loss_sum(const double *partial, double *out) {
     __shared__ double red[256];
@@ -330,6 +346,12 @@ void net_backward(NetState &S, float *din, hipStream_t st) {
 void net_loss(NetState &S, const float *out, const float *target, const float *mask, float inv_n, hipStream_t st) {
     hipLaunchKernelGGL(k_loss, dim3(NET_RED_BLOCKS), dim3(256), 0, st, S.d, out, target, mask, inv_n, (bf16 *)S.g[0], S.lpartial);
     hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, st, (const double *)S.lpartial, S.lsum);
+    halo(S, S.g[0], st);
+}
+
+void net_output_grad(NetState &S, const float *gd, hipStream_t st) {
+    const int64_t nv = (int64_t)S.d.nx * S.d.ny * S.d.nz;
+    hipLaunchKernelGGL(k_out_grad, dim3(blocks_for(nv, 256)), dim3(256), 0, st, S.d, gd, (bf16 *)S.g[0]);
     halo(S, S.g[0], st);
 }
 

@@ -26,6 +26,7 @@ OM_PRECOND_LEARNED = 5
 OM_LEARNED_NWEIGHTS = 4 * 27 * 16 * 16
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
+OM_TUNE_LEARNED_ITERATIONS = 2
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
 
@@ -119,6 +120,7 @@ SIGNATURES = {
     "om_learned_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]),
     "om_learned_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "om_learned_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),

@@ -339,7 +339,7 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_last_solve_info(self._h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t)), self._h)
         return it.value, r.value, t.value
 
-    def set_tuning(self, tendency_kernel=None, kchunk=None):
+    def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None):
         """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';
         kchunk = z levels per block of the tiled kernel.  Results do not depend on them."""
         if tendency_kernel is not None:
@@ -347,6 +347,8 @@ class NonhydrostaticModel:
             _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_TENDENCY_KERNEL, v), self._h)
         if kchunk is not None:
             _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_KCHUNK, int(kchunk)), self._h)
+        if learned_iterations is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_LEARNED_ITERATIONS, int(learned_iterations)), self._h)
 
     def get_tuning(self):
         v = ctypes.c_int32()
@@ -410,6 +412,13 @@ class NonhydrostaticModel:
         a, b = ctypes.c_double(), ctypes.c_double()
         _abi.check(self._lib.om_learned_train(self._h, int(steps), float(lr), ctypes.byref(a), ctypes.byref(b)), self._h)
         return a.value, b.value
+
+    def learned_evaluate(self):
+        """Mean relative A-energy of the remaining error over the held pairs (1: the FFT
+        preconditioner alone)."""
+        x = ctypes.c_double()
+        _abi.check(self._lib.om_learned_evaluate(self._h, ctypes.byref(x)), self._h)
+        return x.value
 
     @staticmethod
     def _fp(a):
