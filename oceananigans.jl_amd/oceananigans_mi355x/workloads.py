@@ -8,10 +8,14 @@ wind_convection   : WENO5, SmagorinskyLilly(C=0.16), linear EOS (α=2e-4, β=8e-
 turbulence2d      : (Periodic, Periodic, Flat), WENO5, u and v ~ U(-1, 1).
 stretched_convection : WENO5 + AMD + BuoyancyTracer (N² = 1e-6 s⁻²) on a geometrically stretched z with
                     u = 0.1 m/s + U(-1e-3, 1e-3) inflow noise.
+seamount          : 10 km × 10 km × 1 km, z faces geometrically stretched (ratio 1.02, finest at the
+                    surface), GridFittedBottom Gaussian seamount (600 m high, e-folding 1.5 km) at the
+                    centre, WENO5 + AMD + BuoyancyTracer (N² = 1e-6 s⁻²), u = 0.1 m/s + 1e-3 noise
+                    (seed 3), conjugate-gradient pressure solver to a 1e-8 relative residual.
 """
 import numpy as np
 
-from .grids import Flat, Periodic, RectilinearGrid
+from .grids import Flat, GridFittedBottom, ImmersedBoundaryGrid, Periodic, RectilinearGrid
 from .models import field_nodes
 from .physics import (AnisotropicMinimumDissipation, BuoyancyTracer, FieldBoundaryConditions, FluxBoundaryCondition,
                       FPlane, GradientBoundaryCondition, LinearEquationOfState, SeawaterBuoyancy, SmagorinskyLilly, WENO)
@@ -76,4 +80,25 @@ def stretched_convection(N=(16, 16, 16), float_type=np.float64, seed=3, ratio=1.
     z = grid.znodes("c")
     Z = np.broadcast_to(z[None, None, :], tuple(N))
     init = {"b": 1e-6 * Z, "u": 0.1 + 1e-3 * uniform(seed + 1, tuple(N), -1, 1)}
+    return grid, kw, init
+
+
+def seamount(N=(256, 256, 128), float_type=np.float32, seed=3, preconditioner="default", reltol=1e-8):
+    from .physics import ConjugateGradientPoissonSolver
+    Lx = Ly = 10e3
+    Lz = 1e3
+    r = 1.02
+    d = [Lz * (r - 1) / (r ** N[2] - 1) * r ** q for q in range(N[2])]   # finest at the surface
+    faces = [0.0]
+    for dz in d:
+        faces.append(faces[-1] - dz)
+    z = np.array(faces[::-1])
+    z = z - z[0] - Lz
+    z[-1] = 0.0
+    under = RectilinearGrid(float_type=float_type, size=N, x=(0, Lx), y=(0, Ly), z=z)
+    hill = lambda x, y: -Lz + 600 * np.exp(-((x - Lx / 2) ** 2 + (y - Ly / 2) ** 2) / 1500 ** 2)
+    grid = ImmersedBoundaryGrid(under, GridFittedBottom(hill))
+    kw = dict(advection=WENO(), closure=AnisotropicMinimumDissipation(), buoyancy=BuoyancyTracer(), tracers=("b",),
+              pressure_solver=ConjugateGradientPoissonSolver(preconditioner=preconditioner, reltol=reltol, abstol=0))
+    init = {"u": 0.1 + 1e-3 * uniform(seed, tuple(N), -1, 1), "b": lambda x, y, z: 1e-6 * z}
     return grid, kw, init
