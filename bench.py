@@ -116,9 +116,10 @@ def run_cg_seamount(args):
     learned.learned_set_weights(w)
     learned.learned_log(iterations_per_solve=8, capacity=args.cg_pairs)
     t0 = time.perf_counter()
-    for _ in range(args.cg_rounds):
+    for rnd in range(args.cg_rounds):
         learned.time_step(20.0, 1)
-        learned.learned_train(args.cg_train_steps, lr=5e-4)
+        l0, l1 = learned.learned_train(args.cg_train_steps, lr=5e-4)
+        print(f"bench.py: seamount training round {rnd}: energy loss {l0:.3f} -> {l1:.3f}", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t0
     learned.learned_log(iterations_per_solve=8, capacity=24)
     state = om.prognostic_state(learned)

@@ -571,16 +571,19 @@ template <class T> struct ModelT : ModelBase {
         const int pk = cg_precond();
         cg_solve_with(pk);
         // the recomputed FP64 residual judges the learned preconditioner: a solve it leaves above
-        // ten times the tolerance is finished by FFT-preconditioned CG from where it stopped
-        if (pk == OM_PRECOND_LEARNED && cg_true_res_last > 10.0 * std::max(cg_tol_last, 1e-300)) {
+        // a hundred times the tolerance is finished by FFT-preconditioned CG from where it stopped
+        // (FFT-preconditioned CG itself ends with the recomputed residual up to ~15× the recursive
+        // one: the usual drift of the recursively updated residual)
+        if (pk == OM_PRECOND_LEARNED && !(cg_true_res_last <= 100.0 * std::max(cg_tol_last, 1e-300))) {
             const int it0 = cg_iterations;
+            const double tol0 = cg_tol_last;
             ln_fell_back = 1;
-            cg_solve_with(OM_PRECOND_FFT);
+            cg_solve_with(OM_PRECOND_FFT, tol0);   // the original absolute tolerance, not one relative to the small residual left
             cg_iterations += it0;
         }
     }
     double cg_tol_last = 0;
-    void cg_solve_with(const int pk) {
+    void cg_solve_with(const int pk, const double tol_override = -1.0) {
         double *x = cg_x;
         const double seps = std::sqrt((double)std::numeric_limits<T>::epsilon());
         const double reltol = cfg.cg_reltol < 0 ? (double)(T)seps : cfg.cg_reltol;
@@ -590,7 +593,7 @@ template <class T> struct ModelT : ModelBase {
         cg_halo_r(g, x, st);
         cg_residual(g, x, cg_b, cg_r, cg_partial, cg_out, st);
         const double res0 = std::sqrt(cg_read(1));
-        const double tol = std::max(reltol * res0, abstol);
+        const double tol = tol_override >= 0 ? tol_override : std::max(reltol * res0, abstol);
         cg_tol_last = tol;
         cg_gauge_norm(g, x, cg_r, 0.0, 0.0, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
         double res = std::sqrt(cg_read(1));

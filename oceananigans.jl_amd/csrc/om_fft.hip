@@ -34,23 +34,48 @@ __device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, int R, int n, const C2<T> *__restr
     const int nt = blockDim.x * blockDim.y;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     if ((n & (n - 1)) == 0) {
-        // power of two: every index split is a shift and a mask (integer division by a runtime
-        // value costs tens of instructions per butterfly on CDNA)
-        const int half = n >> 1;
-        const int lg = __builtin_ctz((unsigned)n), lh = lg - 1;
-        for (int sg = 0; sg < lg; ++sg) {
-            const int m = 1 << sg, lm = (half >> sg) << sg;   // lm = l·m = n/2
+        // power of two: Stockham stages of radix 4 (one radix-2 stage first when log2 n is odd),
+        // half as many LDS passes and barriers as radix 2.  A radix-R stage at sub-length m maps
+        // a[kk + jj·m + p·n/R] (p < R) to b[kk + R·jj·m + q·m] = w^{q·jj·m} Σ_p c_p e^{∓2πi pq/R}.
+        // Every index split is a shift and a mask (an integer division by a runtime value costs
+        // tens of instructions per butterfly on CDNA).
+        const int lg = __builtin_ctz((unsigned)n);
+        int lm = 0;   // log2 of the current sub-length m
+        if (lg & 1) {
+            const int half = n >> 1, lh = lg - 1;
             for (int t = tid; t < R * half; t += nt) {
-                const int r = t >> lh, bf = t & (half - 1);
-                const int jj = bf >> sg, kk = bf & (m - 1);
-                const int base = (r << lg) + kk + (jj << sg);
+                const int r = t >> lh, jj = t & (half - 1);
+                const int base = (r << lg) + jj;
                 const C2<T> c0 = a[base];
-                const C2<T> c1 = a[base + lm];
-                C2<T> w = tw[jj << sg];
+                const C2<T> c1 = a[base + half];
+                C2<T> w = tw[jj];
                 if (inverse) w.y = -w.y;
-                const int ob = (r << lg) + kk + (jj << (sg + 1));
+                const int ob = (r << lg) + (jj << 1);
                 b[ob] = cadd<T>(c0, c1);
-                b[ob + m] = cmul<T>(w, csub<T>(c0, c1));
+                b[ob + 1] = cmul<T>(w, csub<T>(c0, c1));
+            }
+            __syncthreads();
+            C2<T> *tmp = a; a = b; b = tmp;
+            lm = 1;
+        }
+        const int quarter = n >> 2, lq = lg - 2;
+        for (; lm < lg; lm += 2) {
+            const int m = 1 << lm;
+            for (int t = tid; t < R * quarter; t += nt) {
+                const int r = t >> lq, bf = t & (quarter - 1);
+                const int jj = bf >> lm, kk = bf & (m - 1);
+                const int ib = (r << lg) + kk + (jj << lm);
+                const C2<T> c0 = a[ib], c1 = a[ib + quarter], c2 = a[ib + 2 * quarter], c3 = a[ib + 3 * quarter];
+                const C2<T> t0 = cadd<T>(c0, c2), t1 = csub<T>(c0, c2), t2 = cadd<T>(c1, c3), d = csub<T>(c1, c3);
+                const C2<T> t3 = inverse ? cmk<T>(-d.y, d.x) : cmk<T>(d.y, -d.x);   // ±i·(c1 − c3)
+                const int e = jj << lm;
+                C2<T> w1 = tw[e], w2 = tw[2 * e], w3 = tw[3 * e];
+                if (inverse) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+                const int ob = (r << lg) + kk + (jj << (lm + 2));
+                b[ob] = cadd<T>(t0, t2);
+                b[ob + m] = cmul<T>(w1, cadd<T>(t1, t3));
+                b[ob + 2 * m] = cmul<T>(w2, csub<T>(t0, t2));
+                b[ob + 3 * m] = cmul<T>(w3, csub<T>(t1, t3));
             }
             __syncthreads();
             C2<T> *tmp = a; a = b; b = tmp;

@@ -83,7 +83,7 @@ def stretched_convection(N=(16, 16, 16), float_type=np.float64, seed=3, ratio=1.
     return grid, kw, init
 
 
-def seamount(N=(256, 256, 128), float_type=np.float32, seed=3, preconditioner="default", reltol=1e-8):
+def seamount(N=(256, 256, 128), float_type=np.float32, seed=3, preconditioner="default", reltol=1e-8, maxiter=1000):
     from .physics import ConjugateGradientPoissonSolver
     Lx = Ly = 10e3
     Lz = 1e3
@@ -99,6 +99,7 @@ def seamount(N=(256, 256, 128), float_type=np.float32, seed=3, preconditioner="d
     hill = lambda x, y: -Lz + 600 * np.exp(-((x - Lx / 2) ** 2 + (y - Ly / 2) ** 2) / 1500 ** 2)
     grid = ImmersedBoundaryGrid(under, GridFittedBottom(hill))
     kw = dict(advection=WENO(), closure=AnisotropicMinimumDissipation(), buoyancy=BuoyancyTracer(), tracers=("b",),
-              pressure_solver=ConjugateGradientPoissonSolver(preconditioner=preconditioner, reltol=reltol, abstol=0))
+              pressure_solver=ConjugateGradientPoissonSolver(preconditioner=preconditioner, reltol=reltol, abstol=0,
+                                                             maxiter=maxiter))
     init = {"u": 0.1 + 1e-3 * uniform(seed, tuple(N), -1, 1), "b": lambda x, y, z: 1e-6 * z}
     return grid, kw, init
