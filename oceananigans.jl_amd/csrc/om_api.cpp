@@ -278,6 +278,18 @@ template <class T> struct ModelT : ModelBase {
         g.idzc = upload(idzc) + g.Hz;
         g.idzf = upload(idzf) + g.Hz;
         g.stretched = hg.stretched;
+        g.amdr = nullptr;
+        if (g.tz != OM_FLAT) {
+            std::vector<T> amd((size_t)(g.Nz + 2) * 5);
+            const T Dfx = T(2) * hg.dx, Dfy = T(2) * hg.dy;
+            for (int k = -1; k <= g.Nz; ++k) {
+                const T Dfz = T(2) * hg.dzc[(size_t)(k + g.Hz)];
+                T *e = &amd[(size_t)(k + 1) * 5];
+                e[0] = Dfx / Dfz; e[1] = Dfz / Dfx; e[2] = Dfy / Dfz; e[3] = Dfz / Dfy;
+                e[4] = T(3) / ((T(1) / (Dfx * Dfx) + T(1) / (Dfy * Dfy)) + T(1) / (Dfz * Dfz));
+            }
+            g.amdr = upload(amd);
+        }
         // GridFittedBottom: kbot = the number of cells of a column whose centre lies at or below
         // the bottom height, compared in FT (grid_fitted_bottom.jl:118-147)
         g.kbot = nullptr;

@@ -139,9 +139,12 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
             continue;
         }
 
-        // ---- AMD
+        // ---- AMD (the level ratios come from the grid's table: same values, no per-thread division)
         const T Dfx = T(2) * g.dx, Dfy = T(2) * g.dy;
         const T Dfz0 = T(2) * g.dzc[k], Dfz1 = T(2) * g.dzc[k + 1];
+        const T *r0 = g.amdr + (size_t)(k + 1) * 5, *r1 = r0 + 5;
+        const T rxz0 = r0[0], rzx0 = r0[1], ryz0 = r0[2], rzy0 = r0[3];
+        const T rxz1 = r1[0], rzx1 = r1[1], ryz1 = r1[2], rzy1 = r1[3];
         T ndxv[4], ndyu[4];
         {
             const int di[4] = {0, 1, 0, 1}, dj[4] = {0, 0, 1, 1};
@@ -154,22 +157,22 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
         T ndxw[4], ndzu[4], ndyw_x[4];
         {
             // points (i,k), (i+1,k), (i,k+1), (i+1,k+1)
-            ndxw[0] = Dfx / Dfz0 * dx_w(W0, 0);            ndzu[0] = Dfz0 / Dfx * dz_u(U0, Um, 0, k);
-            ndxw[1] = Dfx / Dfz0 * dx_w(W0, 1);            ndzu[1] = Dfz0 / Dfx * dz_u(U0, Um, 1, k);
-            ndxw[2] = Dfx / Dfz1 * dx_w(W1, 0);            ndzu[2] = Dfz1 / Dfx * dz_u(Up, U0, 0, k + 1);
-            ndxw[3] = Dfx / Dfz1 * dx_w(W1, 1);            ndzu[3] = Dfz1 / Dfx * dz_u(Up, U0, 1, k + 1);
-            ndyw_x[0] = Dfy / Dfz0 * dy_w(W0, 0, 0);
-            ndyw_x[1] = Dfy / Dfz0 * dy_w(W0, 1, 0);
-            ndyw_x[2] = Dfy / Dfz1 * dy_w(W1, 0, 0);
-            ndyw_x[3] = Dfy / Dfz1 * dy_w(W1, 1, 0);
+            ndxw[0] = rxz0 * dx_w(W0, 0);            ndzu[0] = rzx0 * dz_u(U0, Um, 0, k);
+            ndxw[1] = rxz0 * dx_w(W0, 1);            ndzu[1] = rzx0 * dz_u(U0, Um, 1, k);
+            ndxw[2] = rxz1 * dx_w(W1, 0);            ndzu[2] = rzx1 * dz_u(Up, U0, 0, k + 1);
+            ndxw[3] = rxz1 * dx_w(W1, 1);            ndzu[3] = rzx1 * dz_u(Up, U0, 1, k + 1);
+            ndyw_x[0] = ryz0 * dy_w(W0, 0, 0);
+            ndyw_x[1] = ryz0 * dy_w(W0, 1, 0);
+            ndyw_x[2] = ryz1 * dy_w(W1, 0, 0);
+            ndyw_x[3] = ryz1 * dy_w(W1, 1, 0);
         }
         T ndyw[4], ndzv[4];
         {
             // points (j,k), (j+1,k), (j,k+1), (j+1,k+1)
-            ndyw[0] = Dfy / Dfz0 * dy_w(W0, 0, 0);         ndzv[0] = Dfz0 / Dfy * dz_v(V0, Vm, 0, k);
-            ndyw[1] = Dfy / Dfz0 * dy_w(W0, 0, 1);         ndzv[1] = Dfz0 / Dfy * dz_v(V0, Vm, 1, k);
-            ndyw[2] = Dfy / Dfz1 * dy_w(W1, 0, 0);         ndzv[2] = Dfz1 / Dfy * dz_v(Vp, V0, 0, k + 1);
-            ndyw[3] = Dfy / Dfz1 * dy_w(W1, 0, 1);         ndzv[3] = Dfz1 / Dfy * dz_v(Vp, V0, 1, k + 1);
+            ndyw[0] = ryz0 * dy_w(W0, 0, 0);         ndzv[0] = rzy0 * dz_v(V0, Vm, 0, k);
+            ndyw[1] = ryz0 * dy_w(W0, 0, 1);         ndzv[1] = rzy0 * dz_v(V0, Vm, 1, k);
+            ndyw[2] = ryz1 * dy_w(W1, 0, 0);         ndzv[2] = rzy1 * dz_v(Vp, V0, 0, k + 1);
+            ndyw[3] = ryz1 * dy_w(W1, 0, 1);         ndzv[3] = rzy1 * dz_v(Vp, V0, 1, k + 1);
         }
         T nS12[4], nS13[4], nS23[4];
 #pragma unroll
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
                    + A4(ndxv[q] * ndxv[q]) + A4(ndyu[q] * ndyu[q])
                    + A4(ndxw[q] * ndxw[q]) + A4(ndzu[q] * ndzu[q])
                    + A4(ndyw[q] * ndyw[q]) + A4(ndzv[q] * ndzv[q]);
-        const T delta2 = T(3) / ((T(1) / (Dfx * Dfx) + T(1) / (Dfy * Dfy)) + T(1) / (Dfz0 * Dfz0));
+        const T delta2 = r0[4];
         T nu = T(0);
         if (q2 != T(0)) {
             const T S11 = dxu, S22 = dyv, S33 = dzw;
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
                        + T(2) * A4(ndzu[q]) * A4(ndzv[q]) * A4(nS12[q])
                        + T(2) * dzw * A4(ndzu[q] * nS13[q]) + T(2) * dzw * A4(ndzv[q] * nS23[q]);
             const T r = (t1 + t2) + t3;
-            nu = -ph.c0 * delta2 * (r - T(0)) / q2;
+            nu = ft_div<T>(-ph.c0 * delta2 * (r - T(0)), q2);
         }
         C.s.nu[o] = fmax(T(0), nu);
 
@@ -216,7 +219,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
                 const T cy_uy = A4(ndyu[q]) * Iy * Ix + dyv * Iy2 + A4(ndyw_x[q]) * Iy * Iz;
                 const T cz_uz = A4(ndzu[q]) * Iz * Ix + A4(ndzv[q]) * Iz * Iy + dzw * Iz2;
                 const T theta = (cx_ux + cy_uy) + cz_uz;
-                kap = -ph.c1 * delta2 * theta / sig;
+                kap = ft_div<T>(-ph.c1 * delta2 * theta, sig);
             }
             C.s.kap[n][o] = fmax(T(0), kap);
         }

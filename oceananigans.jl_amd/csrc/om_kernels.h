@@ -119,6 +119,9 @@ template <> __device__ __forceinline__ float weno5<float>(float s0, float s1, fl
 // 1 / x: IEEE division in FP64 (the reference's rounding); the hardware reciprocal in FP32
 template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
 template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// a / b: IEEE in FP64, a times the hardware reciprocal of b in FP32
+template <class T> __device__ __forceinline__ T ft_div(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float ft_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 template <class T> __device__ __forceinline__ T beta2_q(T a, T b) {
 #pragma clang fp contract(off)

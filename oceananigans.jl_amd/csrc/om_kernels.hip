@@ -219,7 +219,7 @@ template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C) {
                    + T(2) * A4(ndzu[q]) * A4(ndzv[q]) * A4(nS12[q])
                    + T(2) * dzw * A4(ndzu[q] * nS13[q]) + T(2) * dzw * A4(ndzv[q] * nS23[q]);
         const T r = (t1 + t2) + t3;
-        nu = -C.ph.c0 * delta2 * (r - T(0)) / q2;
+        nu = ft_div<T>(-C.ph.c0 * delta2 * (r - T(0)), q2);
     }
     C.s.nu[o] = fmax(T(0), nu);
 
@@ -238,7 +238,7 @@ template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C) {
             const T cy_uy = A4(ndyu[q]) * Iy * Ix + dyv * Iy2 + A4(ndyw_x[q]) * Iy * Iz;
             const T cz_uz = A4(ndzu[q]) * Iz * Ix + A4(ndzv[q]) * Iz * Iy + dzw * Iz2;
             const T theta = (cx_ux + cy_uy) + cz_uz;
-            kap = -C.ph.c1 * delta2 * theta / sig;
+            kap = ft_div<T>(-C.ph.c1 * delta2 * theta, sig);
         }
         C.s.kap[n][o] = fmax(T(0), kap);
     }
@@ -260,7 +260,23 @@ template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C
     T bkm = C.buoy(i, j, Nz - 1);
     T p = -(T(0.5) * (bkm + bk)) * C.g.dzf[Nz];
     C.s.pHY[C.at(i, j, Nz - 1)] = p;
-    for (int k = Nz - 2; k >= 0; --k) {
+    // the buoyancy of 16 levels is loaded before their pressures are stored: a load placed after
+    // a store to pHY′ cannot be moved above it (the arrays might alias), which would expose one
+    // memory round trip per level
+    int k = Nz - 2;
+    for (; k >= 15; k -= 16) {
+        T bb[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) bb[q] = C.buoy(i, j, k - q);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            bk = bkm;
+            bkm = bb[q];
+            p -= (T(0.5) * (bkm + bk)) * C.g.dzf[k - q + 1];
+            C.s.pHY[C.at(i, j, k - q)] = p;
+        }
+    }
+    for (; k >= 0; --k) {
         bk = bkm;
         bkm = C.buoy(i, j, k);
         p -= (T(0.5) * (bkm + bk)) * C.g.dzf[k + 1];
