@@ -353,8 +353,19 @@ template <class T> struct ModelT : ModelBase {
         // the Fourier–tridiagonal plan serves the FT solver and, on stretched z, the CG preconditioner
         P.tridiagonal = (c.pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL ||
                          (c.pressure_solver == OM_SOLVER_CONJUGATE_GRADIENT && hg.stretched)) ? 1 : 0;
-        P.Lz_line = g.tz == OM_BOUNDED ? 2 * g.Nz : g.Nz;
+        P.zmode = g.tz != OM_BOUNDED ? 0 : (g.Nz % 2 == 0 ? 1 : 2);
+        P.Lz_line = P.zmode == 2 ? 2 * g.Nz : g.Nz;
+        P.Lz_norm = g.tz == OM_BOUNDED ? 2 * g.Nz : g.Nz;
         P.tw_z = upload(twiddles<T>(P.Lz_line));
+        {
+            std::vector<typename Vec2<T>::type> c(g.Nz);
+            for (int k = 0; k < g.Nz; ++k) {
+                const double a = -M_PI * (double)k / (2.0 * g.Nz);
+                c[k].x = (T)std::cos(a);
+                c[k].y = (T)std::sin(a);
+            }
+            P.tw_dct = upload(c);
+        }
         P.lx = upload(eigen<T>(g.Nx, hg.Lx, g.tx, 1));
         P.ly = upload(eigen<T>(g.Ny, hg.Ly, g.ty, 1));
         P.lz = upload(eigen<T>(g.Nz, hg.Lz, g.tz, g.tz == OM_BOUNDED ? 2 : 1));
@@ -839,6 +850,19 @@ static std::string validate(const om_config_t *c) {
     }
     if (c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL && c->topology[2] != OM_BOUNDED)
         return "the Fourier-tridiagonal solver needs a Bounded z";
+    {
+        // FFT lines are staged in LDS (om_kernels.h: fft_lines_per_block)
+        const size_t e = 2 * (size_t)c->precision;
+        const int xl = c->N[0] % 2 == 0 ? c->N[0] / 2 : c->N[0];
+        const int Nz = c->N[2];
+        bool ok = om::fft_lines_per_block(xl, xl, e) > 0 && om::fft_lines_per_block(c->N[1], c->N[1], e) > 0;
+        const bool z_tridiagonal = c->pressure_solver == OM_SOLVER_FOURIER_TRIDIAGONAL || c->z_faces;
+        if (c->topology[2] != OM_FLAT && !z_tridiagonal) {
+            const int L = (c->topology[2] == OM_BOUNDED && Nz % 2 == 1) ? 2 * Nz : Nz;
+            ok = ok && om::fft_lines_per_block(L, L + (c->topology[2] == OM_BOUNDED && Nz % 2 == 0 ? Nz : 0), e) > 0;
+        }
+        if (!ok) return "grid too large for the FFT pressure solver: a line of Nx/2, Ny or Nz points and its tables must fit in 64 KiB of LDS";
+    }
     if (c->bottom_height) {
         if (c->topology[2] != OM_BOUNDED) return "an immersed bottom needs a Bounded z";
         if (c->advection != OM_ADV_WENO5 && c->advection != OM_ADV_NONE)

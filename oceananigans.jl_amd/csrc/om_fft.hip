@@ -5,14 +5,15 @@
 //   X-forward  : source term (∇·U* formed on the fly, optionally ×Δzᶜ) → real-to-complex FFT
 //   Y-forward  : complex FFT along y                    (skipped when y is the last axis)
 //   last axis  : forward transform · eigenvalue division · backward transform in one kernel
-//                (Bounded z: the Neumann problem on N points is the periodic problem on the
-//                 2N-point even extension, so the z line is mirrored in LDS and solved there;
-//                 Fourier–tridiagonal: Thomas algorithm along z)
+//                (Bounded z: DCT-II / DCT-III pair through one complex FFT of length Nz, or the
+//                 2Nz-point even extension for odd Nz; Fourier–tridiagonal: Thomas algorithm)
 //   Y-backward, X-backward (complex-to-real, written into the pressure field's interior)
-// Each line transform is a radix-2 Stockham FFT in LDS for power-of-two lengths and a direct
-// DFT otherwise.  Reference: src/Solvers/fft_based_poisson_solver.jl:94-124,
+// Each line transform is an in-place radix-4 Stockham FFT in LDS for power-of-two lengths and a
+// direct DFT otherwise; the global loads that stage the lines are batched per thread.  Reference: src/Solvers/fft_based_poisson_solver.jl:94-124,
 // src/Solvers/fourier_tridiagonal_poisson_solver.jl:231-260, src/Solvers/batched_tridiagonal_solver.jl:211-243.
 #include <hip/hip_runtime.h>
+
+#include <stdexcept>
 
 #include "om_internal.h"
 #include "om_kernels.h"
@@ -27,58 +28,104 @@ template <class T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { C2<
 template <class T> __device__ __forceinline__ C2<T> cconj(C2<T> a) { a.y = -a.y; return a; }
 template <class T> __device__ __forceinline__ C2<T> cmk(T x, T y) { C2<T> r; r.x = x; r.y = y; return r; }
 
-// R lines of length n stored [r][n] in buffer a; b is scratch of the same size.
-// tw[m] = exp(-2πi m / n), m < n.  Returns the buffer that holds the transform (unnormalised).
+// Global → LDS staging.  get(t) issues the global loads of element t, put(t, v) stores it in LDS.
+// Each thread first issues the loads of B elements and only then stores them, so a wave keeps B
+// loads in flight (a plain load-then-store loop waits out the full memory latency per element).
+template <int B, class Get, class Put>
+__device__ __forceinline__ void staged_copy(int count, Get get, Put put) {
+    const int nt = blockDim.x;
+    for (int t0 = threadIdx.x; t0 < count; t0 += B * nt) {
+        decltype(get(0)) v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int t = t0 + u * nt;
+            if (t < count) v[u] = get(t);
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int t = t0 + u * nt;
+            if (t < count) put(t, v[u]);
+        }
+    }
+}
+
+// R lines of length n, row r at a + r·ld; tw (in LDS) holds exp(-2πi m / n), m < n.
+// Power-of-two n: in-place Stockham stages of radix 4 (one radix-2 stage first when log2 n is
+// odd).  A radix-R stage at sub-length m maps a[kk + jj·m + p·n/R] (p < R) to
+// a[kk + R·jj·m + q·m] = w^{q·jj·m} Σ_p c_p e^{∓2πi pq/R}; every thread reads its butterflies'
+// inputs into registers, the block synchronises, and the outputs overwrite the same buffer, so
+// one buffer serves every stage.  Each thread holds at most 2 radix-4 (4 radix-2) butterflies:
+// R·n ≤ 8·blockDim.x.  Index splits are shifts and masks.
+// Other n: a direct DFT into b.  Returns the buffer that holds the (unnormalised) transform.
 template <class T>
-__device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, int R, int n, const C2<T> *__restrict__ tw, bool inverse) {
-    const int nt = blockDim.x * blockDim.y;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+__device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, int R, int n, int ld, const C2<T> *tw, bool inverse) {
+    const int nt = blockDim.x;
+    const int tid = threadIdx.x;
     if ((n & (n - 1)) == 0) {
-        // power of two: Stockham stages of radix 4 (one radix-2 stage first when log2 n is odd),
-        // half as many LDS passes and barriers as radix 2.  A radix-R stage at sub-length m maps
-        // a[kk + jj·m + p·n/R] (p < R) to b[kk + R·jj·m + q·m] = w^{q·jj·m} Σ_p c_p e^{∓2πi pq/R}.
-        // Every index split is a shift and a mask (an integer division by a runtime value costs
-        // tens of instructions per butterfly on CDNA).
         const int lg = __builtin_ctz((unsigned)n);
         int lm = 0;   // log2 of the current sub-length m
         if (lg & 1) {
-            const int half = n >> 1, lh = lg - 1;
-            for (int t = tid; t < R * half; t += nt) {
-                const int r = t >> lh, jj = t & (half - 1);
-                const int base = (r << lg) + jj;
-                const C2<T> c0 = a[base];
-                const C2<T> c1 = a[base + half];
-                C2<T> w = tw[jj];
-                if (inverse) w.y = -w.y;
-                const int ob = (r << lg) + (jj << 1);
-                b[ob] = cadd<T>(c0, c1);
-                b[ob + 1] = cmul<T>(w, csub<T>(c0, c1));
+            const int half = n >> 1, lh = lg - 1, nb = R * half;
+            C2<T> c0[4], c1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = tid + u * nt;
+                if (t < nb) {
+                    const int base = (t >> lh) * ld + (t & (half - 1));
+                    c0[u] = a[base];
+                    c1[u] = a[base + half];
+                }
             }
             __syncthreads();
-            C2<T> *tmp = a; a = b; b = tmp;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = tid + u * nt;
+                if (t < nb) {
+                    const int jj = t & (half - 1);
+                    C2<T> w = tw[jj];
+                    if (inverse) w.y = -w.y;
+                    const int ob = (t >> lh) * ld + (jj << 1);
+                    a[ob] = cadd<T>(c0[u], c1[u]);
+                    a[ob + 1] = cmul<T>(w, csub<T>(c0[u], c1[u]));
+                }
+            }
+            __syncthreads();
             lm = 1;
         }
-        const int quarter = n >> 2, lq = lg - 2;
+        const int quarter = n >> 2, lq = lg - 2, nb = R * quarter;
         for (; lm < lg; lm += 2) {
             const int m = 1 << lm;
-            for (int t = tid; t < R * quarter; t += nt) {
-                const int r = t >> lq, bf = t & (quarter - 1);
-                const int jj = bf >> lm, kk = bf & (m - 1);
-                const int ib = (r << lg) + kk + (jj << lm);
-                const C2<T> c0 = a[ib], c1 = a[ib + quarter], c2 = a[ib + 2 * quarter], c3 = a[ib + 3 * quarter];
-                const C2<T> t0 = cadd<T>(c0, c2), t1 = csub<T>(c0, c2), t2 = cadd<T>(c1, c3), d = csub<T>(c1, c3);
-                const C2<T> t3 = inverse ? cmk<T>(-d.y, d.x) : cmk<T>(d.y, -d.x);   // ±i·(c1 − c3)
-                const int e = jj << lm;
-                C2<T> w1 = tw[e], w2 = tw[2 * e], w3 = tw[3 * e];
-                if (inverse) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-                const int ob = (r << lg) + kk + (jj << (lm + 2));
-                b[ob] = cadd<T>(t0, t2);
-                b[ob + m] = cmul<T>(w1, cadd<T>(t1, t3));
-                b[ob + 2 * m] = cmul<T>(w2, csub<T>(t0, t2));
-                b[ob + 3 * m] = cmul<T>(w3, csub<T>(t1, t3));
+            C2<T> c[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = tid + u * nt;
+                if (t < nb) {
+                    const int ib = (t >> lq) * ld + (t & (quarter - 1));   // = row + kk + jj·m
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) c[u][p] = a[ib + p * quarter];
+                }
             }
             __syncthreads();
-            C2<T> *tmp = a; a = b; b = tmp;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int t = tid + u * nt;
+                if (t < nb) {
+                    const int bf = t & (quarter - 1);
+                    const int jj = bf >> lm, kk = bf & (m - 1);
+                    const C2<T> t0 = cadd<T>(c[u][0], c[u][2]), t1 = csub<T>(c[u][0], c[u][2]);
+                    const C2<T> t2 = cadd<T>(c[u][1], c[u][3]), d = csub<T>(c[u][1], c[u][3]);
+                    const C2<T> t3 = inverse ? cmk<T>(-d.y, d.x) : cmk<T>(d.y, -d.x);   // ±i·(c1 − c3)
+                    const int e = jj << lm;
+                    C2<T> w1 = tw[e], w2 = tw[2 * e], w3 = tw[3 * e];
+                    if (inverse) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+                    const int ob = (t >> lq) * ld + kk + (jj << (lm + 2));
+                    a[ob] = cadd<T>(t0, t2);
+                    a[ob + m] = cmul<T>(w1, cadd<T>(t1, t3));
+                    a[ob + 2 * m] = cmul<T>(w2, csub<T>(t0, t2));
+                    a[ob + 3 * m] = cmul<T>(w3, csub<T>(t1, t3));
+                }
+            }
+            __syncthreads();
         }
         return a;
     }
@@ -88,13 +135,26 @@ __device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, int R, int n, const C2<T> *__restr
         for (int q = 0; q < n; ++q) {
             C2<T> w = tw[(int)(((int64_t)ko * q) % n)];
             if (inverse) w.y = -w.y;
-            s = cadd<T>(s, cmul<T>(a[r * n + q], w));
+            s = cadd<T>(s, cmul<T>(a[r * ld + q], w));
         }
-        b[t] = s;
+        b[r * ld + ko] = s;
     }
     __syncthreads();
     return b;
 }
+
+template <class T> __device__ __forceinline__ void load_table(C2<T> *dst, const C2<T> *src, int n) {
+    staged_copy<4>(n, [&](int t) { return src[t]; }, [&](int t, C2<T> v) { dst[t] = v; });
+}
+
+constexpr int kFftThreads = 512;   // threads per block of the line kernels (a multiple of R ≤ 16)
+
+// Line staging geometry, chosen on the host: R lines per block, row stride ld (n + 1 when R > 1,
+// so the R rows start in different LDS banks), tables first, then buffer a, then b (only for a
+// length that is not a power of two).
+struct LineGeom {
+    int R, lgR, ld;
+};
 
 // ---------------------------------------------------------------------------------------------
 // X-forward: source term → R2C along x.  One block handles R rows (j, k).
@@ -105,10 +165,10 @@ template <class T> struct XFwdArgs {
     Ctx<T> C;
     const T *src;
     C2<T> *S;
-    const C2<T> *tw;      // length Nx/2 table (for the half-length complex FFT)
+    const C2<T> *tw;      // length Nx/2 table (for the half-length complex FFT), Nx when Nx is odd
     const C2<T> *post;    // exp(-2πi k / Nx), k = 0 .. Nx/2
     int scale_dzc;
-    int R;
+    LineGeom L;
     FastDiv fNx, fNy, fMx;
 };
 
@@ -120,43 +180,47 @@ template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const
     return C.divU(i, j, k, ip, jp);
 }
 
-template <class T> __global__ void __launch_bounds__(256) k_xfwd(XFwdArgs<T> A) {
+template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
     const Ctx<T> &C = A.C;
-    const int Nx = C.g.Nx, Ny = C.g.Ny, Nz = C.g.Nz;
+    const int Nx = C.g.Nx, Ny = C.g.Ny;
     const bool half = (Nx & 1) == 0;
     const int M = half ? Nx / 2 : Nx;      // line length of the complex FFT
     const int Mx = Nx / 2 + 1;
-    const int R = A.R;
-    C2<T> *a = reinterpret_cast<C2<T> *>(smem_raw);
-    C2<T> *b = a + R * M;
+    const int R = A.L.R, ld = A.L.ld;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *a = twl + M;
+    C2<T> *b = a + R * ld;
     const int row0 = blockIdx.x * R;
-    const int nrows = Ny * Nz;
-    const int tid = threadIdx.x;
-    if (!half) for (int t = tid; t < R * M; t += blockDim.x) a[t] = cmk<T>(T(0), T(0));
+    const int nrows = Ny * C.g.Nz;
+    load_table<T>(twl, A.tw, M);
+    T *ap = reinterpret_cast<T *>(a);
+    staged_copy<4>(R * Nx,
+        [&](int t) {
+            const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
+            const int row = row0 + r;
+            T x = T(0);
+            if (row < nrows) {
+                const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
+                x = source_at<T>(C, A.src, i, j, k);
+                if (A.scale_dzc) x = C.g.dzc[k] * x;
+            }
+            return x;
+        },
+        [&](int t, T x) {
+            const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
+            if (half) ap[2 * (r * ld + (i >> 1)) + (i & 1)] = x;
+            else a[r * ld + i] = cmk<T>(x, T(0));
+        });
     __syncthreads();
-    for (int t = tid; t < R * Nx; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
-        const int row = row0 + r;
-        T x = T(0);
-        if (row < nrows) {
-            const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
-            x = source_at<T>(C, A.src, i, j, k);
-            if (A.scale_dzc) x = C.g.dzc[k] * x;
-        }
-        T *ap = reinterpret_cast<T *>(a);
-        if (half) ap[2 * (r * M + (i >> 1)) + (i & 1)] = x;
-        else ap[2 * (r * M + i)] = x;
-    }
-    __syncthreads();
-    C2<T> *Z = lds_fft<T>(a, b, R, M, A.tw, false);
-    for (int t = tid; t < R * Mx; t += blockDim.x) {
+    const C2<T> *Z = lds_fft<T>(a, b, R, M, ld, twl, false);
+    for (int t = threadIdx.x; t < R * Mx; t += blockDim.x) {
         const int r = (int)fdiv((uint32_t)t, A.fMx), kk = t - r * Mx;
         const int row = row0 + r;
         if (row >= nrows) continue;
-        if (!half) { A.S[(int64_t)row * Mx + kk] = Z[r * M + kk]; continue; }
-        const C2<T> zk = Z[r * M + (kk == M ? 0 : kk)];
-        const C2<T> zc = cconj<T>(Z[r * M + (kk == 0 ? 0 : M - kk)]);
+        if (!half) { A.S[(int64_t)row * Mx + kk] = Z[r * ld + kk]; continue; }
+        const C2<T> zk = Z[r * ld + (kk == M ? 0 : kk)];
+        const C2<T> zc = cconj<T>(Z[r * ld + (kk == 0 ? 0 : M - kk)]);
         const C2<T> E = cmk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
         // O = (zk - zc) / (2i) = (-i/2)(zk - zc)
         const C2<T> d = csub<T>(zk, zc);
@@ -176,59 +240,71 @@ template <class T> struct XBwdArgs {
     int dense;
     T scale;           // 1 / (Nx · Ny · Nz-normalisations)
     T divisor;         // the written value is (x · scale) / divisor
-    int R;
+    LineGeom L;
     FastDiv fNx, fNy, fM;
 };
 
-template <class T> __global__ void __launch_bounds__(256) k_xbwd(XBwdArgs<T> A) {
+template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
-    const int Nx = A.g.Nx, Ny = A.g.Ny, Nz = A.g.Nz;
+    const int Nx = A.g.Nx, Ny = A.g.Ny;
     const bool half = (Nx & 1) == 0;
     const int M = half ? Nx / 2 : Nx;
     const int Mx = Nx / 2 + 1;
-    const int R = A.R;
-    C2<T> *a = reinterpret_cast<C2<T> *>(smem_raw);
-    C2<T> *b = a + R * M;
+    const int R = A.L.R, ld = A.L.ld;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *a = twl + M;
+    C2<T> *b = a + R * ld;
     const int row0 = blockIdx.x * R;
-    const int nrows = Ny * Nz;
-    const int tid = threadIdx.x;
-    for (int t = tid; t < R * M; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
-        const int row = row0 + r;
-        C2<T> Zk = cmk<T>(T(0), T(0));
-        if (row < nrows) {
-            if (!half) {
-                Zk = kk < Mx ? A.S[(int64_t)row * Mx + kk] : cconj<T>(A.S[(int64_t)row * Mx + (Nx - kk)]);
-            } else {
-                const C2<T> Xk = A.S[(int64_t)row * Mx + kk];
-                const C2<T> Xc = cconj<T>(A.S[(int64_t)row * Mx + (M - kk)]);
-                const C2<T> E = cmk<T>(T(0.5) * (Xk.x + Xc.x), T(0.5) * (Xk.y + Xc.y));
-                const C2<T> d = csub<T>(Xk, Xc);
-                // O = (Xk - conj(X_{M-k})) · e^{+2πik/N} / 2 ;  Z = E + i O
-                const C2<T> O = cmul<T>(cmk<T>(T(0.5) * d.x, T(0.5) * d.y), cconj<T>(A.post[kk]));
-                Zk = cmk<T>(E.x - O.y, E.y + O.x);
+    const int nrows = Ny * A.g.Nz;
+    load_table<T>(twl, A.tw, M);
+    staged_copy<4>(R * M,
+        [&](int t) {
+            const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
+            const int row = row0 + r;
+            C2<T> Zk = cmk<T>(T(0), T(0));
+            if (row < nrows) {
+                if (!half) {
+                    Zk = kk < Mx ? A.S[(int64_t)row * Mx + kk] : cconj<T>(A.S[(int64_t)row * Mx + (Nx - kk)]);
+                } else {
+                    const C2<T> Xk = A.S[(int64_t)row * Mx + kk];
+                    const C2<T> Xc = cconj<T>(A.S[(int64_t)row * Mx + (M - kk)]);
+                    const C2<T> E = cmk<T>(T(0.5) * (Xk.x + Xc.x), T(0.5) * (Xk.y + Xc.y));
+                    const C2<T> d = csub<T>(Xk, Xc);
+                    // O = (Xk - conj(X_{M-k})) · e^{+2πik/N} / 2 ;  Z = E + i O
+                    const C2<T> O = cmul<T>(cmk<T>(T(0.5) * d.x, T(0.5) * d.y), cconj<T>(A.post[kk]));
+                    Zk = cmk<T>(E.x - O.y, E.y + O.x);
+                }
             }
-        }
-        a[r * M + kk] = Zk;
-    }
+            return Zk;
+        },
+        [&](int t, C2<T> v) {
+            const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
+            a[r * ld + kk] = v;
+        });
     __syncthreads();
-    C2<T> *z = lds_fft<T>(a, b, R, M, A.tw, true);
-    for (int t = tid; t < R * Nx; t += blockDim.x) {
+    const C2<T> *z = lds_fft<T>(a, b, R, M, ld, twl, true);
+    for (int t = threadIdx.x; t < R * Nx; t += blockDim.x) {
         const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
         const int row = row0 + r;
         if (row >= nrows) continue;
         const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
         T x;
-        if (half) { const C2<T> zz = z[r * M + (i >> 1)]; x = (i & 1) ? zz.y : zz.x; }
-        else x = z[r * M + i].x;
+        if (half) { const C2<T> zz = z[r * ld + (i >> 1)]; x = (i & 1) ? zz.y : zz.x; }
+        else x = z[r * ld + i].x;
         x = (x * A.scale) / A.divisor;
         if (A.dense) A.out[(int64_t)i + (int64_t)Nx * ((int64_t)j + (int64_t)Ny * k)] = x;
         else A.out[idx3(i, j, k, A.g.sy, A.g.sz)] = x;
     }
 }
 
+// −v / den, and 0 for the zero mode
+template <class T> __device__ __forceinline__ C2<T> eig_divide(C2<T> v, T den, bool zero_mode) {
+    return zero_mode ? cmk<T>(T(0), T(0)) : cmk<T>(ft_div<T>(-v.x, den), ft_div<T>(-v.y, den));
+}
+
 // ---------------------------------------------------------------------------------------------
-// Y pass: complex FFT along y for R consecutive kx columns at fixed k.
+// Y pass: complex FFT along y.  Lines ℓ = k·Mx + kx (ℓ < Mx·Nz), R consecutive lines per block;
+// element j of line ℓ sits at S[(k·Ny + j)·Mx + kx].
 // fused_divide: forward → divide by eigenvalues → backward (y is the last non-Flat axis).
 // ---------------------------------------------------------------------------------------------
 template <class T> struct YArgs {
@@ -239,94 +315,128 @@ template <class T> struct YArgs {
     int inverse;
     int fused_divide;
     const T *lx, *ly;   // eigenvalues (fused)
-    int R, lgR;         // rows per block (a power of two) and its log2
+    LineGeom L;
+    FastDiv fMx;
 };
 
-template <class T> __global__ void __launch_bounds__(256) k_ypass(YArgs<T> A) {
+template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
-    const int Ny = A.g.Ny, Mx = A.Mx, R = A.R;
-    C2<T> *a = reinterpret_cast<C2<T> *>(smem_raw);
-    C2<T> *b = a + R * Ny;
-    const int kx0 = blockIdx.x * R;
-    const int k = blockIdx.y;
-    const int tid = threadIdx.x;
-    for (int t = tid; t < R * Ny; t += blockDim.x) {
-        const int r = t & (R - 1), j = t >> A.lgR;
-        const int kx = kx0 + r;
-        a[r * Ny + j] = kx < Mx ? A.S[((int64_t)k * Ny + j) * Mx + kx] : cmk<T>(T(0), T(0));
-    }
+    const int Ny = A.g.Ny, Mx = A.Mx, R = A.L.R, lgR = A.L.lgR, ld = A.L.ld;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *a = twl + Ny;
+    C2<T> *b = a + R * ld;
+    const int nlines = Mx * A.g.Nz;
+    // blockDim.x is a multiple of R, so every element a thread stages belongs to the same line
+    const int r = threadIdx.x & (R - 1);
+    const int line = blockIdx.x * R + r;
+    const bool valid = line < nlines;
+    const int k = (int)fdiv((uint32_t)line, A.fMx), kx = line - k * Mx;
+    C2<T> *col = A.S + (int64_t)k * Ny * Mx + kx;
+    load_table<T>(twl, A.tw, Ny);
+    staged_copy<8>(R * Ny,
+        [&](int t) { return valid ? col[(int64_t)(t >> lgR) * Mx] : cmk<T>(T(0), T(0)); },
+        [&](int t, C2<T> v) { a[r * ld + (t >> lgR)] = v; });
     __syncthreads();
-    C2<T> *res = lds_fft<T>(a, b, R, Ny, A.tw, A.inverse != 0);
+    C2<T> *res = lds_fft<T>(a, b, R, Ny, ld, twl, A.inverse != 0);
     if (A.fused_divide) {
         C2<T> *other = (res == a) ? b : a;
-        for (int t = tid; t < R * Ny; t += blockDim.x) {
-            const int r = t & (R - 1), j = t >> A.lgR;
-            const int kx = kx0 + r;
-            C2<T> v = res[r * Ny + j];
-            if (kx < Mx) {
-                const T den = A.lx[kx] + A.ly[j];
-                v = (kx == 0 && j == 0) ? cmk<T>(T(0), T(0)) : cmk<T>(-v.x / den, -v.y / den);
-            }
-            res[r * Ny + j] = v;
+        for (int t = threadIdx.x; t < R * Ny; t += blockDim.x) {
+            const int j = t >> lgR;
+            if (valid) res[r * ld + j] = eig_divide<T>(res[r * ld + j], A.lx[kx] + A.ly[j], kx == 0 && j == 0);
         }
         __syncthreads();
-        res = lds_fft<T>(res, other, R, Ny, A.tw, true);
+        res = lds_fft<T>(res, other, R, Ny, ld, twl, true);
     }
-    for (int t = tid; t < R * Ny; t += blockDim.x) {
-        const int r = t & (R - 1), j = t >> A.lgR;
-        const int kx = kx0 + r;
-        if (kx < Mx) A.S[((int64_t)k * Ny + j) * Mx + kx] = res[r * Ny + j];
+    if (!valid) return;
+    for (int t = threadIdx.x; t < R * Ny; t += blockDim.x) {
+        const int j = t >> lgR;
+        col[(int64_t)j * Mx] = res[r * ld + j];
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Z pass (last axis): forward transform, eigenvalue division, backward transform.
-// Bounded z: even extension of length 2Nz (the mirrored periodic problem), Periodic z: length Nz.
+// Z pass (last axis): forward transform, eigenvalue division, backward transform.  Lines
+// ℓ = j·Mx + kx (ℓ < Mx·Ny), R consecutive lines per block; element k of line ℓ sits at
+// S[k·Ny·Mx + ℓ], so a block reads R contiguous values per level.
+//   zmode 0 (Periodic z): FFT of length Nz.
+//   zmode 1 (Bounded z, Nz even): the DCT-II of length N = Nz through one complex FFT of length N
+//     (Makhoul 1980): v = (x0, x2, …, x_{N−2}, x_{N−1}, …, x3, x1), V = FFT(v),
+//     X_k = c_k V_k + c̄_k V_{N−k} with c_k = e^{−iπk/2N}; after the division the inverse is
+//     V'_k = c̄_k (Φ_k − i Φ_{N−k}) (Φ_N = 0), v' = IFFT(V'), x read back through the same
+//     permutation.  Each thread handles the pair (k, N−k).  The result equals the inverse
+//     transform of the 2N-point even extension, so it shares that normalisation (1 / 2N).
+//   zmode 2 (Bounded z, Nz odd): FFT of the 2N-point even extension, mirrored in LDS.
 // ---------------------------------------------------------------------------------------------
 template <class T> struct ZArgs {
     DevGrid<T> g;
     C2<T> *S;
-    const C2<T> *tw;    // table for length L
-    int Mx, L;          // L = 2Nz (Bounded) or Nz (Periodic)
-    const T *lx, *ly, *lz;   // lz has L entries
-    int R, lgR;
+    const C2<T> *tw;         // table for the FFT length L
+    const C2<T> *dct;        // c_k, k < Nz (zmode 1)
+    int Mx, L, zmode;        // L = Nz (zmodes 0, 1) or 2Nz (zmode 2)
+    const T *lx, *ly, *lz;   // lz: Nz entries (Periodic), 2Nz (Bounded, symmetric)
+    LineGeom G;
 };
 
-template <class T> __global__ void __launch_bounds__(256) k_zpass(ZArgs<T> A) {
+template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
-    const int Ny = A.g.Ny, Nz = A.g.Nz, Mx = A.Mx, R = A.R, L = A.L;
-    const bool mirror = (L == 2 * Nz);
-    C2<T> *a = reinterpret_cast<C2<T> *>(smem_raw);
-    C2<T> *b = a + R * L;
-    const int kx0 = blockIdx.x * R;
-    const int j = blockIdx.y;
-    const int tid = threadIdx.x;
-    for (int t = tid; t < R * Nz; t += blockDim.x) {
-        const int r = t & (R - 1), k = t >> A.lgR;
-        const int kx = kx0 + r;
-        const C2<T> v = kx < Mx ? A.S[((int64_t)k * Ny + j) * Mx + kx] : cmk<T>(T(0), T(0));
-        a[r * L + k] = v;
-        if (mirror) a[r * L + (L - 1 - k)] = v;
-    }
+    const int Ny = A.g.Ny, Nz = A.g.Nz, Mx = A.Mx, L = A.L, zmode = A.zmode;
+    const int R = A.G.R, lgR = A.G.lgR, ld = A.G.ld;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *dctl = twl + L;
+    C2<T> *a = dctl + (zmode == 1 ? Nz : 0);
+    C2<T> *b = a + R * ld;
+    const int nlines = Mx * Ny;
+    const int r = threadIdx.x & (R - 1);   // blockDim.x is a multiple of R
+    const int line = blockIdx.x * R + r;
+    const bool valid = line < nlines;
+    const int j = valid ? line / Mx : 0, kx = line - j * Mx;
+    const int64_t plane = (int64_t)Ny * Mx;
+    C2<T> *col = A.S + line;
+    load_table<T>(twl, A.tw, L);
+    if (zmode == 1) load_table<T>(dctl, A.dct, Nz);
+    staged_copy<8>(R * Nz,
+        [&](int t) { return valid ? col[(int64_t)(t >> lgR) * plane] : cmk<T>(T(0), T(0)); },
+        [&](int t, C2<T> v) {
+            const int k = t >> lgR;
+            if (zmode == 1) a[r * ld + ((k & 1) ? Nz - ((k + 1) >> 1) : (k >> 1))] = v;
+            else {
+                a[r * ld + k] = v;
+                if (zmode == 2) a[r * ld + (L - 1 - k)] = v;
+            }
+        });
     __syncthreads();
-    C2<T> *res = lds_fft<T>(a, b, R, L, A.tw, false);
+    C2<T> *res = lds_fft<T>(a, b, R, L, ld, twl, false);
     C2<T> *other = (res == a) ? b : a;
-    for (int t = tid; t < R * L; t += blockDim.x) {
-        const int r = t & (R - 1), k = t >> A.lgR;
-        const int kx = kx0 + r;
-        C2<T> v = res[r * L + k];
-        if (kx < Mx) {
-            const T den = (A.lx[kx] + A.ly[j]) + A.lz[k];
-            v = (kx == 0 && j == 0 && k == 0) ? cmk<T>(T(0), T(0)) : cmk<T>(-v.x / den, -v.y / den);
+    const T lxy = valid ? A.lx[kx] + A.ly[j] : T(1);
+    const bool zero_col = kx == 0 && j == 0;
+    if (zmode == 1) {
+        const int hN = Nz >> 1;
+        for (int t = threadIdx.x; t < R * (hN + 1); t += blockDim.x) {
+            const int kk = t >> lgR, p = kk == 0 ? 0 : Nz - kk;
+            C2<T> *row = res + r * ld;
+            const C2<T> Vk = row[kk], Vp = row[p];
+            const C2<T> ck = dctl[kk], cp = dctl[p];
+            const C2<T> Xk = cadd<T>(cmul<T>(ck, Vk), cmul<T>(cconj<T>(ck), Vp));
+            const C2<T> Xp = cadd<T>(cmul<T>(cp, Vp), cmul<T>(cconj<T>(cp), Vk));
+            const C2<T> Fk = eig_divide<T>(Xk, lxy + A.lz[kk], zero_col && kk == 0);
+            const C2<T> Fp = kk == 0 ? cmk<T>(T(0), T(0)) : eig_divide<T>(Xp, lxy + A.lz[p], false);
+            // V'_k = c̄_k (Φ_k − i Φ_{N−k}),  −i·z = (z.y, −z.x)
+            row[kk] = cmul<T>(cconj<T>(ck), cmk<T>(Fk.x + Fp.y, Fk.y - Fp.x));
+            if (p != kk) row[p] = cmul<T>(cconj<T>(cp), cmk<T>(Fp.x + Fk.y, Fp.y - Fk.x));
         }
-        res[r * L + k] = v;
+    } else {
+        for (int t = threadIdx.x; t < R * L; t += blockDim.x) {
+            const int k = t >> lgR;
+            res[r * ld + k] = eig_divide<T>(res[r * ld + k], lxy + A.lz[k], zero_col && k == 0);
+        }
     }
     __syncthreads();
-    res = lds_fft<T>(res, other, R, L, A.tw, true);
-    for (int t = tid; t < R * Nz; t += blockDim.x) {
-        const int r = t & (R - 1), k = t >> A.lgR;
-        const int kx = kx0 + r;
-        if (kx < Mx) A.S[((int64_t)k * Ny + j) * Mx + kx] = res[r * L + k];
+    res = lds_fft<T>(res, other, R, L, ld, twl, true);
+    if (!valid) return;
+    for (int t = threadIdx.x; t < R * Nz; t += blockDim.x) {
+        const int k = t >> lgR;
+        const int pos = zmode == 1 ? ((k & 1) ? Nz - ((k + 1) >> 1) : (k >> 1)) : k;
+        col[(int64_t)k * plane] = res[r * ld + pos];
     }
 }
 
@@ -397,14 +507,17 @@ template <class T> __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A)
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-static int rows_for(int n, size_t elem, size_t budget) {
-    int r = (int)(budget / (2 * (size_t)n * elem));
-    if (r < 1) return 1;
-    int p = 1;
-    while (p * 2 <= r) p *= 2;   // a power of two, so row indices split with shifts
-    return p;
-}
 static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
+
+// R lines of length n with `tables` complex table entries in front (om_kernels.h chooses R)
+static LineGeom line_geom(int n, int tables, size_t e, size_t &lds) {
+    const int R = fft_lines_per_block(n, tables, e);
+    if (R < 1) throw std::runtime_error("FFT line does not fit in LDS");
+    const int ld = R > 1 ? n + 1 : n;
+    const bool p2 = (n & (n - 1)) == 0;
+    lds = ((size_t)(p2 ? 1 : 2) * R * ld + tables) * e;
+    return LineGeom{R, ilog2(R), ld};
+}
 
 template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
                                        T divisor, hipStream_t st) {
@@ -415,16 +528,20 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     const int M = (Nx & 1) == 0 ? Nx / 2 : Nx;
     const int Mx = Nx / 2 + 1;
     const int nrows = Ny * Nz;
-    auto rows = [&](int n) { int r = rows_for(n, e, budget); return r > 16 ? 16 : r; };
+    size_t lds = 0;
     {
-        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, rows(M),
+        const LineGeom L = line_geom(M, M, e, lds);
+        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L,
                       make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx)};
-        hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + A.R - 1) / A.R), dim3(256), 2 * A.R * M * e, st, A);
+        hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
     const bool z_active = g.tz != OM_FLAT;
+    const LineGeom Ly = line_geom(Ny, Ny, e, lds);
+    const size_t lds_y = lds;
+    const int ygrid = (Mx * Nz + Ly.R - 1) / Ly.R;
     {
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 0, z_active ? 0 : 1, P.lx, P.ly, rows(Ny), ilog2(rows(Ny))};
-        hipLaunchKernelGGL(k_ypass<T>, dim3((Mx + A.R - 1) / A.R, Nz), dim3(256), 2 * A.R * Ny * e, st, A);
+        YArgs<T> A{g, P.S, P.tw_y, Mx, 0, z_active ? 0 : 1, P.lx, P.ly, Ly, make_fastdiv((uint32_t)Mx)};
+        hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     T scale = T(1) / (T(M) * T(Ny));
     if (z_active) {
@@ -434,17 +551,19 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
             while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
             hipLaunchKernelGGL(k_tridiag<T>, dim3((Mx + bw - 1) / bw, Ny), dim3(bw), (size_t)Nz * bw * sizeof(T), st, A);
         } else {
-            ZArgs<T> A{g, P.S, P.tw_z, Mx, P.Lz_line, P.lx, P.ly, P.lz, rows(P.Lz_line), ilog2(rows(P.Lz_line))};
-            hipLaunchKernelGGL(k_zpass<T>, dim3((Mx + A.R - 1) / A.R, Ny), dim3(256), 2 * A.R * P.Lz_line * e, st, A);
-            scale = T(1) / ((T(M) * T(Ny)) * T(P.Lz_line));
+            const LineGeom Lz = line_geom(P.Lz_line, P.Lz_line + (P.zmode == 1 ? Nz : 0), e, lds);
+            ZArgs<T> A{g, P.S, P.tw_z, P.tw_dct, Mx, P.Lz_line, P.zmode, P.lx, P.ly, P.lz, Lz};
+            hipLaunchKernelGGL(k_zpass<T>, dim3((Mx * Ny + Lz.R - 1) / Lz.R), dim3(kFftThreads), lds, st, A);
+            scale = T(1) / ((T(M) * T(Ny)) * T(P.Lz_norm));
         }
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 1, 0, P.lx, P.ly, rows(Ny), ilog2(rows(Ny))};
-        hipLaunchKernelGGL(k_ypass<T>, dim3((Mx + A.R - 1) / A.R, Nz), dim3(256), 2 * A.R * Ny * e, st, A);
+        YArgs<T> A{g, P.S, P.tw_y, Mx, 1, 0, P.lx, P.ly, Ly, make_fastdiv((uint32_t)Mx)};
+        hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     {
-        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, rows(M),
+        const LineGeom L = line_geom(M, M, e, lds);
+        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L,
                       make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)M)};
-        hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + A.R - 1) / A.R), dim3(256), 2 * A.R * M * e, st, A);
+        hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
 }
 

@@ -563,11 +563,27 @@ template <class T> struct PoissonPlan {
     const typename Vec2<T>::type *post;  // exp(-2πi k / Nx), k = 0 .. Nx/2
     const typename Vec2<T>::type *tw_y;  // length Ny
     const typename Vec2<T>::type *tw_z;  // length Lz_line
-    const T *lx, *ly, *lz;               // eigenvalues (lz has Lz_line entries)
-    int Lz_line;                         // 2·Nz (Bounded), Nz (Periodic)
+    const typename Vec2<T>::type *tw_dct; // exp(-iπk / 2Nz), k < Nz (zmode 1)
+    const T *lx, *ly, *lz;               // eigenvalues (lz: Nz entries, 2·Nz for Bounded z)
+    int zmode;                           // 0 Periodic z, 1 Bounded z via a length-Nz DCT, 2 Bounded z, odd Nz, 2Nz extension
+    int Lz_line;                         // FFT length along z: Nz (zmodes 0, 1), 2·Nz (zmode 2)
+    int Lz_norm;                         // normalisation of the z transform pair: Nz (Periodic), 2·Nz (Bounded)
     int tridiagonal;                     // Fourier–tridiagonal z solve (stretched z)
     T eps10;                             // 10·eps(FT): the Thomas pivot guard
 };
+
+// Lines of the FFT passes staged per block: the largest power of two R ≤ 16 with R·n ≤ 4096 whose
+// R rows (stride n + 1 when R > 1) and `tables` complex table entries fit in 64 KiB of LDS, with a
+// second buffer when n is not a power of two (direct DFT).  0: not even one line fits.
+inline int fft_lines_per_block(int n, int tables, size_t elem) {
+    const bool p2 = (n & (n - 1)) == 0;
+    for (int R = 16; R >= 1; R >>= 1) {
+        if ((size_t)R * n > 4096) continue;
+        const size_t ld = R > 1 ? (size_t)n + 1 : (size_t)n;
+        if (((size_t)(p2 ? 1 : 2) * R * ld + (size_t)tables) * elem <= 65536) return R;
+    }
+    return 0;
+}
 
 template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st);
 // z-marching LDS-tiled tendency kernel (om_tend_tiled.hip) and the configurations it covers
