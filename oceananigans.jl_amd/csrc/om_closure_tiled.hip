@@ -56,7 +56,14 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
 
     auto P = [&](const T *p, int dx, int dy) -> T { return p[(b + 1 + dy) * CW + (a + 1 + dx)]; };
     auto slot3 = [](int level) { return ((level % 3) + 3) % 3; };
-    auto ldcol = [&](const T *f, int k) -> T { return colok ? f[col + (int64_t)k * g.sz] : T(0); };
+    // own-column access through scalar level origins and one per-thread byte offset (the column's
+    // offset within a level, < 2³² bytes), refreshed opaquely each level so that no per-field
+    // pointer stays live across the level's arithmetic
+    const uint32_t colb0 = (uint32_t)(col * (int64_t)sizeof(T));
+    uint32_t colb = colb0;
+    auto ldcol = [&](const T *f, int k) -> T {
+        return colok ? *reinterpret_cast<const T *>(reinterpret_cast<const char *>(f + (int64_t)k * g.sz) + colb) : T(0);
+    };
 
     // own-column tracers at levels kk-1, kk, kk+1 ([0], [1], [2])
     T cw[NT > 0 ? NT : 1][3];
@@ -70,15 +77,27 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
 
     const T half = T(0.5);
     for (int k = k0; k < k1; ++k) {
-        __syncthreads();   // previous level's readers are done
-        stage_ring(S.u[slot3(k + 1)], C.s.U[0], g, i0, j0, k + 1, tid);
-        stage_ring(S.v[slot3(k + 1)], C.s.U[1], g, i0, j0, k + 1, tid);
-        stage_ring(S.w[(k + 1) & 1], C.s.U[2], g, i0, j0, k + 1, tid);
+        int otid = tid;
+        asm volatile("" : "+v"(otid));
+        colb = colb0;
+        asm volatile("" : "+v"(colb));
+        // every load of the level is issued before the barrier and the LDS stores
+        const PlaneShare<T, CW> ps(g, i0, j0, 1, otid);
+        T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2];
+        ps.load(ru, ps.origin(C.s.U[0], g, i0, j0, 1, k + 1));
+        ps.load(rv, ps.origin(C.s.U[1], g, i0, j0, 1, k + 1));
+        ps.load(rw, ps.origin(C.s.U[2], g, i0, j0, 1, k + 1));
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
-            stage_ring(S.c[n], C.s.U[3 + n], g, i0, j0, k, tid);
+            ps.load(rc[n], ps.origin(C.s.U[3 + n], g, i0, j0, 1, k));
             cw[n][0] = cw[n][1]; cw[n][1] = cw[n][2]; cw[n][2] = ldcol(C.s.U[3 + n], k + 1);
         }
+        __syncthreads();   // previous level's readers are done
+        ps.store(S.u[slot3(k + 1)], ru, tid);
+        ps.store(S.v[slot3(k + 1)], rv, tid);
+        ps.store(S.w[(k + 1) & 1], rw, tid);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) ps.store(S.c[n], rc[n], tid);
         __syncthreads();
         if (!cell) continue;
 
@@ -96,7 +115,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
         auto dz_u = [&](const T *Ul, const T *Ulm, int di, int kk) { return (P(Ul, di, 0) - P(Ulm, di, 0)) * g.idzf[kk]; };
         auto dy_w = [&](const T *Wl, int di, int dj) { return (P(Wl, di, dj) - P(Wl, di, dj - 1)) * idy; };
         auto dz_v = [&](const T *Vl, const T *Vlm, int dj, int kk) { return (P(Vl, 0, dj) - P(Vlm, 0, dj)) * g.idzf[kk]; };
-        const int64_t o = col + (int64_t)k * g.sz;
+        auto at = [&](T *f) -> T & { return *reinterpret_cast<T *>(reinterpret_cast<char *>(f + (int64_t)k * g.sz) + colb); };
 
         if (smag) {
             auto S12 = [&](int di, int dj) { return half * (dy_u(di, dj) + dx_v(di, dj)); };
@@ -135,7 +154,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
             T N2p = fmax(T(0), N2);
             T s2 = T(1) - fmin(T(1), cb * N2p / S2);
             T sig = (S2 == T(0)) ? T(0) : sqrt(s2);
-            C.s.nu[o] = (sig * (c0 * c0)) * (Df * Df) * sqrt(T(2) * S2);
+            at(C.s.nu) = (sig * (c0 * c0)) * (Df * Df) * sqrt(T(2) * S2);
             continue;
         }
 
@@ -202,7 +221,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
             const T r = (t1 + t2) + t3;
             nu = ft_div<T>(-ph.c0 * delta2 * (r - T(0)), q2);
         }
-        C.s.nu[o] = fmax(T(0), nu);
+        at(C.s.nu) = fmax(T(0), nu);
 
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
@@ -221,7 +240,7 @@ __global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int
                 const T theta = (cx_ux + cy_uy) + cz_uz;
                 kap = ft_div<T>(-ph.c1 * delta2 * theta, sig);
             }
-            C.s.kap[n][o] = fmax(T(0), kap);
+            at(C.s.kap[n]) = fmax(T(0), kap);
         }
 #undef A4
     }

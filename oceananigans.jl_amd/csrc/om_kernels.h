@@ -171,6 +171,42 @@ __device__ __forceinline__ int xcd_work_item(int id, int nb) {
     return (id & 7) * per + (id >> 3);
 }
 
+// A thread's share of a W × W plane staged into LDS by a 256-thread block: plane elements tid and
+// tid + 256 (row-major, origin (i0 − h, j0 − h)).  Their byte offsets from the plane origin and
+// their validity are computed once per block, so staging a level costs one scalar origin per
+// field and a load per element, and every load of a level can be issued before the first LDS
+// store.  Elements beyond the field's last halo point in x or y are staged as 0.
+template <class T, int W> struct PlaneShare {
+    uint32_t off[2];
+    bool in1;       // element tid + 256 lies inside the plane (element tid always does: W² ≥ 256)
+    bool ok[2];     // inside the plane and the field
+    __device__ __forceinline__ PlaneShare(const DevGrid<T> &g, int i0, int j0, int h, int tid) {
+        static_assert(W * W >= 256 && W * W <= 512, "two elements per thread");
+        const int imax = g.Nx + g.Hx - 1, jmax = g.Ny + g.Hy - 1;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + 256 * u;
+            const int y = e / W, x = e - y * W;
+            const bool in = e < W * W;
+            ok[u] = in && i0 - h + x <= imax && j0 - h + y <= jmax;
+            off[u] = (uint32_t)(((int64_t)y * g.sy + x) * (int64_t)sizeof(T));
+            if (u == 1) in1 = in;
+        }
+    }
+    // plane origin of field f at level k (uniform across the block)
+    __device__ __forceinline__ static const char *origin(const T *f, const DevGrid<T> &g, int i0, int j0, int h, int k) {
+        return reinterpret_cast<const char *>(f + idx3(i0 - h, j0 - h, k, g.sy, g.sz));
+    }
+    __device__ __forceinline__ void load(T (&v)[2], const char *base) const {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) v[u] = ok[u] ? *reinterpret_cast<const T *>(base + off[u]) : T(0);
+    }
+    __device__ __forceinline__ void store(T *dst, const T (&v)[2], int tid) const {
+        dst[tid] = v[0];
+        if (in1) dst[tid + 256] = v[1];
+    }
+};
+
 // ----------------------------------------------------------------------------------------
 // Immersed-bottom predicates (GridFittedBottom; src/Grids/inactive_node.jl:36-137,
 // immersed_boundary_interface.jl), device indices (0-based).  A node at location (LX, LY, LZ)

@@ -98,7 +98,13 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
 
     // own-column windows: [m] holds level kk-2+m
     T wu[6], wv[6], ww[6], wc[NT > 0 ? NT : 1][6], ko[NT > 0 ? NT : 1][2];
-    auto ldcol = [&](const T *f, int k) -> T { return colok ? f[col + (int64_t)k * g.sz] : T(0); };
+    // own-column loads: a scalar level origin per field plus one per-thread byte offset (the
+    // column's offset within a level, < 2³² bytes), shared by every field
+    const uint32_t colb0 = (uint32_t)(col * (int64_t)sizeof(T));
+    uint32_t colb = colb0;   // refreshed opaquely each level (below), so no per-field pointer is kept live
+    auto ldcol = [&](const T *f, int k) -> T {
+        return colok ? *reinterpret_cast<const T *>(reinterpret_cast<const char *>(f + (int64_t)k * g.sz) + colb) : T(0);
+    };
 
     int kk = k0 - 1;
 #pragma unroll
@@ -178,8 +184,28 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
 #pragma unroll
     for (int q = 0; q < 3 + NT; ++q) { zb[q][0] = zt[q][0]; zb[q][1] = zt[q][1]; }
 
+    const bool stage_nu = has_clo && clo != OM_CLOSURE_SCALAR_DIFFUSIVITY;
     for (kk = k0; kk < k1; ++kk) {
-        __syncthreads();   // previous level's readers are done with the planes and flux arrays
+        // The plane shares are rebuilt each level from an opaque copy of tid: a few integer
+        // operations, where keeping them live across the flux evaluation would cost registers.
+        int otid = tid;
+        asm volatile("" : "+v"(otid));
+        colb = colb0;
+        asm volatile("" : "+v"(colb));
+        const PlaneShare<T, PW> pbig(g, i0, j0, 3, otid);
+        const PlaneShare<T, NWD> psmall(g, i0, j0, 1, otid);
+        // this level's plane values: every load is issued before the barrier and the LDS stores
+        T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2], rn[2], rk[NT > 0 ? NT : 1][2];
+        pbig.load(ru, pbig.origin(C.s.U[0], g, i0, j0, 3, kk));
+        pbig.load(rv, pbig.origin(C.s.U[1], g, i0, j0, 3, kk));
+#pragma unroll
+        for (int n = 0; n < NT; ++n) pbig.load(rc[n], pbig.origin(C.s.U[3 + n], g, i0, j0, 3, kk));
+        pbig.load(rw, pbig.origin(C.s.U[2], g, i0, j0, 3, kk + 1));
+        if (stage_nu) psmall.load(rn, psmall.origin(C.s.nu, g, i0, j0, 1, kk + 1));
+        if (clo == OM_CLOSURE_AMD) {
+#pragma unroll
+            for (int n = 0; n < NT; ++n) psmall.load(rk[n], psmall.origin(C.s.kap[n], g, i0, j0, 1, kk));
+        }
         // advance the own-column windows to kk-2 .. kk+3
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
@@ -196,15 +222,16 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
 #pragma unroll
             for (int n = 0; n < NT; ++n) { ko[n][0] = ko[n][1]; ko[n][1] = ldcol(C.s.kap[n], kk + 1); }
         }
-        stage_plane(S.u, C.s.U[0], g, i0, j0, kk, tid);
-        stage_plane(S.v, C.s.U[1], g, i0, j0, kk, tid);
+        __syncthreads();   // previous level's readers are done with the planes and flux arrays
+        pbig.store(S.u, ru, tid);
+        pbig.store(S.v, rv, tid);
 #pragma unroll
-        for (int n = 0; n < NT; ++n) stage_plane(S.c[n], C.s.U[3 + n], g, i0, j0, kk, tid);
-        stage_plane(S.w[(kk + 1) & 1], C.s.U[2], g, i0, j0, kk + 1, tid);
-        if (has_clo && clo != OM_CLOSURE_SCALAR_DIFFUSIVITY) stage_small(S.nu[nslot(kk + 1)], C.s.nu, g, i0, j0, kk + 1, tid);
+        for (int n = 0; n < NT; ++n) pbig.store(S.c[n], rc[n], tid);
+        pbig.store(S.w[(kk + 1) & 1], rw, tid);
+        if (stage_nu) psmall.store(S.nu[nslot(kk + 1)], rn, tid);
         if (clo == OM_CLOSURE_AMD) {
 #pragma unroll
-            for (int n = 0; n < NT; ++n) stage_small(S.kap[n], C.s.kap[n], g, i0, j0, kk, tid);
+            for (int n = 0; n < NT; ++n) psmall.store(S.kap[n], rk[n], tid);
         }
         __syncthreads();
 
@@ -311,7 +338,9 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
         __syncthreads();
 
         if (cell) {
-            const int64_t o = col + (int64_t)kk * g.sz;
+            // outputs and pHY′ through scalar level origins and the column byte offset, as ldcol
+            const int64_t lo = (int64_t)kk * g.sz;
+            auto at = [&](T *f, int64_t shift) -> T & { return *reinterpret_cast<T *>(reinterpret_cast<char *>(f + lo + shift) + colb); };
             const T iVc = recip<T>(g.Az * g.dzc[kk]), iVf = recip<T>(g.Az * g.dzf[kk]);
             const int sx0 = b * TT + a, sx1 = sx0 + 1, sy1 = sx0 + TT;
             auto diffs = [&](int q, T &sx, T &sy, T &sz, int part) {
@@ -331,9 +360,9 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
                     const T I = half * (ca + cb);
                     G += (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
-                if (ph.hydrostatic) G -= (C.s.pHY[o] - C.s.pHY[o - 1]) * g.idx;
+                if (ph.hydrostatic) G -= (at(C.s.pHY, 0) - at(C.s.pHY, -1)) * g.idx;
                 if (has_clo) { diffs(0, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
-                C.s.Gn[0][o] = G;
+                at(C.s.Gn[0], 0) = G;
             }
             // v
             {
@@ -346,9 +375,9 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
                     const T I = half * (ca + cb);
                     G -= (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
-                if (ph.hydrostatic) G -= (C.s.pHY[o] - C.s.pHY[o - g.sy]) * g.idy;
+                if (ph.hydrostatic) G -= (at(C.s.pHY, 0) - at(C.s.pHY, -g.sy)) * g.idy;
                 if (has_clo) { diffs(1, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
-                C.s.Gn[1][o] = G;
+                at(C.s.Gn[1], 0) = G;
             }
             // w
             {
@@ -372,14 +401,14 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
                     G += half * (bm + b0);
                 }
                 if (has_clo) { diffs(2, sx, sy, sz, 1); G -= iVf * ((sx + sy) + sz); }
-                C.s.Gn[2][o] = G;
+                at(C.s.Gn[2], 0) = G;
             }
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
                 diffs(3 + n, sx, sy, sz, 0);
                 T G = T(0) - iVc * ((sx + sy) + sz);
                 if (has_clo) { diffs(3 + n, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
-                C.s.Gn[3 + n][o] = G;
+                at(C.s.Gn[3 + n], 0) = G;
             }
         }
 #pragma unroll
