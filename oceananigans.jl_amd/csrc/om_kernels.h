@@ -96,22 +96,24 @@ template <class T> __device__ __forceinline__ T weno5(T s0, T s1, T s2, T s3, T 
 // never bitwise, so the kernel spends its VALU budget on fewer instructions instead.
 template <> __device__ __forceinline__ float weno5<float>(float s0, float s1, float s2, float s3, float s4, float s5, bool left) {
     typedef RC<float> K;
-    const float a0 = left ? s2 : s3, a1 = left ? s3 : s2, a2 = left ? s4 : s1;
-    const float b0 = left ? s1 : s4, b1 = left ? s2 : s3, b2 = left ? s3 : s2;
-    const float c0 = left ? s0 : s5, c1 = left ? s1 : s4, c2 = left ? s2 : s3;
-    // β with the middle value subtracted (weno_interpolants.jl:276-281): x(k₀x + k₂z) + k₅z²
-    const float xa = a0 - a1, za = a2 - a1, xb = b0 - b1, zb = b2 - b1, xc = c0 - c1, zc = c2 - c1;
+    // Mirror the stencil once: for either bias the three candidate stencils are then
+    // a = (t2, t3, t4), b = (t1, t2, t3), c = (t0, t1, t2).
+    const float t0 = left ? s0 : s5, t1 = left ? s1 : s4, t2 = left ? s2 : s3, t3 = left ? s3 : s2, t4 = left ? s4 : s1;
+    // differences from each stencil's middle value (weno_interpolants.jl:276-281); neighbouring
+    // stencils share them: b's right difference is −xa, c's is −xb
+    const float xa = t2 - t3, za = t4 - t3, xb = t1 - t2, xc = t0 - t1;
     const float ba = fmaf(xa, fmaf(10.0f, xa, 11.0f * za), 4.0f * za * za);
-    const float bb = fmaf(xb, fmaf(4.0f, xb, 5.0f * zb), 4.0f * zb * zb);
-    const float bc = fmaf(xc, fmaf(4.0f, xc, 11.0f * zc), 10.0f * zc * zc);
+    const float bb = fmaf(xb, fmaf(4.0f, xb, -5.0f * xa), 4.0f * xa * xa);
+    const float bc = fmaf(xc, fmaf(4.0f, xc, -11.0f * xb), 10.0f * xb * xb);
     const float tau = fabsf(ba - bc);
     const float qa = tau * __builtin_amdgcn_rcpf(ba + K::eps);
     const float qb = tau * __builtin_amdgcn_rcpf(bb + K::eps);
     const float qc = tau * __builtin_amdgcn_rcpf(bc + K::eps);
     const float al0 = K::C50 * fmaf(qa, qa, 1.0f), al1 = K::C51 * fmaf(qb, qb, 1.0f), al2 = K::C52 * fmaf(qc, qc, 1.0f);
-    const float pa = fmaf(K::a0, a0, fmaf(K::a1, a1, K::a2 * a2));
-    const float pb = fmaf(K::b0, b0, fmaf(K::b1, b1, K::b2 * b2));
-    const float pc = fmaf(K::c0, c0, fmaf(K::c1, c1, K::c2 * c2));
+    // candidate values: each stencil's coefficients sum to 1, so p = middle + k₀·x + k₂·z
+    const float pa = fmaf(K::a0, xa, fmaf(K::a2, za, t3));
+    const float pb = fmaf(K::b0, xb, fmaf(-K::b2, xa, t2));
+    const float pc = fmaf(K::c0, xc, fmaf(-K::c2, xb, t1));
     const float num = fmaf(al0, pa, fmaf(al1, pb, al2 * pc));
     return num * __builtin_amdgcn_rcpf((al0 + al1) + al2);
 }
