@@ -147,8 +147,9 @@ template <class T> struct ModelT : ModelBase {
     hipStream_t st = nullptr;
     int nt = 0, nf = 0;
     bool has_pHY = false;
-    int pnx = 0, pny = 0, pnz = 0;
-    size_t fsize = 0;                  // elements per field
+    int pnx = 0, pny = 0, pnz = 0;     // parent array size (pnz: one more for w on a Bounded z)
+    size_t lead = 0;                   // elements before the parent array in a field's block
+    size_t fsize = 0;                  // elements per field block
     std::vector<void *> allocs;
     T *fields_block = nullptr;
     T *rhs_dense = nullptr, *phi_dense = nullptr;
@@ -252,8 +253,13 @@ template <class T> struct ModelT : ModelBase {
         return d;
     }
 
-    template <class U> U *field_origin(U *base) const { return base + (size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny; }
-    T *field_base(const T *origin) const { return const_cast<T *>(origin) - ((size_t)g.Hx + (size_t)g.Hy * pnx + (size_t)g.Hz * pnx * pny); }
+    // Field layout: rows of sy elements (pnx rounded up to 128 bytes), planes of sz = sy·pny, and a
+    // lead of `lead` elements before the parent array so that every interior row starts on a
+    // 128-byte line (an interior row of a kernel then touches Nx·size/128 lines, not one more).
+    // Element strides are shared by the T fields and the FP64 CG vectors.
+    template <class U> U *field_origin(U *base) const { return base + lead + (size_t)g.Hx + (size_t)g.Hy * g.sy + (size_t)g.Hz * g.sz; }
+    T *field_base(const T *origin) const { return const_cast<T *>(origin) - ((size_t)g.Hx + (size_t)g.Hy * g.sy + (size_t)g.Hz * g.sz); }
+    size_t parent_index(int x, int y, int z) const { return (size_t)x + (size_t)g.sy * y + (size_t)g.sz * z; }
 
     void init(const om_config_t &c) {
         cfg = c;
@@ -268,7 +274,12 @@ template <class T> struct ModelT : ModelBase {
         g.Hx = hg.H[0]; g.Hy = hg.H[1]; g.Hz = hg.H[2];
         g.tx = hg.topo[0]; g.ty = hg.topo[1]; g.tz = hg.topo[2];
         pnx = g.Nx + 2 * g.Hx; pny = g.Ny + 2 * g.Hy; pnz = g.Nz + 2 * g.Hz + 1;
-        g.sy = pnx; g.sz = (int64_t)pnx * pny;
+        {
+            const int line = (int)(128 / sizeof(T));   // elements per 128-byte line
+            g.sy = (pnx + line - 1) / line * line;
+            g.sz = g.sy * pny;
+            lead = (size_t)((line - g.Hx % line) % line);
+        }
         g.dx = hg.dx; g.dy = hg.dy; g.idx = T(1) / hg.dx; g.idy = T(1) / hg.dy;
         g.Az = hg.dx * hg.dy; g.iAz = T(1) / g.Az;
         std::vector<T> idzc(hg.dzc.size()), idzf(hg.dzf.size());
@@ -319,8 +330,8 @@ template <class T> struct ModelT : ModelBase {
         else has_pHY = c.hydrostatic_pressure == 1;
         ph.hydrostatic = has_pHY ? 1 : 0;
         // fields: one block; order U[nf], Gn[nf], Gm[nf], pNHS, pHY, nu, kap[nt]
-        fsize = (size_t)pnx * pny * pnz;
-        fsize = (fsize + 63) & ~(size_t)63;   // 256-byte aligned field starts
+        fsize = lead + (size_t)g.sz * pnz;
+        fsize = (fsize + 63) & ~(size_t)63;   // 256-byte aligned field blocks
         const int nblocks = 3 * nf + 3 + nt;
         fields_block = dalloc<T>(fsize * nblocks);
         memset(&s, 0, sizeof(s));
@@ -743,35 +754,37 @@ template <class T> struct ModelT : ModelBase {
         T *o = field_ptr(id, is_w);
         if (!o) throw ArgError{"unknown field id"};
         HIPCHECK(hipStreamSynchronize(st));
-        std::vector<T> par(fsize);
+        const size_t n = (size_t)g.sz * nz_parent(is_w);   // the parent array's span
+        std::vector<T> par(n);
         T *base = field_base(o);
-        HIPCHECK(hipMemcpy(par.data(), base, fsize * sizeof(T), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(par.data(), base, n * sizeof(T), hipMemcpyDeviceToHost));
+        size_t t = 0;
         if (with_halos) {
-            const size_t n = (size_t)pnx * pny * nz_parent(is_w);
-            for (size_t q = 0; q < n; ++q) par[q] = (T)h[q];
+            for (int z = 0; z < nz_parent(is_w); ++z) for (int y = 0; y < pny; ++y) for (int x = 0; x < pnx; ++x)
+                par[parent_index(x, y, z)] = (T)h[t++];
         } else {
-            size_t t = 0;
             const int nzi = nz_interior(is_w);
             for (int k = 0; k < nzi; ++k) for (int j = 0; j < g.Ny; ++j) for (int i = 0; i < g.Nx; ++i)
-                par[(size_t)(i + g.Hx) + (size_t)pnx * ((size_t)(j + g.Hy) + (size_t)pny * (size_t)(k + g.Hz))] = (T)h[t++];
+                par[parent_index(i + g.Hx, j + g.Hy, k + g.Hz)] = (T)h[t++];
         }
-        HIPCHECK(hipMemcpy(base, par.data(), fsize * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(base, par.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
     void get_field(int id, double *h, bool with_halos) {
         bool is_w;
         T *o = field_ptr(id, is_w);
         if (!o) throw ArgError{"unknown field id"};
         HIPCHECK(hipStreamSynchronize(st));
-        std::vector<T> par(fsize);
-        HIPCHECK(hipMemcpy(par.data(), field_base(o), fsize * sizeof(T), hipMemcpyDeviceToHost));
+        const size_t n = (size_t)g.sz * nz_parent(is_w);
+        std::vector<T> par(n);
+        HIPCHECK(hipMemcpy(par.data(), field_base(o), n * sizeof(T), hipMemcpyDeviceToHost));
+        size_t t = 0;
         if (with_halos) {
-            const size_t n = (size_t)pnx * pny * nz_parent(is_w);
-            for (size_t q = 0; q < n; ++q) h[q] = (double)par[q];
+            for (int z = 0; z < nz_parent(is_w); ++z) for (int y = 0; y < pny; ++y) for (int x = 0; x < pnx; ++x)
+                h[t++] = (double)par[parent_index(x, y, z)];
         } else {
-            size_t t = 0;
             const int nzi = nz_interior(is_w);
             for (int k = 0; k < nzi; ++k) for (int j = 0; j < g.Ny; ++j) for (int i = 0; i < g.Nx; ++i)
-                h[t++] = (double)par[(size_t)(i + g.Hx) + (size_t)pnx * ((size_t)(j + g.Hy) + (size_t)pny * (size_t)(k + g.Hz))];
+                h[t++] = (double)par[parent_index(i + g.Hx, j + g.Hy, k + g.Hz)];
         }
     }
 };
