@@ -439,14 +439,6 @@ template <class T> struct ModelT : ModelBase {
         PhaseScope ps(this, OM_PHASE_HALOS);
         launch_halos(g, J, st);
     }
-    void halos_kappa() {
-        if (nt == 0) return;
-        HaloJob<T> J;
-        memset(&J, 0, sizeof(J));
-        for (int n = 0; n < nt; ++n) { J.f[J.nf] = s.kap[n]; J.kind_bottom[J.nf] = OM_BC_DEFAULT; J.kind_top[J.nf] = OM_BC_DEFAULT; J.nf++; }
-        PhaseScope ps(this, OM_PHASE_HALOS);
-        launch_halos(g, J, st);
-    }
     uint32_t all_mask() const { return (1u << nf) - 1u; }
 
     // ---- phases
@@ -462,6 +454,11 @@ template <class T> struct ModelT : ModelBase {
             for (int n = 0; n < nt; ++n) launch_mask(g, s.U[3 + n], 0, st);
         }
         halos_prognostic(all_mask(), false);
+        // νₑ, κₑ and pHY′ all take default (copy) halo fills, and neither the closure nor the
+        // pHY′ kernel reads the others: one fill after both serves the three
+        HaloJob<T> J;
+        memset(&J, 0, sizeof(J));
+        auto add = [&](T *f) { J.f[J.nf] = f; J.kind_bottom[J.nf] = OM_BC_DEFAULT; J.kind_top[J.nf] = OM_BC_DEFAULT; J.nf++; };
         if (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) {
             {
                 PhaseScope ps(this, OM_PHASE_CLOSURE);
@@ -469,13 +466,14 @@ template <class T> struct ModelT : ModelBase {
                 if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) launch_closure_tiled(C, tune_kchunk, st);
                 else launch_closure(C, st);
             }
-            halos_aux({s.nu});
-            if (cfg.closure == OM_CLOSURE_AMD) halos_kappa();
+            add(s.nu);
+            if (cfg.closure == OM_CLOSURE_AMD) for (int n = 0; n < nt; ++n) add(s.kap[n]);
         }
         if (has_pHY) {
             { PhaseScope ps(this, OM_PHASE_HYDROSTATIC); launch_hydrostatic(ctx(), st); }
-            halos_aux({s.pHY});
+            add(s.pHY);
         }
+        if (J.nf > 0) { PhaseScope ps(this, OM_PHASE_HALOS); launch_halos(g, J, st); }
         compute_tendencies();
     }
     void pressure_correction(T dtau) {

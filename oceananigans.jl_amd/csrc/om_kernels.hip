@@ -260,27 +260,33 @@ template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C
     T bkm = C.buoy(i, j, Nz - 1);
     T p = -(T(0.5) * (bkm + bk)) * C.g.dzf[Nz];
     C.s.pHY[C.at(i, j, Nz - 1)] = p;
-    // the buoyancy of 16 levels is loaded before their pressures are stored: a load placed after
-    // a store to pHY′ cannot be moved above it (the arrays might alias), which would expose one
-    // memory round trip per level
-    int k = Nz - 2;
-    for (; k >= 15; k -= 16) {
-        T bb[16];
+    // One thread per column gives one wave per SIMD, so the memory latency is hidden within each
+    // thread: the buoyancy is loaded in batches of B levels, double-buffered, the next batch's
+    // loads issued before the current batch is integrated (a load placed after a store to pHY′
+    // cannot be moved above it, as the arrays might alias).
+    constexpr int B = sizeof(T) == 4 ? 32 : 16;
+    T ba[B], bb[B];
+    auto load = [&](T (&buf)[B], int top) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) bb[q] = C.buoy(i, j, k - q);
+        for (int q = 0; q < B; ++q) buf[q] = top - q >= 0 ? C.buoy(i, j, top - q) : T(0);
+    };
+    auto integrate = [&](const T (&buf)[B], int top) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = 0; q < B; ++q) {
+            if (top - q < 0) break;
             bk = bkm;
-            bkm = bb[q];
-            p -= (T(0.5) * (bkm + bk)) * C.g.dzf[k - q + 1];
-            C.s.pHY[C.at(i, j, k - q)] = p;
+            bkm = buf[q];
+            p -= (T(0.5) * (bkm + bk)) * C.g.dzf[top - q + 1];
+            C.s.pHY[C.at(i, j, top - q)] = p;
         }
-    }
-    for (; k >= 0; --k) {
-        bk = bkm;
-        bkm = C.buoy(i, j, k);
-        p -= (T(0.5) * (bkm + bk)) * C.g.dzf[k + 1];
-        C.s.pHY[C.at(i, j, k)] = p;
+    };
+    int k = Nz - 2;
+    load(ba, k);
+    for (; k >= 0; k -= 2 * B) {
+        load(bb, k - B);
+        integrate(ba, k);
+        load(ba, k - 2 * B);
+        integrate(bb, k - B);
     }
 }
 
