@@ -274,6 +274,13 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
 #define OM_TUNE_LEARNED_ITERATIONS 2 /* leading iterations of each CG solve that apply the learned
                                         correction (0: all); later iterations use the FFT
                                         preconditioner.  Solutions agree to the solver tolerance. */
+#define OM_TUNE_POISSON_REFINEMENT 3 /* 1: each FFT / Fourier-tridiagonal Poisson solve is followed by
+                                        one step of iterative refinement: the residual b - Lp of the
+                                        computed solution in FP64 arithmetic (for om_poisson_solve
+                                        against the FP64 right-hand side), one more solve for the
+                                        correction, p += correction.  0 (default): off.  Not with
+                                        the conjugate-gradient solver (its residuals are FP64
+                                        already) or an immersed bottom. */
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 

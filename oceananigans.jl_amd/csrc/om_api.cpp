@@ -193,6 +193,10 @@ template <class T> struct ModelT : ModelBase {
     int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
     int tune_kchunk = 16;              // z levels per block of the tiled kernel
     int tune_learned_iterations = 0;   // leading CG iterations that apply the learned correction (0: all)
+    int tune_refine = 0;               // FP64 iterative refinement of FFT / Fourier–tridiagonal solves
+    RefineOp refop{};                  // the solver's operator for the refinement residual
+    T *ref_r = nullptr, *ref_d = nullptr;   // dense residual and correction
+    double *ref_b64 = nullptr;         // FP64 right-hand side of om_poisson_solve
 
     struct PhaseScope {
         ModelT *M;
@@ -381,6 +385,26 @@ template <class T> struct ModelT : ModelBase {
         P.ly = upload(eigen<T>(g.Ny, hg.Ly, g.ty, 1));
         P.lz = upload(eigen<T>(g.Nz, hg.Lz, g.tz, g.tz == OM_BOUNDED ? 2 : 1));
         P.eps10 = T(10) * std::numeric_limits<T>::epsilon();
+        {
+            // the inverse squared spacings behind the solver's eigenvalues (eigen(): h = T(L / N))
+            auto ih2 = [](T L, int N, int topo) { if (topo == OM_FLAT) return 0.0; const double h = (double)(T)(L / T(N)); return 1.0 / (h * h); };
+            refop.ihx2 = ih2(hg.Lx, g.Nx, g.tx);
+            refop.ihy2 = ih2(hg.Ly, g.Ny, g.ty);
+            refop.ihz2 = P.tridiagonal ? -1.0 : ih2(hg.Lz, g.Nz, g.tz);
+            if (P.tridiagonal) {
+                // FP64 spacings of the column from the configured face heights (uniform without them)
+                const int Nz = g.Nz;
+                std::vector<double> zf(Nz + 1), zcen(Nz), idzc(Nz), idzf(Nz + 1, 0.0);
+                for (int k = 0; k <= Nz; ++k) zf[k] = c.z_faces ? c.z_faces[k] : c.origin[2] + c.extent[2] * k / Nz;
+                for (int k = 0; k < Nz; ++k) {
+                    idzc[k] = 1.0 / (zf[k + 1] - zf[k]);
+                    zcen[k] = 0.5 * (zf[k] + zf[k + 1]);
+                }
+                for (int k = 1; k < Nz; ++k) idzf[k] = 1.0 / (zcen[k] - zcen[k - 1]);
+                refop.idzc = upload(idzc);
+                refop.idzf = upload(idzf);
+            }
+        }
         diag_partial = dalloc<double>(3 * 1024);
         diag_out = dalloc<double>(4);
         HIPCHECK(hipHostMalloc((void **)&diag_host, 4 * sizeof(double), hipHostMallocDefault));
@@ -503,7 +527,11 @@ template <class T> struct ModelT : ModelBase {
             }
             return;
         }
-        { PhaseScope ps(this, OM_PHASE_POISSON); launch_poisson(P, ctx(), (const T *)nullptr, s.pNHS, 0, dtp, st); }
+        {
+            PhaseScope ps(this, OM_PHASE_POISSON);
+            launch_poisson(P, ctx(), (const T *)nullptr, s.pNHS, 0, dtp, st);
+            if (tune_refine) refine(s.pNHS, g.sy, g.sz, nullptr, (double)dtp);
+        }
         halos_aux({s.pNHS});
         { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, 0, st); }
     }
@@ -703,6 +731,20 @@ template <class T> struct ModelT : ModelBase {
         const T dtau = first ? (T)(dt * (double)gamma) : (T)(dt * (double)(gamma + zeta));
         { PhaseScope ps(this, OM_PHASE_RK_UPDATE); launch_rk3(ctx(), kdt, gamma, zeta, first ? 1 : 0, nf, st); }
         pressure_correction(dtau);
+    }
+    void set_refinement(int on) {
+        if (on && (cg || g.kbot)) throw ArgError{"iterative refinement applies to the FFT and Fourier-tridiagonal solvers on a grid without an immersed bottom"};
+        if (on && !ref_r) {
+            const size_t n = (size_t)g.Nx * g.Ny * g.Nz;
+            ref_r = dalloc<T>(n); ref_d = dalloc<T>(n); ref_b64 = dalloc<double>(n);
+        }
+        tune_refine = on;
+    }
+    // one refinement step of the solution p (strides psy, psz) of L p = b (b64, or ∇·U / divisor)
+    void refine(T *p, int64_t psy, int64_t psz, const double *b64, double divisor) {
+        launch_poisson_residual(ctx(), refop, p, psy, psz, b64, divisor, ref_r, st);
+        launch_poisson(P, ctx(), ref_r, ref_d, 1, T(1), st);
+        launch_add_correction(g, p, psy, psz, ref_d, st);
     }
     void cache_previous_tendencies() {
         // G⁻ ← Gⁿ by exchanging the buffers: the next tendency evaluation overwrites every
@@ -1043,8 +1085,13 @@ int om_poisson_solve(om_model_t *m, const double *rhs, double *phi) {
             cg_xio(M->g, M->cg_z, M->cg_x, (TT)0, 0, M->st);
             M->cg_solve();
             cg_gather(M->g, M->cg_x, M->phi_dense, M->st);
-        } else
+        } else {
             launch_poisson(M->P, M->ctx(), M->rhs_dense, M->phi_dense, 1, (TT)1, M->st);
+            if (M->tune_refine) {
+                HIPCHECK(hipMemcpyAsync(M->ref_b64, rhs, n * sizeof(double), hipMemcpyHostToDevice, M->st));
+                M->refine(M->phi_dense, M->g.Nx, (int64_t)M->g.Nx * M->g.Ny, M->ref_b64, 1.0);
+            }
+        }
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpy(h.data(), M->phi_dense, n * sizeof(TT), hipMemcpyDeviceToHost));
@@ -1249,27 +1296,35 @@ int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *f
     });
 }
 
+static bool known_tuning_key(int32_t key) {
+    return key == OM_TUNE_TENDENCY_KERNEL || key == OM_TUNE_KCHUNK || key == OM_TUNE_LEARNED_ITERATIONS ||
+           key == OM_TUNE_POISSON_REFINEMENT;
+}
+
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (!m) return OM_EINVAL;
     if (key == OM_TUNE_TENDENCY_KERNEL && (value < 0 || value > 1)) return OM_EINVAL;
     if (key == OM_TUNE_KCHUNK && (value < 1 || value > 1 << 20)) return OM_EINVAL;
     if (key == OM_TUNE_LEARNED_ITERATIONS && value < 0) return OM_EINVAL;
-    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK && key != OM_TUNE_LEARNED_ITERATIONS) return OM_EINVAL;
+    if (key == OM_TUNE_POISSON_REFINEMENT && (value < 0 || value > 1)) return OM_EINVAL;
+    if (!known_tuning_key(key)) return OM_EINVAL;
     return WITH_MODEL(m, {
         if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
         else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = value;
-        else M->tune_learned_iterations = value;
+        else if (key == OM_TUNE_LEARNED_ITERATIONS) M->tune_learned_iterations = value;
+        else M->set_refinement(value);
     });
 }
 
 int om_get_tuning(const om_model_t *mc, int32_t key, int32_t *value) {
     om_model_t *m = const_cast<om_model_t *>(mc);
     if (!m || !value) return OM_EINVAL;
-    if (key != OM_TUNE_TENDENCY_KERNEL && key != OM_TUNE_KCHUNK && key != OM_TUNE_LEARNED_ITERATIONS) return OM_EINVAL;
+    if (!known_tuning_key(key)) return OM_EINVAL;
     return WITH_MODEL(m, {
-        *value = key == OM_TUNE_TENDENCY_KERNEL ? M->tune_tendency_kernel
-               : key == OM_TUNE_KCHUNK          ? M->tune_kchunk
-                                                : M->tune_learned_iterations;
+        *value = key == OM_TUNE_TENDENCY_KERNEL      ? M->tune_tendency_kernel
+               : key == OM_TUNE_KCHUNK               ? M->tune_kchunk
+               : key == OM_TUNE_LEARNED_ITERATIONS   ? M->tune_learned_iterations
+                                                     : M->tune_refine;
     });
 }
 

@@ -505,6 +505,70 @@ template <class T> __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A)
 }
 
 // ---------------------------------------------------------------------------------------------
+// FP64 iterative refinement: residual of a computed solution and the correction update
+// ---------------------------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_poisson_residual(Ctx<T> C, RefineOp L, const T *p, int64_t psy, int64_t psz,
+                                                          const double *b64, double idiv, T *r) {
+    const DevGrid<T> &g = C.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.Nx || j >= g.Ny) return;
+    const int64_t dense = (int64_t)i + (int64_t)g.Nx * ((int64_t)j + (int64_t)g.Ny * k);
+    const int ip = i + 1 == g.Nx ? 0 : i + 1, im = i == 0 ? g.Nx - 1 : i - 1;
+    const int jp = j + 1 == g.Ny ? 0 : j + 1, jm = j == 0 ? g.Ny - 1 : j - 1;
+    double b;
+    if (b64) b = b64[dense];
+    else {
+        // ∇·U in FP64 from the stored velocities (divergence_operators.jl:16-19 on a rectilinear grid)
+        b = 0;
+        if (!C.flx()) b += ((double)C.ld(C.s.U[0], ip, j, k) - (double)C.ld(C.s.U[0], i, j, k)) * (double)g.idx;
+        if (!C.fly()) b += ((double)C.ld(C.s.U[1], i, jp, k) - (double)C.ld(C.s.U[1], i, j, k)) * (double)g.idy;
+        if (!C.flz())
+            b += ((double)C.ld(C.s.U[2], i, j, k + 1) - (double)C.ld(C.s.U[2], i, j, k)) * (L.ihz2 < 0 ? L.idzc[k] : (double)g.idzc[k]);
+        b *= idiv;
+    }
+    auto P = [&](int ii, int jj, int kk) -> double { return (double)p[(int64_t)ii + psy * jj + psz * kk]; };
+    const double pc = P(i, j, k);
+    double lap = 0;
+    if (!C.flx()) lap += ((P(ip, j, k) - pc) - (pc - P(im, j, k))) * L.ihx2;
+    if (!C.fly()) lap += ((P(i, jp, k) - pc) - (pc - P(i, jm, k))) * L.ihy2;
+    if (!C.flz()) {
+        const bool per = g.tz == OM_PERIODIC;
+        const int kp = k + 1 == g.Nz ? (per ? 0 : -1) : k + 1, km = k == 0 ? (per ? g.Nz - 1 : -1) : k - 1;
+        if (L.ihz2 >= 0) {
+            const double up = kp >= 0 ? P(i, j, kp) - pc : 0.0, dn = km >= 0 ? pc - P(i, j, km) : 0.0;
+            lap += (up - dn) * L.ihz2;
+        } else {   // stretched: (flux through face k+1 − flux through face k) / Δzᶜ
+            const double up = kp >= 0 ? (P(i, j, kp) - pc) * L.idzf[k + 1] : 0.0;
+            const double dn = km >= 0 ? (pc - P(i, j, km)) * L.idzf[k] : 0.0;
+            lap += (up - dn) * L.idzc[k];
+        }
+    }
+    r[dense] = (T)(b - lap);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_add_correction(DevGrid<T> g, T *p, int64_t psy, int64_t psz, const T *d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.Nx || j >= g.Ny) return;
+    p[(int64_t)i + psy * j + psz * k] += d[(int64_t)i + (int64_t)g.Nx * ((int64_t)j + (int64_t)g.Ny * k)];
+}
+
+template <class T> void launch_poisson_residual(const Ctx<T> &C, const RefineOp &L, const T *p, int64_t psy, int64_t psz,
+                                                const double *b64, double divisor, T *r, hipStream_t st) {
+    const dim3 blk(64, 4, 1), grid((C.g.Nx + 63) / 64, (C.g.Ny + 3) / 4, C.g.Nz);
+    hipLaunchKernelGGL(k_poisson_residual<T>, grid, blk, 0, st, C, L, p, psy, psz, b64, 1.0 / divisor, r);
+}
+template <class T> void launch_add_correction(const DevGrid<T> &g, T *p, int64_t psy, int64_t psz, const T *d, hipStream_t st) {
+    const dim3 blk(64, 4, 1), grid((g.Nx + 63) / 64, (g.Ny + 3) / 4, g.Nz);
+    hipLaunchKernelGGL(k_add_correction<T>, grid, blk, 0, st, g, p, psy, psz, d);
+}
+
+// ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
 static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
@@ -568,10 +632,16 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
 }
 
 #ifndef OM_ONLY_DOUBLE
+template void launch_poisson_residual<float>(const Ctx<float> &, const RefineOp &, const float *, int64_t, int64_t,
+                                             const double *, double, float *, hipStream_t);
+template void launch_add_correction<float>(const DevGrid<float> &, float *, int64_t, int64_t, const float *, hipStream_t);
 template void launch_poisson<float>(const PoissonPlan<float> &, const Ctx<float> &, const float *, float *, int, float,
                                     hipStream_t);
 #endif
 #ifndef OM_ONLY_FLOAT
+template void launch_poisson_residual<double>(const Ctx<double> &, const RefineOp &, const double *, int64_t, int64_t,
+                                              const double *, double, double *, hipStream_t);
+template void launch_add_correction<double>(const DevGrid<double> &, double *, int64_t, int64_t, const double *, hipStream_t);
 template void launch_poisson<double>(const PoissonPlan<double> &, const Ctx<double> &, const double *, double *, int, double,
                                      hipStream_t);
 #endif

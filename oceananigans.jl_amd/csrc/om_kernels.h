@@ -683,6 +683,20 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
 template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
                                        T divisor, hipStream_t st);
 
+// FP64 iterative refinement of a Poisson solve.  Residual r = b − ∇²p in FP64 arithmetic, stored
+// dense in T: ∇² is the operator the FFT / Fourier–tridiagonal solver inverts (periodic x, y;
+// Neumann walls on a Bounded z; ihx2 etc. the FP64 inverse squared spacings of its eigenvalues),
+// evaluated without halos; p has element strides psy, psz.  b: b64 (dense FP64) if given, else the
+// FP64 divergence of the velocities of C divided by `divisor`.  Then p += d (dense correction).
+struct RefineOp {
+    double ihx2, ihy2, ihz2;   // ihz2 < 0: stretched z, coefficients from idzf, idzc
+    const double *idzf;        // FP64 1/Δzᶠ at faces k = 0 .. Nz and 1/Δzᶜ of cells k = 0 .. Nz − 1,
+    const double *idzc;        // from the configuration's FP64 face heights (stretched z)
+};
+template <class T> void launch_poisson_residual(const Ctx<T> &C, const RefineOp &L, const T *p, int64_t psy, int64_t psz,
+                                                const double *b64, double divisor, T *r, hipStream_t st);
+template <class T> void launch_add_correction(const DevGrid<T> &g, T *p, int64_t psy, int64_t psz, const T *d, hipStream_t st);
+
 // diagnostics reductions (om_diagnostics.hip); partial holds 3·1024 doubles, out 3 doubles
 // (max, min, NaN count); the advection timescale reduction leaves max = max inverse timescale
 template <class T> void launch_adv_timescale(const Ctx<T> &C, double *partial, double *out, hipStream_t st);

@@ -27,6 +27,7 @@ OM_LEARNED_NWEIGHTS = 4 * 27 * 16 * 16
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 OM_TUNE_LEARNED_ITERATIONS = 2
+OM_TUNE_POISSON_REFINEMENT = 3
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
 

@@ -339,9 +339,13 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_last_solve_info(self._h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t)), self._h)
         return it.value, r.value, t.value
 
-    def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None):
+    def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None, poisson_refinement=None):
         """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';
-        kchunk = z levels per block of the tiled kernel.  Results do not depend on them."""
+        kchunk = z levels per block of the tiled kernel.  Results do not depend on them.
+        poisson_refinement: one FP64 iterative-refinement step after each FFT / Fourier–tridiagonal
+        Poisson solve (off by default)."""
+        if poisson_refinement is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_POISSON_REFINEMENT, 1 if poisson_refinement else 0), self._h)
         if tendency_kernel is not None:
             v = {"tiled": 0, "generic": 1}[tendency_kernel]
             _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_TENDENCY_KERNEL, v), self._h)
@@ -356,6 +360,11 @@ class NonhydrostaticModel:
         kernel = ("tiled", "generic")[v.value]
         _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_KCHUNK, ctypes.byref(v)), self._h)
         return {"tendency_kernel": kernel, "kchunk": v.value}
+
+    def poisson_refinement(self):
+        v = ctypes.c_int32()
+        _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_POISSON_REFINEMENT, ctypes.byref(v)), self._h)
+        return bool(v.value)
 
     def set_phase_timing(self, enable=True):
         _abi.check(self._lib.om_set_phase_timing(self._h, 1 if enable else 0), self._h)

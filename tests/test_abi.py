@@ -125,3 +125,13 @@ def test_synthetic_stream_is_deterministic():
     a = uniform(1, (4, 3, 2), -1, 1)
     b = uniform(1, (4, 3, 2), -1, 1)
     assert np.array_equal(a, b) and a.min() >= -1 and a.max() < 1
+
+
+def test_tuning_keys_match_the_header():
+    import re
+
+    from oceananigans_mi355x import _abi
+
+    text = open(HEADER).read()
+    keys = dict(re.findall(r"#define\s+(OM_TUNE_\w+)\s+(\d+)", text))
+    assert keys and all(getattr(_abi, k) == int(v) for k, v in keys.items()), keys
