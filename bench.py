@@ -47,6 +47,7 @@ def run_gpu(args):
     dt = 1.0
     if args.warmup:
         model.time_step(dt, args.warmup)
+    kchunk = model.autotune()                # z-chunk length of the tiled kernels, timed on this state
     model.set_phase_timing(True)
     t0 = time.perf_counter()
     model.time_step(dt, args.steps)          # synchronous: returns after the device has finished
@@ -69,7 +70,7 @@ def run_gpu(args):
     div = float(np.abs(model.divergence()).max())
     finite = all(np.isfinite(model.velocities[v].interior()).all() for v in "uvw")
     model.close()
-    return dict(N=N, ncell=ncell, wall=wall, ev_ms=ev_ms, phases=phases, dom=dom, dom_avg_ms=dom_avg_s * 1e3,
+    return dict(N=N, kchunk=kchunk, ncell=ncell, wall=wall, ev_ms=ev_ms, phases=phases, dom=dom, dom_avg_ms=dom_avg_s * 1e3,
                 dom_bytes=dom_bytes, achieved=achieved, bytes_step=bytes_step, word=word, div=div, finite=finite)
 
 
@@ -141,6 +142,59 @@ def run_cg_seamount(args):
                          "pair_capacity": args.cg_pairs, "wall_s": t_train}}
 
 
+def run_ensemble(args):
+    """Batched ensemble of uneven-size free-convection LES (FP32, WENO5 + AMD): members advanced one
+    after another (om_time_step per member, as the reference would) against all members advanced
+    together (one captured two-step hipGraph per member, members' streams in flight at once), after
+    the autotuner has picked each size's z-chunk length.  Total cell-updates/s of both."""
+    import oceananigans_mi355x as om
+    from oceananigans_mi355x import workloads
+
+    t0 = time.perf_counter()
+    cfgs = workloads.ensemble_members(n=args.ensemble_members, seed=5)
+    members = []
+    for grid, kw, init in cfgs:
+        m = om.NonhydrostaticModel(grid, **kw)
+        om.set_(m, **init)
+        members.append(m)
+    print(f"bench.py: ensemble of {len(members)} members created in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    tuned = {}
+    for m in members:                       # one autotune per distinct size
+        key = m.grid.N
+        if key not in tuned:
+            m.time_step(1.0, 1)
+            tuned[key] = m.autotune()
+        else:
+            m.set_tuning(kchunk=tuned[key])
+    ncell = sum(int(np.prod(m.grid.N)) for m in members)
+    steps, dt = args.ensemble_steps, 1.0
+    for m in members:                       # warm-up: first steps and captured graphs
+        m.time_step(dt, 1)
+    ens = om.Ensemble(members)
+    ens.time_step(dt, 4)
+    t0 = time.perf_counter()
+    for m in members:
+        m.time_step(dt, steps)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.time_step(dt, steps)
+    t_ens = time.perf_counter() - t0
+    finite = all(np.isfinite(m.velocities.w.interior()).all() for m in members[:4])
+    sizes = {}
+    for m in members:
+        k = "x".join(map(str, m.grid.N))
+        sizes[k] = sizes.get(k, 0) + 1
+    ens.close()
+    for m in members:
+        m.close()
+    return {"members": len(members), "sizes": sizes, "cells": ncell, "steps": steps,
+            "kchunk_by_size": {"x".join(map(str, k)): v for k, v in tuned.items()},
+            "one_after_another_cell_updates_per_s": ncell * steps / t_seq,
+            "together_cell_updates_per_s": ncell * steps / t_ens,
+            "speedup": t_seq / t_ens, "finite": bool(finite)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +210,9 @@ def main():
     ap.add_argument("--cg-rounds", type=int, default=4)
     ap.add_argument("--cg-train-steps", type=int, default=200)
     ap.add_argument("--cg-pairs", type=int, default=64)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble measurement")
+    ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_free_convection_256x256x128_fp32.json"),
@@ -185,6 +242,12 @@ def main():
             cg = run_cg_seamount(args)
         except Exception as e:                       # reported, not hidden: the main metric stands alone
             cg = {"error": f"{type(e).__name__}: {e}"}
+    ens = None
+    if not args.no_ensemble:
+        try:
+            ens = run_ensemble(args)
+        except Exception as e:                       # reported, not hidden
+            ens = {"error": f"{type(e).__name__}: {e}"}
     step_gbs = r["bytes_step"] * args.steps / r["wall"] / 1e9
     traffic = None
     if os.path.exists(args.traffic):
@@ -206,7 +269,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 initial buoyancy noise; the reference's configuration)",
         "config": {"workload": "free_convection_LES", "grid": "x".join(map(str, r["N"])),
                    "advection": "WENO5", "closure": "AnisotropicMinimumDissipation", "buoyancy": "BuoyancyTracer",
-                   "coriolis": "FPlane(f=1e-4)", "pressure_solver": "FFT", "dt": 1.0, "parallelism": "single GPU"},
+                   "coriolis": "FPlane(f=1e-4)", "pressure_solver": "FFT", "dt": 1.0, "parallelism": "single GPU",
+                   "kchunk": r["kchunk"]},
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": r["dom"],
                      "kernel_avg_ms": r["dom_avg_ms"], "algorithmic_bytes_per_launch": r["dom_bytes"]},
@@ -218,6 +282,7 @@ def main():
         "finite": r["finite"],
         "cpu_baseline": cpu,
         "cg_seamount": cg,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

@@ -284,6 +284,25 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 
+/* Autotuner (no reference counterpart): times the tiled tendency and closure kernels on the
+ * model's current state for z-chunk lengths 4 .. 64 and keeps the fastest (OM_TUNE_KCHUNK).
+ * Results do not depend on the choice.  *kchunk (may be NULL) receives it. */
+int om_autotune(om_model_t *m, int32_t *kchunk);
+
+/* Ensembles of independent models (no reference counterpart for NonhydrostaticModel: the
+ * reference advances such members one after another, `time_step!` per model).  Members are
+ * created with om_create, may differ in size and configuration, and stay owned by the caller;
+ * the conjugate-gradient solver is not allowed (its steps read scalars on the host).
+ * om_ensemble_time_step advances every member nsteps steps of Δt: each member's two-step launch
+ * sequence is recorded once into a hipGraph on its own stream (use_graphs = 1) and replayed,
+ * members issued in rounds so that their streams run concurrently; it returns when every member
+ * is done.  use_graphs = 0 issues the same launches directly. */
+typedef struct om_ensemble om_ensemble_t;
+int         om_ensemble_create(om_model_t *const *members, int32_t n, om_ensemble_t **out);
+int         om_ensemble_time_step(om_ensemble_t *e, double dt, int32_t nsteps, int32_t use_graphs);
+int         om_ensemble_destroy(om_ensemble_t *e);
+const char *om_ensemble_last_error(const om_ensemble_t *e);   /* NULL: the last create error */
+
 #ifdef __cplusplus
 }
 #endif

@@ -231,6 +231,7 @@ template <class T> struct ModelT : ModelBase {
 
     ~ModelT() override {
         if (st) (void)hipStreamSynchronize(st);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void *p : allocs) (void)hipFree(p);
         if (cg_host) (void)hipHostFree(cg_host);
         if (diag_host) (void)hipHostFree(diag_host);
@@ -750,28 +751,109 @@ template <class T> struct ModelT : ModelBase {
         // G⁻ ← Gⁿ by exchanging the buffers: the next tendency evaluation overwrites every
         // interior point of Gⁿ, so the copy is not needed.
         for (int f = 0; f < nf; ++f) { T *t = s.Gm[f]; s.Gm[f] = s.Gn[f]; s.Gn[f] = t; }
+        swap_parity ^= 1;
     }
-    void time_step(double dt) {
-        const T g1 = T(8.0 / 15), g2 = T(5.0 / 12), g3 = T(3.0 / 4), z2 = T(-17.0 / 60), z3 = T(-5.0 / 12);
-        if (iteration == 0) {
-            last_dt = dt;
-            last_stage_dt = dt * (double)g1;
-            update_state();
-        }
-        const double tn1 = time + dt;
+    // RK3 coefficients (runge_kutta_3.jl:36-48)
+    static constexpr double kG1 = 8.0 / 15, kG2 = 5.0 / 12, kG3 = 3.0 / 4, kZ2 = -17.0 / 60, kZ3 = -5.0 / 12;
+    // the device work of one time step: every launch depends only on Δt and the buffer assignment
+    void step_launches(double dt) {
+        const T g1 = T(kG1), g2 = T(kG2), g3 = T(kG3), z2 = T(kZ2), z3 = T(kZ3);
         rk3_substep(dt, g1, T(0), true);
         cache_previous_tendencies();
-        time += dt * (double)g1; stage = 2; last_stage_dt = dt * (double)g1;
         update_state();
         rk3_substep(dt, g2, z2, false);
         cache_previous_tendencies();
-        time += dt * (double)(g2 + z2); stage = 3; last_stage_dt = dt * (double)(g2 + z2);
         update_state();
         rk3_substep(dt, g3, z3, false);
         cache_previous_tendencies();
+        update_state();
+    }
+    // the clock of one time step (runge_kutta_3.jl:103-168: tick! per substage, the last one
+    // corrected so that the step ends at t + Δt)
+    void step_clock(double dt) {
+        const T g1 = T(kG1), g2 = T(kG2), z2 = T(kZ2);
+        const double tn1 = time + dt;
+        time += dt * (double)g1;
+        time += dt * (double)(g2 + z2);
         const double corrected = tn1 - time;
         time += corrected; iteration += 1; stage = 1; last_dt = dt; last_stage_dt = corrected;
-        update_state();
+    }
+    void time_step(double dt) {
+        if (iteration == 0) {
+            last_dt = dt;
+            last_stage_dt = dt * (double)T(kG1);
+            update_state();
+        }
+        step_launches(dt);
+        step_clock(dt);
+    }
+
+    // ---- captured steps: two time steps recorded once into a hipGraph and replayed.  Two steps
+    // return G⁻/Gⁿ to the buffers they started in (three exchanges per step), so a replay is valid
+    // whenever the exchange parity matches the one at capture.  CG solves read scalars on the
+    // host and are never captured.
+    hipGraphExec_t graph_exec = nullptr;
+    double graph_dt = 0;
+    int graph_parity = 0;
+    int swap_parity = 0;   // parity of the G⁻/Gⁿ exchanges so far
+    bool capturable() const { return !cg; }
+    void drop_graph() {
+        if (graph_exec) { HIPCHECK(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
+    }
+    void capture(double dt) {
+        drop_graph();
+        const bool pt = phase_timing;
+        phase_timing = false;
+        hipGraph_t gr = nullptr;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        step_launches(dt);
+        step_launches(dt);
+        HIPCHECK(hipStreamEndCapture(st, &gr));
+        phase_timing = pt;
+        HIPCHECK(hipGraphInstantiate(&graph_exec, gr, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphDestroy(gr));
+        graph_dt = dt;
+        graph_parity = swap_parity;
+    }
+    // enqueue (without waiting) up to two steps of Δt; returns the number of steps issued
+    int issue_steps(double dt, int left, bool use_graph) {
+        if (!use_graph || !capturable() || iteration == 0 || left < 2) { time_step(dt); return 1; }
+        if (!graph_exec || graph_dt != dt) capture(dt);
+        if (graph_parity != swap_parity) { time_step(dt); return 1; }
+        HIPCHECK(hipGraphLaunch(graph_exec, st));
+        step_clock(dt);
+        step_clock(dt);
+        return 2;
+    }
+
+    // ---- autotuner: the z-chunk length of the tiled kernels, timed on this model's own state
+    int autotune() {
+        const Ctx<T> C = ctx();
+        if (!tiled_tendencies_supported(C)) return tune_kchunk;
+        const bool clo = (cfg.closure == OM_CLOSURE_AMD || cfg.closure == OM_CLOSURE_SMAGORINSKY_LILLY) && tiled_closure_supported(C);
+        std::vector<int> cand;
+        for (int kc : {4, 8, 12, 16, 24, 32, 64}) if (kc <= g.Nz) cand.push_back(kc);
+        if (cand.empty()) cand.push_back(g.Nz);
+        auto run = [&](int kc) {
+            if (clo) launch_closure_tiled(C, kc, st);
+            launch_tend_tiled(C, kc, st);
+        };
+        float best = INFINITY;
+        int pick = tune_kchunk;
+        for (int kc : cand) {
+            run(kc);
+            HIPCHECK(hipEventRecord(ev0, st));
+            for (int r = 0; r < 5; ++r) run(kc);
+            HIPCHECK(hipEventRecord(ev1, st));
+            HIPCHECK(hipEventSynchronize(ev1));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            if (ms < best) { best = ms; pick = kc; }
+        }
+        // the re-evaluations leave νₑ, κₑ and Gⁿ as they were (same state, same values)
+        tune_kchunk = pick;
+        drop_graph();
+        return pick;
     }
 
     T *field_ptr(int id, bool &is_w) {
@@ -1301,6 +1383,78 @@ static bool known_tuning_key(int32_t key) {
            key == OM_TUNE_POISSON_REFINEMENT;
 }
 
+int om_autotune(om_model_t *m, int32_t *kchunk) {
+    if (!m) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        const int kc = M->autotune();
+        if (kchunk) *kchunk = kc;
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+struct om_ensemble {
+    std::vector<om_model_t *> members;
+    std::string err;
+};
+
+int om_ensemble_create(om_model_t *const *members, int32_t n, om_ensemble_t **out) {
+    if (!out || !members || n < 1) return OM_EINVAL;
+    *out = nullptr;
+    auto *e = new om_ensemble;
+    for (int q = 0; q < n; ++q) {
+        om_model_t *m = members[q];
+        if (!m || m->broken) { delete e; g_create_error = "ensemble member " + std::to_string(q) + " is null or unusable"; return OM_EINVAL; }
+        bool ok = true;
+        const int rc = WITH_MODEL(m, { ok = M->capturable(); });
+        if (rc != OM_OK || !ok) {
+            delete e;
+            g_create_error = "ensemble member " + std::to_string(q) + " uses the conjugate-gradient solver, whose steps cannot be captured";
+            return OM_EINVAL;
+        }
+        for (om_model_t *x : e->members)
+            if (x == m) { delete e; g_create_error = "ensemble member " + std::to_string(q) + " is listed twice"; return OM_EINVAL; }
+        e->members.push_back(m);
+    }
+    *out = e;
+    return OM_OK;
+}
+
+int om_ensemble_destroy(om_ensemble_t *e) {
+    delete e;
+    return OM_OK;
+}
+
+int om_ensemble_time_step(om_ensemble_t *e, double dt, int32_t nsteps, int32_t use_graphs) {
+    if (!e || nsteps < 0) return OM_EINVAL;
+    // members advance in rounds of up to two steps each, so every member's stream holds work
+    // at once; then every stream is drained
+    std::vector<int> left(e->members.size(), nsteps);
+    bool more = nsteps > 0;
+    while (more) {
+        more = false;
+        for (size_t q = 0; q < e->members.size(); ++q) {
+            if (left[q] <= 0) continue;
+            om_model_t *m = e->members[q];
+            int done = 0;
+            const int rc = WITH_MODEL(m, { done = M->issue_steps(dt, left[q], use_graphs != 0); });
+            if (rc != OM_OK) { e->err = "member " + std::to_string(q) + ": " + m->err; return rc; }
+            left[q] -= done;
+            more = more || left[q] > 0;
+        }
+    }
+    for (size_t q = 0; q < e->members.size(); ++q) {
+        om_model_t *m = e->members[q];
+        const int rc = WITH_MODEL(m, {
+            HIPCHECK(hipStreamSynchronize(M->st));
+            HIPCHECK(hipGetLastError());
+        });
+        if (rc != OM_OK) { e->err = "member " + std::to_string(q) + ": " + m->err; return rc; }
+    }
+    return OM_OK;
+}
+
+const char *om_ensemble_last_error(const om_ensemble_t *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (!m) return OM_EINVAL;
     if (key == OM_TUNE_TENDENCY_KERNEL && (value < 0 || value > 1)) return OM_EINVAL;
@@ -1313,6 +1467,7 @@ int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
         else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = value;
         else if (key == OM_TUNE_LEARNED_ITERATIONS) M->tune_learned_iterations = value;
         else M->set_refinement(value);
+        M->drop_graph();   // a captured step holds the launch choices it was recorded with
     });
 }
 

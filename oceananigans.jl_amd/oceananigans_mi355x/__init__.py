@@ -30,7 +30,7 @@ from .physics import (  # noqa: F401
     ValueBoundaryCondition,
     WENO,
 )
-from .models import NonhydrostaticModel, build_config, set_, time_step_  # noqa: F401
+from .models import Ensemble, NonhydrostaticModel, build_config, set_, time_step_  # noqa: F401
 from .simulations import (  # noqa: F401
     Callback,
     Checkpointer,

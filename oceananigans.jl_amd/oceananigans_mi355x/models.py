@@ -332,6 +332,12 @@ class NonhydrostaticModel:
             warnings.warn("Δt == 0 may cause model blowup!")
         _abi.check(self._lib.om_time_step(self._h, float(dt), int(nsteps)), self._h)
 
+    def autotune(self):
+        """Pick the z-chunk length of the tiled kernels by timing them on the current state."""
+        kc = ctypes.c_int32()
+        _abi.check(self._lib.om_autotune(self._h, ctypes.byref(kc)), self._h)
+        return kc.value
+
     def last_solve_info(self):
         """(iterations, ‖V⁻¹r‖ the iteration stopped on, ‖V⁻¹(b − A x)‖ recomputed in FP64) of the last
         conjugate-gradient pressure solve."""
@@ -507,3 +513,36 @@ def set_(model, enforce_incompressibility=True, **kwargs):
 def time_step_(model, dt):
     """time_step!(model, Δt) — one RK3 step (runge_kutta_3.jl:103-168)."""
     model.time_step(dt, 1)
+
+
+class Ensemble:
+    """Independent models advanced together (no NonhydrostaticModel counterpart in the reference,
+    where members are stepped one after another).  Each member's two-step launch sequence is
+    recorded once into a hipGraph on its own stream and replayed, members issued in rounds so that
+    their streams run concurrently.  Members keep their own sizes and configurations."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        self._lib = _abi.load_library()
+        arr = (ctypes.c_void_p * len(self.models))(*[m._h for m in self.models])
+        h = ctypes.c_void_p()
+        rc = self._lib.om_ensemble_create(arr, len(self.models), ctypes.byref(h))
+        if rc != 0:
+            raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: {self._lib.om_ensemble_last_error(None).decode()}")
+        self._h = h
+
+    def time_step(self, dt, nsteps=1, graphs=True):
+        rc = self._lib.om_ensemble_time_step(self._h, float(dt), int(nsteps), 1 if graphs else 0)
+        if rc != 0:
+            raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: {self._lib.om_ensemble_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self._lib.om_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,9 @@ wind_convection   : WENO5, SmagorinskyLilly(C=0.16), linear EOS (α=2e-4, β=8e-
 turbulence2d      : (Periodic, Periodic, Flat), WENO5, u and v ~ U(-1, 1).
 stretched_convection : WENO5 + AMD + BuoyancyTracer (N² = 1e-6 s⁻²) on a geometrically stretched z with
                     u = 0.1 m/s + U(-1e-3, 1e-3) inflow noise.
+ensemble_members  : free-convection members of uneven sizes drawn from {64³, 96×96×64, 128×128×64,
+                    128³}, each with its own noise seed, N² ~ U(1e-6, 1e-5) s⁻² and surface buoyancy
+                    flux ~ U(1e-9, 1e-8) m²/s³ (FP32, WENO5 + AMD).
 seamount          : 10 km × 10 km × 1 km, z faces geometrically stretched (ratio 1.02, finest at the
                     surface), GridFittedBottom Gaussian seamount (600 m high, e-folding 1.5 km) at the
                     centre, WENO5 + AMD + BuoyancyTracer (N² = 1e-6 s⁻²), u = 0.1 m/s + 1e-3 noise
@@ -39,6 +42,21 @@ def free_convection(N=(256, 256, 128), L=None, float_type=np.float32, noise_velo
         init["v"] = uniform(seed + 11, tuple(N), -noise_velocity, noise_velocity)
         init["w"] = uniform(seed + 12, tuple(N), -noise_velocity, noise_velocity)
     return grid, kw, init
+
+
+ENSEMBLE_SIZES = ((64, 64, 64), (96, 96, 64), (128, 128, 64), (128, 128, 128))
+
+
+def ensemble_members(n=64, seed=5, float_type=np.float32, sizes=ENSEMBLE_SIZES):
+    """[(grid, kw, init)] for n free-convection members; sizes and parameters from one seeded draw."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        N = tuple(sizes[int(rng.integers(len(sizes)))])
+        N2 = float(rng.uniform(1e-6, 1e-5))
+        flux = float(rng.uniform(1e-9, 1e-8))
+        out.append(free_convection(N=N, float_type=float_type, seed=seed * 1000 + q, N2=N2, buoyancy_flux=flux))
+    return out
 
 
 def wind_convection(N=(512, 512, 256), float_type=np.float32, seed=2):
