@@ -270,7 +270,8 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
  * them: every tendency kernel gives identically rounded tendencies. */
 #define OM_TUNE_TENDENCY_KERNEL 0   /* 0: z-marching LDS-tiled tendency and closure kernels where supported
                                        (default); 1: thread-per-cell kernels */
-#define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernels (default 16) */
+#define OM_TUNE_KCHUNK          1   /* z levels marched by one block of the tiled kernels (default 16;
+                                       om_autotune may choose different ones for the two kernels) */
 #define OM_TUNE_LEARNED_ITERATIONS 2 /* leading iterations of each CG solve that apply the learned
                                         correction (0: all); later iterations use the FFT
                                         preconditioner.  Solutions agree to the solver tolerance. */
@@ -284,9 +285,10 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 
-/* Autotuner (no reference counterpart): times the tiled tendency and closure kernels on the
- * model's current state for z-chunk lengths 4 .. 64 and keeps the fastest (OM_TUNE_KCHUNK).
- * Results do not depend on the choice.  *kchunk (may be NULL) receives it. */
+/* Autotuner (no reference counterpart): times the tiled tendency and closure kernels, each on
+ * its own, on the model's current state for z-chunk lengths 4 .. 64 and keeps the fastest for
+ * each (OM_TUNE_KCHUNK sets both).  Results do not depend on the choice.  *kchunk (may be NULL)
+ * receives the tendency kernel's. */
 int om_autotune(om_model_t *m, int32_t *kchunk);
 
 /* Ensembles of independent models (no reference counterpart for NonhydrostaticModel: the

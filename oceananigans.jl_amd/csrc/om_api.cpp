@@ -191,7 +191,8 @@ template <class T> struct ModelT : ModelBase {
     std::vector<double> snap_isz;
     // launch-shape choices (om_set_tuning)
     int tune_tendency_kernel = 0;      // 0: tiled z-marching kernel where supported, 1: thread per cell
-    int tune_kchunk = 16;              // z levels per block of the tiled kernel
+    int tune_kchunk = 16;              // z levels per block of the tiled tendency kernel
+    int tune_kchunk_closure = 16;      // z levels per block of the tiled closure kernel
     int tune_learned_iterations = 0;   // leading CG iterations that apply the learned correction (0: all)
     int tune_refine = 0;               // FP64 iterative refinement of FFT / Fourier–tridiagonal solves
     RefineOp refop{};                  // the solver's operator for the refinement residual
@@ -488,7 +489,7 @@ template <class T> struct ModelT : ModelBase {
             {
                 PhaseScope ps(this, OM_PHASE_CLOSURE);
                 const Ctx<T> C = ctx();
-                if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) launch_closure_tiled(C, tune_kchunk, st);
+                if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) launch_closure_tiled(C, tune_kchunk_closure, st);
                 else launch_closure(C, st);
             }
             add(s.nu);
@@ -834,26 +835,27 @@ template <class T> struct ModelT : ModelBase {
         std::vector<int> cand;
         for (int kc : {4, 8, 12, 16, 24, 32, 64}) if (kc <= g.Nz) cand.push_back(kc);
         if (cand.empty()) cand.push_back(g.Nz);
-        auto run = [&](int kc) {
-            if (clo) launch_closure_tiled(C, kc, st);
-            launch_tend_tiled(C, kc, st);
+        // each kernel is timed on its own, so each gets its own chunk length
+        auto fastest = [&](auto launch) {
+            float best = INFINITY;
+            int pick = cand[0];
+            for (int kc : cand) {
+                launch(kc);
+                HIPCHECK(hipEventRecord(ev0, st));
+                for (int r = 0; r < 5; ++r) launch(kc);
+                HIPCHECK(hipEventRecord(ev1, st));
+                HIPCHECK(hipEventSynchronize(ev1));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+                if (ms < best) { best = ms; pick = kc; }
+            }
+            return pick;
         };
-        float best = INFINITY;
-        int pick = tune_kchunk;
-        for (int kc : cand) {
-            run(kc);
-            HIPCHECK(hipEventRecord(ev0, st));
-            for (int r = 0; r < 5; ++r) run(kc);
-            HIPCHECK(hipEventRecord(ev1, st));
-            HIPCHECK(hipEventSynchronize(ev1));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
-            if (ms < best) { best = ms; pick = kc; }
-        }
         // the re-evaluations leave νₑ, κₑ and Gⁿ as they were (same state, same values)
-        tune_kchunk = pick;
+        if (clo) tune_kchunk_closure = fastest([&](int kc) { launch_closure_tiled(C, kc, st); });
+        tune_kchunk = fastest([&](int kc) { launch_tend_tiled(C, kc, st); });
         drop_graph();
-        return pick;
+        return tune_kchunk;
     }
 
     T *field_ptr(int id, bool &is_w) {
@@ -1464,7 +1466,7 @@ int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (!known_tuning_key(key)) return OM_EINVAL;
     return WITH_MODEL(m, {
         if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
-        else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = value;
+        else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = M->tune_kchunk_closure = value;
         else if (key == OM_TUNE_LEARNED_ITERATIONS) M->tune_learned_iterations = value;
         else M->set_refinement(value);
         M->drop_graph();   // a captured step holds the launch choices it was recorded with

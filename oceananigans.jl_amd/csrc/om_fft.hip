@@ -177,7 +177,14 @@ template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const
     // divergence with periodic index wrap in x and y (halos of U* are not needed)
     const int ip = (i + 1 == C.g.Nx) ? 0 : i + 1;
     const int jp = (j + 1 == C.g.Ny) ? 0 : j + 1;
-    return C.divU(i, j, k, ip, jp);
+    if (C.g.kbot) return C.divU(i, j, k, ip, jp);
+    // without an immersed bottom: Ctx::divU's expression, the six loads addressed from one index
+    const int64_t o = C.at(i, j, k);
+    const T *u = C.s.U[0] + o, *v = C.s.U[1] + o, *w = C.s.U[2] + o;
+    const T sx = C.flx() ? T(0) : C.Axc(k) * u[ip - i] - C.Axc(k) * u[0];
+    const T sy = C.fly() ? T(0) : C.Ayc(k) * v[(int64_t)(jp - j) * C.g.sy] - C.Ayc(k) * v[0];
+    const T sz = C.flz() ? T(0) : C.g.Az * w[C.g.sz] - C.g.Az * w[0];
+    return C.iVc(k) * ((sx + sy) + sz);
 }
 
 template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArgs<T> A) {

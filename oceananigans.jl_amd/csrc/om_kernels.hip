@@ -496,28 +496,52 @@ template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs
 // pNHS = (p Δt) / max(ϵ, Δt) (pressure_correction.jl:103), so the kernel only reads the pressure
 // and never writes it: no thread overwrites a neighbour's value while it is being read.
 // ----------------------------------------------------------------------------------------
+// Each thread marches KC levels of one column: p(k − 1) is carried in a register instead of being
+// fetched again, and the loads of all KC levels are issued before the first store.
+constexpr int kCorrectLevels = 8;
+
 template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T dtp, int phi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int k0 = blockIdx.z * kCorrectLevels;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
-    const int64_t o = C.at(i, j, k);
+    const int n = min(kCorrectLevels, C.g.Nz - k0);
     const T *p = C.s.pNHS;
-    const T pc = p[o];
-    // on an immersed grid the gradient is a conditional δ of cell-centred pressure
     const IB<T> ib(C.g);
-    const T gx = ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : pc - p[C.at(i - 1, j, k)];
-    const T gy = ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : pc - p[C.at(i, j - 1, k)];
-    const T gz = ib.template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : pc - p[C.at(i, j, k - 1)];
-    if (phi) {   // ∂ϕ (pressure_correction.jl:82-89)
-        if (!C.flx()) C.s.U[0][o] -= gx * C.g.idx;
-        if (!C.fly()) C.s.U[1][o] -= gy * C.g.idy;
-        if (!C.flz()) C.s.U[2][o] -= gz * C.g.idzf[k];
-        return;
+    T pc[kCorrectLevels], pw[kCorrectLevels], ps[kCorrectLevels], U0[kCorrectLevels], U1[kCorrectLevels], U2[kCorrectLevels];
+    const T pb = p[C.at(i, j, k0 - 1)];   // the level below the chunk (a halo at the bottom wall)
+#pragma unroll
+    for (int q = 0; q < kCorrectLevels; ++q) {
+        if (q >= n) break;
+        const int k = k0 + q;
+        const int64_t o = C.at(i, j, k);
+        pc[q] = p[o];
+        pw[q] = p[C.at(i - 1, j, k)];
+        ps[q] = p[C.at(i, j - 1, k)];
+        U0[q] = C.s.U[0][o];
+        U1[q] = C.s.U[1][o];
+        U2[q] = C.s.U[2][o];
     }
-    if (!C.flx()) C.s.U[0][o] -= (dtp * gx) * C.g.idx;
-    if (!C.fly()) C.s.U[1][o] -= (dtp * gy) * C.g.idy;
-    if (!C.flz()) C.s.U[2][o] -= (dtp * gz) * C.g.idzf[k];
+#pragma unroll
+    for (int q = 0; q < kCorrectLevels; ++q) {
+        if (q >= n) break;
+        const int k = k0 + q;
+        const int64_t o = C.at(i, j, k);
+        const T pd = q == 0 ? pb : pc[q - 1];
+        // on an immersed grid the gradient is a conditional δ of cell-centred pressure
+        const T gx = ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : pc[q] - pw[q];
+        const T gy = ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : pc[q] - ps[q];
+        const T gz = ib.template zd<0, 0, 0>(i, j, k, i, j, k - 1) ? T(0) : pc[q] - pd;
+        if (phi) {   // ∂ϕ (pressure_correction.jl:82-89)
+            if (!C.flx()) C.s.U[0][o] = U0[q] - gx * C.g.idx;
+            if (!C.fly()) C.s.U[1][o] = U1[q] - gy * C.g.idy;
+            if (!C.flz()) C.s.U[2][o] = U2[q] - gz * C.g.idzf[k];
+            continue;
+        }
+        if (!C.flx()) C.s.U[0][o] = U0[q] - (dtp * gx) * C.g.idx;
+        if (!C.fly()) C.s.U[1][o] = U1[q] - (dtp * gy) * C.g.idy;
+        if (!C.flz()) C.s.U[2][o] = U2[q] - (dtp * gz) * C.g.idzf[k];
+    }
 }
 
 // mask_immersed_field! (mask_immersed_field.jl:65-80): zero on the immersed periphery of the
@@ -571,7 +595,8 @@ template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int f
 }
 template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, dtp, phi);
+    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, (C.g.Nz + kCorrectLevels - 1) / kCorrectLevels, blk), blk, 0, st,
+                       C, dtp, phi);
 }
 template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st) {
     if (!g.kbot) return;
