@@ -47,7 +47,11 @@ def run_gpu(args):
     dt = 1.0
     if args.warmup:
         model.time_step(dt, args.warmup)
-    kchunk = model.autotune()                # z-chunk length of the tiled kernels, timed on this state
+    if args.kchunk:                          # fixed (profiling runs: no autotuner launches in the trace)
+        model.set_tuning(kchunk=args.kchunk)
+        kchunk = args.kchunk
+    else:
+        kchunk = model.autotune()            # z-chunk length of the tiled kernels, timed on this state
     model.set_phase_timing(True)
     t0 = time.perf_counter()
     model.time_step(dt, args.steps)          # synchronous: returns after the device has finished
@@ -63,7 +67,7 @@ def run_gpu(args):
     dom_ms, dom_launches = phases[dom]
     per_cell = {"tendencies": 2 * n * word,                       # read n fields, write n tendencies
                 "poisson": 10 * word, "correction": 7 * word,
-                "rk_update": (4 * n + 1) * word, "closure": (3 + 2 * N_TRACERS) * word}.get(dom, 2 * n * word)
+                "rk_update": (4 * n + 1) * word, "closure": (3 + N_TRACERS + 1 + N_TRACERS) * word}.get(dom, 2 * n * word)
     dom_bytes = per_cell * ncell
     dom_avg_s = dom_ms / max(dom_launches, 1) / 1e3
     achieved = dom_bytes / dom_avg_s / 1e9 if dom_avg_s > 0 else 0.0
@@ -210,6 +214,7 @@ def main():
     ap.add_argument("--cg-rounds", type=int, default=4)
     ap.add_argument("--cg-train-steps", type=int, default=200)
     ap.add_argument("--cg-pairs", type=int, default=64)
+    ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble measurement")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=20)
