@@ -376,3 +376,63 @@ def test_refined_fp32_wind_convection_ten_steps():
     print(f"max |div U| after 10 refined FP32 steps {div:.2e} (max |U| {umax:.2e})", flush=True)
     compare_state(model, orc, kw, fields=("state",), tol=1e-5, label="fp32 refined wind 10 steps")
     model.close()
+
+
+def test_full_size_fp32_ten_steps_against_fp64():
+    """The 256×256×128 free-convection LES (with 1e-2 velocity noise so that every field varies) in
+    FP32 against the same model in FP64 on the GPU (the FP64 path is held to the oracle by the
+    tests above) from identical FP32-rounded inputs: every field within 1e-5 after 10 steps."""
+    grid32, kw, init = convection(N=(256, 256, 128), float_type=np.float32, noise_velocity=1e-2)
+    init = {k: np.asarray(v, dtype=np.float32).astype(np.float64) for k, v in init.items()}
+    grid64, kw64, _ = convection(N=(256, 256, 128), float_type=np.float64, noise_velocity=1e-2)
+    m32, m64 = product_for(grid32, kw, init), product_for(grid64, kw64, init)
+    for _ in range(10):
+        om.time_step_(m32, 1.0)
+        om.time_step_(m64, 1.0)
+    errs = {}
+    for n in ("u", "v", "w", "b"):
+        a = (m32.velocities[n] if n in "uvw" else m32.tracers[n]).interior()
+        b = (m64.velocities[n] if n in "uvw" else m64.tracers[n]).interior()
+        errs[n] = rel_err(a, b)
+    print("256x256x128 fp32 vs fp64 after 10 steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert all(e <= 1e-5 for e in errs.values())
+    m32.close()
+    m64.close()
+
+
+def test_full_size_fp32_thousand_steps_buoyancy_budget():
+    """1000 steps (Δt = 1 s) of the 256×256×128 FP32 benchmark configuration: the fields stay
+    finite and projected, and the buoyancy budget closes: a top flux F counts upward, so the
+    surface removes F·Lx·Ly·t, Σ (b − b₀) ΔV = −F·Lx·Ly·t (no-flux bottom, periodic sides)."""
+    grid, kw, init = convection(N=(256, 256, 128), float_type=np.float32)
+    model = product_for(grid, kw, init)
+    b0 = model.tracers.b.interior().astype(np.float64).sum()
+    for _ in range(10):
+        model.time_step(1.0, 100)
+    w = model.velocities.w.interior()
+    assert np.isfinite(w).all()
+    div = np.abs(model.divergence()).max()
+    gained = model.tracers.b.interior().astype(np.float64).sum() - b0     # ΔV = 1 m³
+    expected = -1e-8 * 256 * 256 * 1000.0
+    print(f"1000 steps: max|w| {np.abs(w).max():.3e}, max|div| {div:.3e}, buoyancy gained {gained:.6e} "
+          f"expected {expected:.6e}", flush=True)
+    assert div <= 1e-4 * np.abs(w).max()
+    assert abs(gained - expected) <= 1e-3 * abs(expected)
+    model.close()
+
+
+def test_turbulence2d_400_steps_fp64():
+    """The 2D periodic turbulence configuration at its full size (128×128, WENO5, u, v ~ U(−1, 1),
+    Δt = 0.01, 400 steps to t = 4) against the FP64 oracle: u, v and the pressure."""
+    from oceananigans_mi355x import workloads
+
+    grid, kw, init = workloads.turbulence2d(N=(128, 128), float_type=np.float64)
+    model = product_for(grid, kw, init)
+    orc = oracle_for(grid, kw, init)
+    model.time_step(0.01, 400)
+    orc.time_step(0.01, 400)
+    errs = {n: rel_err(model.velocities[n].interior(), orc.get(q)) for q, n in enumerate("uv")}
+    errs["p"] = rel_err(model.pressures.pNHS.interior(), orc.get(om._abi.OM_FIELD_PNHS))
+    print("2D turbulence 400 steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert all(e <= 1e-9 for e in errs.values())
+    model.close()
