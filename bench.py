@@ -199,6 +199,40 @@ def run_ensemble(args):
             "speedup": t_seq / t_ens, "finite": bool(finite)}
 
 
+def run_mixed(args):
+    """The mixed-precision wind-and-convection LES at full size (512×512×256, WENO5, SmagorinskyLilly,
+    linear EOS with T and S, surface momentum and heat fluxes; tendencies and FFTs in FP32): device
+    time per step without and with the FP64 iterative-refinement residual on every pressure solve."""
+    import oceananigans_mi355x as om
+    from oceananigans_mi355x import workloads
+
+    N = tuple(int(x) for x in args.mixed_size.split(","))
+    grid, kw, init = workloads.wind_convection(N=N, float_type=np.float32)
+    model = om.NonhydrostaticModel(grid, **kw)
+    om.set_(model, **init)
+    model.time_step(1.0, 2)
+    kchunk = model.autotune()
+    res = {"grid": "x".join(map(str, N)), "kchunk": kchunk, "steps": args.mixed_steps}
+    for name, refine in (("fp32", False), ("fp32_fp64_refinement", True)):
+        model.set_tuning(poisson_refinement=refine)
+        model.time_step(1.0, 1)
+        model.set_phase_timing(True)
+        t0 = time.perf_counter()
+        model.time_step(1.0, args.mixed_steps)
+        wall = time.perf_counter() - t0
+        ph = model.phase_times()
+        model.set_phase_timing(False)
+        res[name] = {"ms_per_step": wall * 1e3 / args.mixed_steps,
+                     "cell_updates_per_s": N[0] * N[1] * N[2] * args.mixed_steps / wall,
+                     "poisson_ms_per_step": ph["poisson"][0] / args.mixed_steps}
+    w = model.velocities.w.interior()
+    res["finite"] = bool(np.isfinite(w).all())
+    res["max_abs_divergence"] = float(np.abs(model.divergence()).max())
+    res["max_abs_w"] = float(np.abs(w).max())
+    model.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +250,9 @@ def main():
     ap.add_argument("--cg-pairs", type=int, default=64)
     ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 512x512x256 mixed-precision wind-and-convection run")
+    ap.add_argument("--mixed-size", default="512,512,256")
+    ap.add_argument("--mixed-steps", type=int, default=10)
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -253,6 +290,12 @@ def main():
             ens = run_ensemble(args)
         except Exception as e:                       # reported, not hidden
             ens = {"error": f"{type(e).__name__}: {e}"}
+    mixed = None
+    if not args.no_mixed:
+        try:
+            mixed = run_mixed(args)
+        except Exception as e:                       # reported, not hidden
+            mixed = {"error": f"{type(e).__name__}: {e}"}
     step_gbs = r["bytes_step"] * args.steps / r["wall"] / 1e9
     traffic = None
     if os.path.exists(args.traffic):
@@ -288,6 +331,7 @@ def main():
         "cpu_baseline": cpu,
         "cg_seamount": cg,
         "ensemble": ens,
+        "mixed_precision_wind_convection": mixed,
     }
     print(json.dumps(out), flush=True)
 

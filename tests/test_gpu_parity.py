@@ -436,3 +436,21 @@ def test_turbulence2d_400_steps_fp64():
     print("2D turbulence 400 steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     assert all(e <= 1e-9 for e in errs.values())
     model.close()
+
+
+def test_mixed_precision_wind_convection_full_size():
+    """The mixed-precision wind-and-convection LES at its full size (512×512×256; tendencies and
+    FFTs in FP32, FP64 refinement residual on every pressure solve): three steps stay finite and
+    projected, and the refined model's divergence is no larger than the unrefined one's."""
+    grid, kw, init = wind_convection(N=(512, 512, 256), float_type=np.float32)
+    divs = {}
+    for refine in (False, True):
+        model = product_for(grid, kw, init)
+        model.set_tuning(poisson_refinement=refine)
+        model.time_step(1.0, 3)
+        w = model.velocities.w.interior()
+        assert np.isfinite(w).all()
+        divs[refine] = np.abs(model.divergence()).max() / np.abs(w).max()
+        model.close()
+    print("512x512x256 max|div U| / max|w|: plain", divs[False], "refined", divs[True], flush=True)
+    assert divs[True] <= 1e-5 and divs[True] <= 2 * divs[False]
