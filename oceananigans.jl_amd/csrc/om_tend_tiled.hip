@@ -185,27 +185,35 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
     for (int q = 0; q < 3 + NT; ++q) { zb[q][0] = zt[q][0]; zb[q][1] = zt[q][1]; }
 
     const bool stage_nu = has_clo && clo != OM_CLOSURE_SCALAR_DIFFUSIVITY;
-    for (kk = k0; kk < k1; ++kk) {
-        // The plane shares are rebuilt each level from an opaque copy of tid: a few integer
-        // operations, where keeping them live across the flux evaluation would cost registers.
+    // Plane values of the next level are loaded into registers while the current level's fluxes
+    // are evaluated, so their memory latency is hidden behind that work.  The plane shares are
+    // rebuilt for each load from an opaque copy of tid: a few integer operations, where keeping
+    // them live across the flux evaluation would cost registers.
+    T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2], rn[2], rk[NT > 0 ? NT : 1][2];
+    auto load_level = [&](int lev) {
         int otid = tid;
         asm volatile("" : "+v"(otid));
-        colb = colb0;
-        asm volatile("" : "+v"(colb));
         const PlaneShare<T, PW> pbig(g, i0, j0, 3, otid);
         const PlaneShare<T, NWD> psmall(g, i0, j0, 1, otid);
-        // this level's plane values: every load is issued before the barrier and the LDS stores
-        T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2], rn[2], rk[NT > 0 ? NT : 1][2];
-        pbig.load(ru, pbig.origin(C.s.U[0], g, i0, j0, 3, kk));
-        pbig.load(rv, pbig.origin(C.s.U[1], g, i0, j0, 3, kk));
+        pbig.load(ru, pbig.origin(C.s.U[0], g, i0, j0, 3, lev));
+        pbig.load(rv, pbig.origin(C.s.U[1], g, i0, j0, 3, lev));
 #pragma unroll
-        for (int n = 0; n < NT; ++n) pbig.load(rc[n], pbig.origin(C.s.U[3 + n], g, i0, j0, 3, kk));
-        pbig.load(rw, pbig.origin(C.s.U[2], g, i0, j0, 3, kk + 1));
-        if (stage_nu) psmall.load(rn, psmall.origin(C.s.nu, g, i0, j0, 1, kk + 1));
+        for (int n = 0; n < NT; ++n) pbig.load(rc[n], pbig.origin(C.s.U[3 + n], g, i0, j0, 3, lev));
+        pbig.load(rw, pbig.origin(C.s.U[2], g, i0, j0, 3, lev + 1));
+        if (stage_nu) psmall.load(rn, psmall.origin(C.s.nu, g, i0, j0, 1, lev + 1));
         if (clo == OM_CLOSURE_AMD) {
 #pragma unroll
-            for (int n = 0; n < NT; ++n) psmall.load(rk[n], psmall.origin(C.s.kap[n], g, i0, j0, 1, kk));
+            for (int n = 0; n < NT; ++n) psmall.load(rk[n], psmall.origin(C.s.kap[n], g, i0, j0, 1, lev));
         }
+    };
+    load_level(k0);
+    for (kk = k0; kk < k1; ++kk) {
+        colb = colb0;
+        asm volatile("" : "+v"(colb));
+        int otid = tid;
+        asm volatile("" : "+v"(otid));
+        const PlaneShare<T, PW> pbig(g, i0, j0, 3, otid);
+        const PlaneShare<T, NWD> psmall(g, i0, j0, 1, otid);
         // advance the own-column windows to kk-2 .. kk+3
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
@@ -234,6 +242,7 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
             for (int n = 0; n < NT; ++n) psmall.store(S.kap[n], rk[n], tid);
         }
         __syncthreads();
+        if (kk + 1 < k1) load_level(kk + 1);
 
         const T *U = S.u, *V = S.v, *W0 = S.w[kk & 1];
         const T *Nm = S.nu[nslot(kk - 1)], *N0 = S.nu[nslot(kk)];
