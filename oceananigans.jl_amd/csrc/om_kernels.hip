@@ -290,6 +290,50 @@ template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C
     }
 }
 
+// FP32 pHY′ with the column split into four z segments, one wave each, for 64 columns of one
+// row j: each thread forms its segment's terms ½(b(m) + b(m+1))·Δzᶠ(m+1) and their suffix sums
+// (at most kHySegLevels in registers), the segment totals are exchanged through LDS, and
+// p(k) = −(terms of the segments above + the local suffix sum).  The same sum as k_hydrostatic,
+// associated in segments, so FP32 results differ from the sequential order by rounding only.
+constexpr int kHySegLevels = 64;   // levels per segment held in registers (Nz ≤ 4·64)
+
+template <class T> __global__ void __launch_bounds__(256) k_hydrostatic_seg(Ctx<T> C) {
+    __shared__ T tot[4][64];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y;
+    const int seg = threadIdx.y;
+    const int Nz = C.g.Nz;
+    const int per = (Nz + 3) / 4;
+    const int lo = seg * per, hi = min(Nz, lo + per);   // levels lo .. hi-1 of this segment
+    const bool col = i < C.g.Nx;
+    T t[kHySegLevels];
+    T run = T(0);
+    if (col) {
+        // terms m = hi-1 down to lo; b(m+1) at m = Nz-1 is the halo value b(Nz)
+        T bu = C.buoy(i, j, hi);
+#pragma unroll
+        for (int q = 0; q < kHySegLevels; ++q) {
+            const int m = hi - 1 - q;
+            if (m < lo) break;
+            const T bm = C.buoy(i, j, m);
+            run += (T(0.5) * (bm + bu)) * C.g.dzf[m + 1];
+            t[q] = run;                                  // suffix sum of the segment from m up
+            bu = bm;
+        }
+    }
+    tot[seg][threadIdx.x] = run;
+    __syncthreads();
+    if (!col) return;
+    T above = T(0);
+    for (int s2 = seg + 1; s2 < 4; ++s2) above += tot[s2][threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < kHySegLevels; ++q) {
+        const int m = hi - 1 - q;
+        if (m < lo) break;
+        C.s.pHY[C.at(i, j, m)] = -(above + t[q]);
+    }
+}
+
 // ----------------------------------------------------------------------------------------
 // Halo fills (fill_halo_regions*.jl).  One launch per stage for up to kMaxFields fields.
 // Stage z: Flux/Value/Gradient one-layer fills of Center fields, impenetrable w faces.
@@ -585,6 +629,12 @@ template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st) {
     hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C);
 }
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
+    if (sizeof(T) == 4 && C.g.Nz >= 8 && C.g.Nz <= 4 * kHySegLevels) {
+        // FP32: four z segments per column scanned in parallel (the FP64 order is the reference's)
+        dim3 blk(64, 4, 1);
+        hipLaunchKernelGGL(k_hydrostatic_seg<T>, dim3((C.g.Nx + 63) / 64, C.g.Ny), blk, 0, st, C);
+        return;
+    }
     dim3 blk(64, 4, 1);
     hipLaunchKernelGGL(k_hydrostatic<T>, cell_grid(C.g.Nx, C.g.Ny, 1, blk), blk, 0, st, C);
 }
