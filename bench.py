@@ -298,10 +298,15 @@ def main():
             mixed = {"error": f"{type(e).__name__}: {e}"}
     step_gbs = r["bytes_step"] * args.steps / r["wall"] / 1e9
     traffic = None
+    valu_issue = None
     if os.path.exists(args.traffic):
         t = json.load(open(args.traffic))
         if t.get("kernel") == r["dom"] and t.get("grid") == "x".join(map(str, r["N"])):
             traffic = t["hbm_bytes_per_launch"]
+            if t.get("sq_insts_valu_per_launch") and r["dom_avg_ms"] > 0:
+                # the dominant kernel is issue-bound: its measured VALU wave-instructions per launch
+                # over what 256 CUs issue in the launch's duration at one per CU per cycle (2.4 GHz)
+                valu_issue = t["sq_insts_valu_per_launch"] / (256 * 2.4e9 * r["dom_avg_ms"] * 1e-3)
     out = {
         "metric": "cell-updates/s per RK3 step (256³-class FP32 LES: 256x256x128 WENO5+AMD free convection)",
         "value": value,
@@ -321,7 +326,8 @@ def main():
                    "kchunk": r["kchunk"]},
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": r["dom"],
-                     "kernel_avg_ms": r["dom_avg_ms"], "algorithmic_bytes_per_launch": r["dom_bytes"]},
+                     "kernel_avg_ms": r["dom_avg_ms"], "algorithmic_bytes_per_launch": r["dom_bytes"],
+                     "valu_issue_frac": valu_issue},
         "step_roofline": {"algorithmic_bytes_per_cell_step": algorithmic_bytes_per_cell_step(3 + N_TRACERS, r["word"]),
                           "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in r["phases"].items()},
