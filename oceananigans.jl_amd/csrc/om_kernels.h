@@ -158,6 +158,12 @@ template <class T> __device__ __forceinline__ T cen4(T m2, T m1, T z0, T p1) {
     typedef RC<T> K;
     return K::q_m2 * m2 + K::q_m1 * m1 + K::q_0 * z0 + K::q_p1 * p1;
 }
+// cen4 of four values that share one area factor A: in FP64 the reference's order,
+// cen4(A·m2, A·m1, A·z0, A·p1); in FP32 the factor applied once to the interpolated value.
+template <class T> __device__ __forceinline__ T acen4(T A, T m2, T m1, T z0, T p1) {
+    if (sizeof(T) == 4) return A * cen4<T>(m2, m1, z0, p1);
+    return cen4<T>(A * m2, A * m1, A * z0, A * p1);
+}
 template <class T> __device__ __forceinline__ T cen2(T m1, T z0) {
     typedef RC<T> K;
     return K::h_m1 * m1 + K::h_0 * z0;

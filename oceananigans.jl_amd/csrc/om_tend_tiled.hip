@@ -140,17 +140,17 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
         const int sf = C.zb_face(lev + 1), sc = C.zb_center(lev);
         // u at fcf face lev+1
         {
-            const T wt = cen4<T>(Az * P(W1, -2, 0), Az * P(W1, -1, 0), Az * P(W1, 0, 0), Az * P(W1, 1, 0));
+            const T wt = acen4<T>(Az, P(W1, -2, 0), P(W1, -1, 0), P(W1, 0, 0), P(W1, 1, 0));
             zt[0][0] = wt * zrec<T>(wu, wt > T(0), sf);
         }
         // v at cff face lev+1
         {
-            const T wt = cen4<T>(Az * P(W1, 0, -2), Az * P(W1, 0, -1), Az * P(W1, 0, 0), Az * P(W1, 0, 1));
+            const T wt = acen4<T>(Az, P(W1, 0, -2), P(W1, 0, -1), P(W1, 0, 0), P(W1, 0, 1));
             zt[1][0] = wt * zrec<T>(wv, wt > T(0), sf);
         }
         // w at ccc centre lev
         {
-            const T wt = C.zs4_center(lev) ? cen4<T>(Az * ww[1], Az * ww[2], Az * ww[3], Az * ww[4]) : cen2<T>(Az * ww[2], Az * ww[3]);
+            const T wt = C.zs4_center(lev) ? acen4<T>(Az, ww[1], ww[2], ww[3], ww[4]) : cen2<T>(Az * ww[2], Az * ww[3]);
             zt[2][0] = wt * zrec<T>(ww, wt > T(0), sc);
         }
 #pragma unroll
@@ -250,15 +250,23 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
         const T Axf = g.dy * g.dzf[kk], Ayf = g.dx * g.dzf[kk];
         const bool s4 = C.zs4_face(kk);
 
+        // νₑ at the ffc edge and the strain S12 there serve both the x-face flux of v and the
+        // y-face flux of u: evaluated once
+        T nffc_e = T(0), s12_e = T(0);
+        if (has_clo) {
+            nffc_e = clo == OM_CLOSURE_SCALAR_DIFFUSIVITY ? ph.c0
+                   : half * (half * (Q(N0, -1, -1) + Q(N0, 0, -1)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
+            s12_e = half * ((P(U, 0, 0) - P(U, 0, -1)) * g.idy + (P(V, 0, 0) - P(V, -1, 0)) * g.idx);
+        }
         // ---------------- x faces (thread a: u flux at centre i0+a-1, others at face i0+a)
         {
             const T A = Axc;
             {   // F_Uu at ccc (i-1): stencil centred one cell to the left
-                const T ut = cen4<T>(A * P(U, -2, 0), A * P(U, -1, 0), A * P(U, 0, 0), A * P(U, 1, 0));
+                const T ut = acen4<T>(A, P(U, -2, 0), P(U, -1, 0), P(U, 0, 0), P(U, 1, 0));
                 S.fx[0][tid] = ut * weno5<T>(P(U, -3, 0), P(U, -2, 0), P(U, -1, 0), P(U, 0, 0), P(U, 1, 0), P(U, 2, 0), ut > T(0));
             }
             {   // F_Uv at ffc face i
-                const T ut = cen4<T>(A * P(U, 0, -2), A * P(U, 0, -1), A * P(U, 0, 0), A * P(U, 0, 1));
+                const T ut = acen4<T>(A, P(U, 0, -2), P(U, 0, -1), P(U, 0, 0), P(U, 0, 1));
                 S.fx[2][tid] = ut * weno5<T>(P(V, -3, 0), P(V, -2, 0), P(V, -1, 0), P(V, 0, 0), P(V, 1, 0), P(V, 2, 0), ut > T(0));
             }
             {   // F_Uw at fcf face i, z-face kk (advecting u from this column's window)
@@ -273,16 +281,14 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
                 S.fx[2 * (3 + n)][tid] = A * ui * weno5<T>(P(Cn, -3, 0), P(Cn, -2, 0), P(Cn, -1, 0), P(Cn, 0, 0), P(Cn, 1, 0), P(Cn, 2, 0), ui > T(0));
             }
             if (has_clo) {
-                T nccc, nffc, nfcf;
-                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = nffc = nfcf = ph.c0; }
+                T nccc, nfcf;
+                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = nfcf = ph.c0; }
                 else {
                     nccc = Q(N0, -1, 0);
-                    nffc = half * (half * (Q(N0, -1, -1) + Q(N0, 0, -1)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
                     nfcf = half * (half * (Q(Nm, -1, 0) + Q(Nm, 0, 0)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
                 }
                 S.fx[1][tid] = A * (T(-2) * (nccc * ((P(U, 0, 0) - P(U, -1, 0)) * g.idx)));
-                const T s12 = half * ((P(U, 0, 0) - P(U, 0, -1)) * g.idy + (P(V, 0, 0) - P(V, -1, 0)) * g.idx);
-                S.fx[3][tid] = A * (T(-2) * (nffc * s12));
+                S.fx[3][tid] = A * (T(-2) * (nffc_e * s12_e));
                 const T s13 = half * ((wu[2] - wu[1]) * g.idzf[kk] + (P(W0, 0, 0) - P(W0, -1, 0)) * g.idx);
                 S.fx[5][tid] = Axf * (T(-2) * (nfcf * s13));
 #pragma unroll
@@ -300,11 +306,11 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
         {
             const T A = Ayc;
             {   // F_Vu at ffc face j
-                const T vt = cen4<T>(A * P(V, -2, 0), A * P(V, -1, 0), A * P(V, 0, 0), A * P(V, 1, 0));
+                const T vt = acen4<T>(A, P(V, -2, 0), P(V, -1, 0), P(V, 0, 0), P(V, 1, 0));
                 S.fy[0][tid] = vt * weno5<T>(P(U, 0, -3), P(U, 0, -2), P(U, 0, -1), P(U, 0, 0), P(U, 0, 1), P(U, 0, 2), vt > T(0));
             }
             {   // F_Vv at ccc (j-1)
-                const T vt = cen4<T>(A * P(V, 0, -2), A * P(V, 0, -1), A * P(V, 0, 0), A * P(V, 0, 1));
+                const T vt = acen4<T>(A, P(V, 0, -2), P(V, 0, -1), P(V, 0, 0), P(V, 0, 1));
                 S.fy[2][tid] = vt * weno5<T>(P(V, 0, -3), P(V, 0, -2), P(V, 0, -1), P(V, 0, 0), P(V, 0, 1), P(V, 0, 2), vt > T(0));
             }
             {   // F_Vw at cff face j, z-face kk
@@ -319,15 +325,13 @@ __global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(
                 S.fy[2 * (3 + n)][tid] = A * vj * weno5<T>(P(Cn, 0, -3), P(Cn, 0, -2), P(Cn, 0, -1), P(Cn, 0, 0), P(Cn, 0, 1), P(Cn, 0, 2), vj > T(0));
             }
             if (has_clo) {
-                T nffc, nccc, ncff;
-                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nffc = nccc = ncff = ph.c0; }
+                T nccc, ncff;
+                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = ncff = ph.c0; }
                 else {
-                    nffc = half * (half * (Q(N0, -1, -1) + Q(N0, 0, -1)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
                     nccc = Q(N0, 0, -1);
                     ncff = half * (half * (Q(Nm, 0, -1) + Q(Nm, 0, 0)) + half * (Q(N0, 0, -1) + Q(N0, 0, 0)));
                 }
-                const T s12 = half * ((P(U, 0, 0) - P(U, 0, -1)) * g.idy + (P(V, 0, 0) - P(V, -1, 0)) * g.idx);
-                S.fy[1][tid] = A * (T(-2) * (nffc * s12));
+                S.fy[1][tid] = A * (T(-2) * (nffc_e * s12_e));
                 S.fy[3][tid] = A * (T(-2) * (nccc * ((P(V, 0, 0) - P(V, 0, -1)) * g.idy)));
                 const T s23 = half * ((wv[2] - wv[1]) * g.idzf[kk] + (P(W0, 0, 0) - P(W0, 0, -1)) * g.idy);
                 S.fy[5][tid] = Ayf * (T(-2) * (ncff * s23));
