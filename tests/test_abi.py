@@ -95,6 +95,24 @@ def test_null_handle_calls_fail_cleanly():
     lib = _abi.load_library()
     assert lib.om_time_step(None, 1.0, 1) == -1
     assert lib.om_destroy(None) == -1
+    assert lib.om_autotune(None, None) == -1
+    kc = ctypes.c_int32()
+    assert lib.om_set_tuning(None, _abi.OM_TUNE_POISSON_REFINEMENT, 1) == -1
+    assert lib.om_get_tuning(None, _abi.OM_TUNE_POISSON_REFINEMENT, ctypes.byref(kc)) == -1
+
+
+def test_ensemble_arguments_are_checked_without_device_work():
+    lib = _abi.load_library()
+    out = ctypes.c_void_p()
+    null_members = (ctypes.c_void_p * 2)(None, None)
+    assert lib.om_ensemble_create(null_members, 2, ctypes.byref(out)) == -1
+    assert out.value is None
+    assert b"null" in lib.om_ensemble_last_error(None)
+    assert lib.om_ensemble_create(null_members, 0, ctypes.byref(out)) == -1
+    assert lib.om_ensemble_create(None, 1, ctypes.byref(out)) == -1
+    assert lib.om_ensemble_time_step(None, 1.0, 1, 1) == -1
+    assert lib.om_ensemble_time_step(None, 1.0, -1, 1) == -1
+    assert lib.om_ensemble_destroy(None) == 0
 
 
 def test_host_api_validation():
