@@ -127,6 +127,8 @@ def run_cg_seamount(args):
         print(f"bench.py: seamount training round {rnd}: energy loss {l0:.3f} -> {l1:.3f}", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t0
     learned.learned_log(iterations_per_solve=8, capacity=24)
+    if args.cg_learned_iterations:
+        learned.set_tuning(learned_iterations=args.cg_learned_iterations)
     state = om.prognostic_state(learned)
     om.restore_prognostic_state_(fft, state)
     res = {}
@@ -143,7 +145,8 @@ def run_cg_seamount(args):
     return {"grid": "x".join(map(str, N)), "reltol": 1e-8, "fft": res["fft"], "learned": res["learned"],
             "learned_over_fft_time": res["learned"]["ms_per_solve"] / res["fft"]["ms_per_solve"],
             "training": {"rounds": args.cg_rounds, "adam_steps_per_round": args.cg_train_steps,
-                         "pair_capacity": args.cg_pairs, "wall_s": t_train}}
+                         "pair_capacity": args.cg_pairs, "wall_s": t_train},
+            "network_iterations_per_solve": args.cg_learned_iterations or "all"}
 
 
 def run_ensemble(args):
@@ -245,9 +248,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the seamount CG measurement (learned vs FFT preconditioner)")
     ap.add_argument("--cg-size", default="256,256,128")
-    ap.add_argument("--cg-rounds", type=int, default=4)
-    ap.add_argument("--cg-train-steps", type=int, default=200)
-    ap.add_argument("--cg-pairs", type=int, default=64)
+    ap.add_argument("--cg-rounds", type=int, default=12)
+    ap.add_argument("--cg-train-steps", type=int, default=100)
+    ap.add_argument("--cg-pairs", type=int, default=96)
+    ap.add_argument("--cg-learned-iterations", type=int, default=2,
+                    help="leading CG iterations that apply the network (0: all)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 512x512x256 mixed-precision wind-and-convection run")
