@@ -512,6 +512,39 @@ template <class T> __global__ void __launch_bounds__(256) k_halo_fused(DevGrid<T
 // first (compute_flux_bcs.jl:113-162, runge_kutta_3.jl:196-204, nonhydrostatic_rk3_substep.jl:37-46)
 // ----------------------------------------------------------------------------------------
 
+// The same update for NF ≤ 5 fields with every load issued before the first store (a load placed
+// after a store to U cannot be moved above it, as the arrays might alias, which would expose one
+// memory round trip per field).  Same operations, same order, per field.
+template <class T, int NF> __global__ void __launch_bounds__(256) k_rk3_batched(Ctx<T> C, RKArgs<T> a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.g.Nx || j >= C.g.Ny) return;
+    const int64_t o = C.at(i, j, k);
+    const bool bz = C.bz();
+    const T Az = C.g.Az;
+    T G[NF], Um[NF], Gp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        G[f] = C.s.Gn[f][o];
+        Um[f] = C.s.U[f][o];
+        Gp[f] = a.first ? T(0) : C.s.Gm[f][o];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        T g = G[f];
+        if (bz && f != 2) {
+            bool changed = false;
+            if (k == 0 && C.ph.bc_bottom[f] == OM_BC_FLUX) { g += C.ph.bc_bottom_value[f] * Az / (Az * C.g.dzc[0]); changed = true; }
+            if (k == C.g.Nz - 1 && C.ph.bc_top[f] == OM_BC_FLUX) { g -= C.ph.bc_top_value[f] * Az / (Az * C.g.dzc[C.g.Nz - 1]); changed = true; }
+            if (changed) C.s.Gn[f][o] = g;
+        }
+        if (f == 2 && bz && k == 0 && C.g.Nz > 1) continue;   // exclude_periphery for w
+        if (a.first) C.s.U[f][o] = Um[f] + a.dt * a.gamma * g;
+        else C.s.U[f][o] = Um[f] + a.dt * (a.gamma * g + a.zeta * Gp[f]);
+    }
+}
+
 template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs<T> a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -641,7 +674,13 @@ template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st) {
     dim3 blk(64, 4, 1);
     RKArgs<T> a{dt, gamma, zeta, first, nfields};
-    hipLaunchKernelGGL(k_rk3<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, a);
+    const dim3 grid = cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk);
+    switch (nfields) {
+        case 3: hipLaunchKernelGGL((k_rk3_batched<T, 3>), grid, blk, 0, st, C, a); return;
+        case 4: hipLaunchKernelGGL((k_rk3_batched<T, 4>), grid, blk, 0, st, C, a); return;
+        case 5: hipLaunchKernelGGL((k_rk3_batched<T, 5>), grid, blk, 0, st, C, a); return;
+        default: hipLaunchKernelGGL(k_rk3<T>, grid, blk, 0, st, C, a);
+    }
 }
 template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
     dim3 blk(64, 4, 1);
