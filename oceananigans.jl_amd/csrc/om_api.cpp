@@ -415,7 +415,7 @@ template <class T> struct ModelT : ModelBase {
             cg_r = field_origin(dalloc<double>(fsize)); cg_b = field_origin(dalloc<double>(fsize));
             cg_x = field_origin(dalloc<double>(fsize)); cg_p = field_origin(dalloc<double>(fsize));
             cg_q = field_origin(dalloc<double>(fsize)); cg_z = field_origin(dalloc<T>(fsize));
-            cg_partial = dalloc<double>(4 * 1024);
+            cg_partial = dalloc<double>((size_t)cg_partials_needed(g.Nx, g.Ny, g.Nz));
             cg_out = dalloc<double>(4);
             if (cg_precond() == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_t = dalloc<T>((size_t)g.Nx * g.Ny * g.Nz);
             if (cg_precond() == OM_PRECOND_LEARNED) {

@@ -11,13 +11,14 @@
 // then one block summing the partials in a fixed order), so a solve is bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <limits>
 
 #include "om_kernels.h"
 
 namespace om {
 
-constexpr int kRedBlocks = 1024;   // partial sums per reduction
+constexpr int kRedBlocks = kCgRedBlocks;   // blocks and partial sums per value of a grid-wide reduction
 constexpr int kRedThreads = 256;
 
 // sum of v over the block, valid in thread 0 (64-wide waves, then across the 4 waves in LDS)
@@ -34,12 +35,29 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-// interior cell n (0 .. Nx·Ny·Nz-1) → (i, j, k)
-template <class T> __device__ __forceinline__ void cell_of(const DevGrid<T> &g, int64_t n, int &i, int &j, int &k) {
-    i = (int)(n % g.Nx);
-    const int64_t r = n / g.Nx;
-    j = (int)(r % g.Ny);
-    k = (int)(r / g.Ny);
+// The interior cells one thread visits.  A work unit is a 64-cell segment of one x row, taken by
+// one wave with a cell per lane: the unit's row (j, k) comes from scalar arithmetic and its
+// accesses are contiguous.  The waves of the grid take the units in turn, two per pass, so the
+// second unit's loads need not wait for the first unit's stores.  f(i, j, k, o, t) receives the
+// padded index o and the dense index t = i + Nx·(j + Ny·k).
+template <class T, class F> __device__ __forceinline__ void for_cells(const DevGrid<T> &g, F &&f) {
+    const int lane = threadIdx.x & 63;
+    const int nseg = (g.Nx + 63) >> 6;
+    const int units = nseg * g.Ny * g.Nz;
+    const int nw = (int)gridDim.x * ((int)blockDim.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * ((int)blockDim.x >> 6) + ((int)threadIdx.x >> 6));
+    auto visit = [&](int u) {
+        const int row = u / nseg, seg = u - row * nseg;
+        const int k = row / g.Ny, j = row - k * g.Ny;
+        const int i = (seg << 6) + lane;
+        if (i < g.Nx) f(i, j, k, idx3(i, j, k, g.sy, g.sz), (int64_t)i + (int64_t)g.Nx * row);
+    };
+    int u = w;
+    for (; u + nw < units; u += 2 * nw) {
+        visit(u);
+        visit(u + nw);
+    }
+    if (u < units) visit(u);
 }
 
 template <class T> __device__ __forceinline__ T vol(const DevGrid<T> &g, int k) { return g.Az * g.dzc[k]; }
@@ -49,99 +67,157 @@ template <class T> __device__ __forceinline__ double active(const DevGrid<T> &g,
     return IB<T>(g).inactive_cell(i, j, k) ? 0.0 : 1.0;
 }
 
-// V∇²ᶜᶜᶜ c (conjugate_gradient_poisson_solver.jl:62-66) evaluated in type E.  On an immersed grid
-// the inner ∂ vanishes across a face next to an immersed cell and the outer δ next to an immersed
-// inactive face (conditional_differences.jl:15-44).
-template <class E, class T, class F>
-__device__ __forceinline__ E vlap(const DevGrid<T> &g, const F *c, int64_t o, int i, int j, int k) {
-    const E idx = E(1) / E(g.dx), idy = E(1) / E(g.dy);
-    const E Ax = E(g.dy) * E(g.dzc[k]), Ay = E(g.dx) * E(g.dzc[k]), Az = E(g.dx) * E(g.dy);
-    const E c0 = E(c[o]);
-    E gxp = (E(c[o + 1]) - c0) * idx, gxm = (c0 - E(c[o - 1])) * idx;
-    E gyp = (E(c[o + g.sy]) - c0) * idy, gym = (c0 - E(c[o - g.sy])) * idy;
-    E gzp = (E(c[o + g.sz]) - c0) * (E(1) / E(g.dzf[k + 1])), gzm = (c0 - E(c[o - g.sz])) * (E(1) / E(g.dzf[k]));
-    bool ox = false, oy = false, oz = false;
-    if (g.kbot) {
-        const IB<T> ib(g);
-        const bool ic = ib.template iin<0, 0, 0>(i, j, k);
-        if (ic || ib.template iin<0, 0, 0>(i + 1, j, k)) gxp = E(0);
-        if (ic || ib.template iin<0, 0, 0>(i - 1, j, k)) gxm = E(0);
-        if (ic || ib.template iin<0, 0, 0>(i, j + 1, k)) gyp = E(0);
-        if (ic || ib.template iin<0, 0, 0>(i, j - 1, k)) gym = E(0);
-        if (ic || ib.template iin<0, 0, 0>(i, j, k + 1)) gzp = E(0);
-        if (ic || ib.template iin<0, 0, 0>(i, j, k - 1)) gzm = E(0);
-        ox = ib.template iin<1, 0, 0>(i, j, k) || ib.template iin<1, 0, 0>(i + 1, j, k);
-        oy = ib.template iin<0, 1, 0>(i, j, k) || ib.template iin<0, 1, 0>(i, j + 1, k);
-        oz = ib.template iin<0, 0, 1>(i, j, k) || ib.template iin<0, 0, 1>(i, j, k + 1);
+// The immersed-boundary predicates of the 7-point operator at any level of one column, from the
+// bottom indices of the column and of its four horizontal neighbours, loaded once (the rules of
+// IB::iin, grid_fitted_immersed_boundaries.jl / immersed_boundary_grid.jl).  Without an immersed
+// bottom every index is INT_MIN, which makes every immersed predicate false.
+template <class T> struct ColumnMask {
+    int c, xm, xp, ym, yp;
+    int Nz;
+    bool bounded;
+    __device__ __forceinline__ ColumnMask(const DevGrid<T> &g, int i, int j) : Nz(g.Nz), bounded(g.tz == OM_BOUNDED) {
+        if (g.kbot) {
+            auto kb = [&](int ii, int jj) { return g.kbot[IB<T>::wrap(ii, g.Nx) + g.Nx * IB<T>::wrap(jj, g.Ny)]; };
+            c = kb(i, j);
+            xm = kb(i - 1, j);
+            xp = kb(i + 1, j);
+            ym = kb(i, j - 1);
+            yp = kb(i, j + 1);
+        } else {
+            c = xm = xp = ym = yp = INT_MIN;
+        }
     }
-    const E tx = ox ? E(0) : Ax * gxp - Ax * gxm;
-    const E ty = oy ? E(0) : Ay * gyp - Ay * gym;
-    const E tz = oz ? E(0) : Az * gzp - Az * gzm;
+    __device__ __forceinline__ bool outside(int k) const { return bounded && (k < 0 || k >= Nz); }
+    __device__ __forceinline__ bool inactive(int kb, int k) const { return k < kb || outside(k); }
+    // immersed_inactive_node of a cell centre, of the x or y face between two cells, and of the
+    // z face below level k
+    __device__ __forceinline__ bool iin(int kb, int k) const { return inactive(kb, k) && !outside(k); }
+    __device__ __forceinline__ bool iin_face(int ka, int kb, int k) const {
+        return inactive(ka, k) && inactive(kb, k) && !outside(k);
+    }
+    __device__ __forceinline__ bool iin_zface(int k) const {
+        return inactive(c, k) && inactive(c, k - 1) && !(outside(k) && outside(k - 1));
+    }
+};
+
+// V∇²ᶜᶜᶜ at level k of a column (conjugate_gradient_poisson_solver.jl:62-66) in FP64 from the seven
+// stencil values.  On an immersed grid the inner ∂ vanishes across a face next to an immersed cell
+// and the outer δ next to an immersed inactive face (conditional_differences.jl:15-44).
+template <class T>
+__device__ __forceinline__ double vlap_col(const DevGrid<T> &g, const ColumnMask<T> &m, int k, double c0, double xm, double xp,
+                                           double ym, double yp, double zm, double zp) {
+    const double idx = 1.0 / (double)g.dx, idy = 1.0 / (double)g.dy;
+    const double Ax = (double)g.dy * (double)g.dzc[k], Ay = (double)g.dx * (double)g.dzc[k];
+    const double Az = (double)g.dx * (double)g.dy;
+    double gxp = (xp - c0) * idx, gxm = (c0 - xm) * idx;
+    double gyp = (yp - c0) * idy, gym = (c0 - ym) * idy;
+    double gzp = (zp - c0) * (1.0 / (double)g.dzf[k + 1]), gzm = (c0 - zm) * (1.0 / (double)g.dzf[k]);
+    const bool ic = m.iin(m.c, k);
+    if (ic || m.iin(m.xp, k)) gxp = 0.0;
+    if (ic || m.iin(m.xm, k)) gxm = 0.0;
+    if (ic || m.iin(m.yp, k)) gyp = 0.0;
+    if (ic || m.iin(m.ym, k)) gym = 0.0;
+    if (ic || m.iin(m.c, k + 1)) gzp = 0.0;
+    if (ic || m.iin(m.c, k - 1)) gzm = 0.0;
+    const bool ox = m.iin_face(m.c, m.xm, k) || m.iin_face(m.xp, m.c, k);
+    const bool oy = m.iin_face(m.c, m.ym, k) || m.iin_face(m.yp, m.c, k);
+    const bool oz = m.iin_zface(k) || m.iin_zface(k + 1);
+    const double tx = ox ? 0.0 : Ax * gxp - Ax * gxm;
+    const double ty = oy ? 0.0 : Ay * gyp - Ay * gym;
+    const double tz = oz ? 0.0 : Az * gzp - Az * gzm;
     return (tx + ty) + tz;
 }
 
 // ---------------------------------------------------------------------------------------------
-// mode 0: f ·= s, 1: f /= s, 2: f = s (interior)
-template <class T> __global__ void __launch_bounds__(256) k_cg_scale(DevGrid<T> g, T *f, T s, int mode) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        f[o] = mode == 2 ? s : (mode == 1 ? f[o] / s : f[o] * s);
+// Sweeps of the 7-point operator.  A block takes a 64 × 4 tile of columns and a chunk of levels
+// (cg_stencil_shape); each thread marches up its column with the values at k − 1, k and k + 1 in
+// registers, so a level costs five loads (the level above and the four horizontal neighbours),
+// and the immersed-boundary rules use the five bottom indices of ColumnMask.  Blocks are dealt to
+// the XCDs in contiguous runs of work items (xcd_work_item), the chunks of one tile consecutive.
+//   STENCIL_APPLY     q = V∇²p; partial sums of p·q
+//   STENCIL_RESIDUAL  r = b − V∇²x, through the initial VolumeInverseNorm; partial sums of (r V⁻¹)²
+//   STENCIL_TRUE      e = b − V∇²x; partial sums of e
+//   STENCIL_ENERGY    partial sums of −d·V∇²d and, when gout is given, gout = −2·s·active·V∇²d
+// ---------------------------------------------------------------------------------------------
+enum { STENCIL_APPLY = 0, STENCIL_RESIDUAL = 1, STENCIL_TRUE = 2, STENCIL_ENERGY = 3 };
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(kCgStTx * kCgStTy)
+    k_cg_stencil(DevGrid<T> g, const double *__restrict__ src, const double *__restrict__ b, double *__restrict__ out,
+                 float *__restrict__ gout, double gscale, double *__restrict__ partial, int kchunk, int ntx) {
+    const int a = threadIdx.x & (kCgStTx - 1), c = threadIdx.x / kCgStTx;
+    const int item = xcd_work_item((int)blockIdx.x, (int)gridDim.x);
+    const int nch = (g.Nz + kchunk - 1) / kchunk;
+    const int tile = item / nch, chunk = item - tile * nch;
+    const int ty = tile / ntx, tx = tile - ty * ntx;
+    const int i = tx * kCgStTx + a, j = ty * kCgStTy + c;
+    const int k0 = chunk * kchunk, k1 = min(k0 + kchunk, g.Nz);
+    double acc = 0;
+    if (i < g.Nx && j < g.Ny) {
+        const ColumnMask<T> m(g, i, j);
+        const int64_t col = idx3(i, j, 0, g.sy, g.sz), sy = g.sy, sz = g.sz;
+        const double *s = src + col;
+        double zm = s[(int64_t)(k0 - 1) * sz], c0 = s[(int64_t)k0 * sz];
+        for (int k = k0; k < k1; ++k) {
+            const int64_t o = (int64_t)k * sz;
+            const double zp = s[o + sz];
+            const double v = vlap_col(g, m, k, c0, s[o - 1], s[o + 1], s[o - sy], s[o + sy], zm, zp);
+            if (MODE == STENCIL_APPLY) {
+                out[col + o] = v;
+                acc += c0 * v;
+            } else if (MODE == STENCIL_RESIDUAL) {
+                const double rv = b[col + o] - v;
+                const double rs = rv * (double)(T(1) / vol(g, k));
+                acc += rs * rs;
+                out[col + o] = rs * (double)vol(g, k);
+            } else if (MODE == STENCIL_TRUE) {
+                const double e = b[col + o] - v;
+                out[col + o] = e;
+                acc += e;
+            } else {
+                acc -= c0 * v;
+                if (gout) {
+                    const double act = m.inactive(m.c, k) ? 0.0 : 1.0;
+                    gout[(int64_t)i + (int64_t)g.Nx * ((int64_t)j + (int64_t)g.Ny * k)] = (float)(-2.0 * gscale * act * v);
+                }
+            }
+            zm = c0;
+            c0 = zp;
+        }
     }
+    const double sum = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sum;
 }
 
-template <class T> __global__ void __launch_bounds__(256) k_cg_gather(DevGrid<T> g, const double *x, T *dense) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        dense[t] = (T)x[idx3(i, j, k, g.sy, g.sz)];
-    }
+// ---------------------------------------------------------------------------------------------
+// mode 0: f ·= s, 1: f /= s, 2: f = s (interior)
+template <class T> __global__ void __launch_bounds__(256) k_cg_scale(DevGrid<T> g, T *__restrict__ f, T s, int mode) {
+    for_cells(g, [&](int, int, int, int64_t o, int64_t) { f[o] = mode == 2 ? s : (mode == 1 ? f[o] / s : f[o] * s); });
+}
+
+template <class T> __global__ void __launch_bounds__(256) k_cg_gather(DevGrid<T> g, const double *__restrict__ x, T *__restrict__ dense) {
+    for_cells(g, [&](int, int, int, int64_t o, int64_t t) { dense[t] = (T)x[o]; });
 }
 
 // x = (double)(f · s) (the warm start, solve_for_pressure.jl:150-153), or f = (T)x
-template <class T> __global__ void __launch_bounds__(256) k_cg_xio(DevGrid<T> g, T *f, double *x, T s, int store) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_cg_xio(DevGrid<T> g, T *__restrict__ f, double *__restrict__ x, T s, int store) {
+    for_cells(g, [&](int, int, int, int64_t o, int64_t) {
         if (store) f[o] = (T)x[o];
         else x[o] = (double)(f[o] * s);
-    }
+    });
 }
 
 // b = active · divᶜᶜᶜ(U) · V (solve_for_pressure.jl:44-50), or b = rhs · V for a given dense rhs
 // (solve!(x, ::ConjugateGradientPoissonSolver, b) takes the caller's right-hand side as it is)
-template <class T> __global__ void __launch_bounds__(256) k_cg_source(Ctx<T> C, const T *dense_rhs, double *b) {
+template <class T> __global__ void __launch_bounds__(256) k_cg_source(Ctx<T> C, const T *__restrict__ dense_rhs, double *__restrict__ b) {
     const DevGrid<T> &g = C.g;
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         T d;
         if (dense_rhs) d = dense_rhs[t];
         else d = C.divU(i, j, k, i + 1, j + 1);
         const T act = (!dense_rhs && IB<T>(g).inactive_cell(i, j, k)) ? T(0) : T(1);
         b[o] = (double)((act * d) * vol(g, k));
-    }
-}
-
-// r = b − A x; partial sums of (r V⁻¹)² (the initial VolumeInverseNorm, which rescales r in place)
-template <class T> __global__ void __launch_bounds__(256) k_cg_residual(DevGrid<T> g, const double *x, const double *b, double *r,
-                                                                        double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double q = vlap<double>(g, x, o, i, j, k);
-        double rv = b[o] - q;
-        const double rs = rv * (double)(T(1) / vol(g, k));
-        acc += rs * rs;
-        r[o] = rs * (double)vol(g, k);
-    }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    });
 }
 
 // DiagonallyDominantPreconditioner (conjugate_gradient_poisson_solver.jl:256-296); r has its halos
@@ -152,11 +228,8 @@ template <class T> __device__ __forceinline__ double dd_ac(const DevGrid<T> &g, 
     const double azp = (double)g.Az * (double)(T(1) / g.dzf[k + 1]);
     return ((((-ax - ax) - ay) - ay) - azm) - azp;
 }
-template <class T> __global__ void __launch_bounds__(256) k_cg_dd(DevGrid<T> g, const double *r, T *z) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_cg_dd(DevGrid<T> g, const double *__restrict__ r, T *__restrict__ z) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double ax = (double)(g.dy * g.dzc[k]) * (double)(T(1) / g.dx);
         const double ay = (double)(g.dx * g.dzc[k]) * (double)(T(1) / g.dy);
         const double azm = (double)g.Az * (double)(T(1) / g.dzf[k]);
@@ -170,7 +243,7 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_dd(DevGrid<T> g, 
         acc = acc - azm / acm * r[o - g.sz];
         acc = acc - azp / acp * r[o + g.sz];
         z[o] = (T)((1.0 / fabs(ac) * acc) * active(g, i, j, k));
-    }
+    });
 }
 
 // ColumnwiseTridiagonalPreconditioner (conjugate_gradient_poisson_solver.jl:298-397): one thread per
@@ -220,127 +293,80 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_ct(DevGrid<T> g, 
 }
 
 // FFT preconditioner source: r · V⁻¹ into a dense interior array (conjugate_gradient_poisson_solver.jl:169-211)
-template <class T> __global__ void __launch_bounds__(256) k_cg_fft_src(DevGrid<T> g, const double *r, T *dense) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        dense[t] = (T)(r[idx3(i, j, k, g.sy, g.sz)] * (double)(T(1) / vol(g, k)));
-    }
+template <class T> __global__ void __launch_bounds__(256) k_cg_fft_src(DevGrid<T> g, const double *__restrict__ r, T *__restrict__ dense) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
+        dense[t] = (T)(r[o] * (double)(T(1) / vol(g, k)));
+    });
 }
 
-template <class T> __global__ void __launch_bounds__(256) k_cg_copy_r(DevGrid<T> g, const double *r, T *z) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_cg_copy_r(DevGrid<T> g, const double *__restrict__ r, T *__restrict__ z) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         z[o] = (T)r[o];
-    }
+    });
 }
 
 // partial sums of z·r
-template <class T> __global__ void __launch_bounds__(256) k_cg_dot_zr(DevGrid<T> g, const T *z, const double *r, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+template <class T> __global__ void __launch_bounds__(256) k_cg_dot_zr(DevGrid<T> g, const T *__restrict__ z, const double *__restrict__ r, double *partial) {
     double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         acc += (double)z[o] * r[o];
-    }
+    });
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // p = z (first iteration) or p = z + β p
-template <class T> __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *p, const T *z, double beta, int first) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__restrict__ p, const T *__restrict__ z, double beta, int first) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         p[o] = first ? (double)z[o] : (double)z[o] + beta * p[o];
-    }
+    });
 }
 
-// q = A p (p has its halos); partial sums of p·q
-template <class T> __global__ void __launch_bounds__(256) k_cg_apply(DevGrid<T> g, const double *p, double *q, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double qq = vlap<double>(g, p, o, i, j, k);
-        q[o] = qq;
-        acc += p[o] * qq;
-    }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
 
 // x += α p, r −= α q; partial sums of x and of r (for the zero-mean gauge)
-template <class T> __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *x, const double *p, double *r, const double *q,
+template <class T> __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *__restrict__ x, const double *__restrict__ p, double *__restrict__ r,
+                                                                  const double *__restrict__ q,
                                                                   double alpha, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double sx = 0, sr = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double xn = x[o] + alpha * p[o];
         const double rn = r[o] - alpha * q[o];
         x[o] = xn;
         r[o] = rn;
         sx += xn;
         sr += rn;
-    }
+    });
     const double a = block_sum(sx);
     const double b = block_sum(sr);
     if (threadIdx.x == 0) { partial[blockIdx.x] = a; partial[kRedBlocks + blockIdx.x] = b; }
 }
 
 // zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep
-template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *x, double *r, double mx, double mr,
+template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, double *__restrict__ r, double mx, double mr,
                                                                           double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double act = active(g, i, j, k);
         x[o] = (x[o] - mx) * act;
         const double rg = (r[o] - mr) * act;
         const double rs = rg * (double)(T(1) / vol(g, k));
         acc += rs * rs;
         r[o] = rs * (double)vol(g, k);
-    }
+    });
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// e = b − A x with A applied in FP64 to x (x has its halos); partial sums of e
-template <class T> __global__ void __launch_bounds__(256) k_cg_true_residual(DevGrid<T> g, const double *x, const double *b,
-                                                                             double *e, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double v = b[o] - vlap<double>(g, x, o, i, j, k);
-        e[o] = v;
-        acc += v;
-    }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
 
 // ‖V⁻¹(e − ē)‖² partial sums: the recomputed residual in the zero-mean gauge the iteration uses
 // (the mean of b − A x is the part of b outside the operator's range, which no x removes)
 template <class T> __global__ void __launch_bounds__(256) k_cg_gauged_norm(DevGrid<T> g, const double *e, double mean,
                                                                            double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const double v = (e[idx3(i, j, k, g.sy, g.sz)] - mean) / (double)vol(g, k);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
+        const double v = (e[o] - mean) / (double)vol(g, k);
         acc += v * v;
-    }
+    });
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -349,63 +375,48 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_gauged_norm(DevGr
 // Learned correction to the FFT preconditioner (the network is in om_learned.hip) and flexible CG
 // ---------------------------------------------------------------------------------------------
 // partial sums over active cells of (r V⁻¹)², z² and 1 (the input scales)
-template <class T> __global__ void __launch_bounds__(256) k_ln_scales(DevGrid<T> g, const double *r, const T *z, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+template <class T> __global__ void __launch_bounds__(256) k_ln_scales(DevGrid<T> g, const double *__restrict__ r, const T *__restrict__ z, double *partial) {
     double a0 = 0, a1 = 0, a2 = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double act = active(g, i, j, k);
         const double rv = r[o] * (double)(T(1) / vol(g, k)), zv = (double)z[o];
         a0 += act * rv * rv;
         a1 += act * zv * zv;
         a2 += act;
-    }
+    });
     const double s0 = block_sum(a0), s1 = block_sum(a1), s2 = block_sum(a2);
     if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; partial[2 * kRedBlocks + blockIdx.x] = s2; }
 }
 // network input channels, dense FP32 in cell order: r V⁻¹ / s_r, z / s_z, active
-template <class T> __global__ void __launch_bounds__(256) k_ln_input(DevGrid<T> g, const double *r, const T *z, double isr, double isz,
-                                                                     float *in3) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_ln_input(DevGrid<T> g, const double *__restrict__ r, const T *__restrict__ z, double isr,
+                                                                     double isz, float *__restrict__ in3) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         in3[3 * t] = (float)(r[o] * (double)(T(1) / vol(g, k)) * isr);
         in3[3 * t + 1] = (float)((double)z[o] * isz);
         in3[3 * t + 2] = (float)active(g, i, j, k);
-    }
+    });
 }
 // z += s_z · active · (network output)
-template <class T> __global__ void __launch_bounds__(256) k_ln_correct(DevGrid<T> g, T *z, const float *out, double sz) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_ln_correct(DevGrid<T> g, T *__restrict__ z, const float *__restrict__ out, double sz) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         z[o] = (T)((double)z[o] + sz * active(g, i, j, k) * (double)out[t]);
-    }
+    });
 }
 // interior copy of an FP64 cell field
-template <class T> __global__ void __launch_bounds__(256) k_cg_copy_d(DevGrid<T> g, const double *src, double *dst) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+template <class T> __global__ void __launch_bounds__(256) k_cg_copy_d(DevGrid<T> g, const double *__restrict__ src, double *__restrict__ dst) {
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         dst[o] = src[o];
-    }
+    });
 }
 // z·r and z·r_prev (the flexible-CG β of Polak–Ribière form)
-template <class T> __global__ void __launch_bounds__(256) k_cg_dot2(DevGrid<T> g, const T *z, const double *r, const double *rp,
+template <class T> __global__ void __launch_bounds__(256) k_cg_dot2(DevGrid<T> g, const T *__restrict__ z, const double *__restrict__ r, const double *__restrict__ rp,
                                                                     double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double a0 = 0, a1 = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double zv = (double)z[o];
         a0 += zv * r[o];
         a1 += zv * rp[o];
-    }
+    });
     const double s0 = block_sum(a0), s1 = block_sum(a1);
     if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; }
 }
@@ -414,65 +425,42 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_dot2(DevGrid<T> g
 // constant is free)
 template <class T> __global__ void __launch_bounds__(256) k_ln_target(DevGrid<T> g, const double *xf, const double *xk, const float *in3k,
                                                                       double isz, float *tgt, double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
     double a0 = 0, a1 = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double act = active(g, i, j, k);
         const double v = act * ((xf[o] - xk[o]) * isz - (double)in3k[3 * t + 1]);
         tgt[t] = (float)v;
         a0 += v;
         a1 += act;
-    }
+    });
     const double s0 = block_sum(a0), s1 = block_sum(a1);
     if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; }
 }
 template <class T> __global__ void __launch_bounds__(256) k_ln_center(DevGrid<T> g, float *tgt, float mean) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         tgt[t] = active(g, i, j, k) > 0 ? tgt[t] - mean : 0.0f;
-    }
+    });
 }
 template <class T> __global__ void __launch_bounds__(256) k_ln_mask(DevGrid<T> g, float *mask) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         mask[t] = (float)active(g, i, j, k);
-    }
+    });
 }
 
 // A-energy training loss of the learned correction.  d = act · (y − t) (dimensionless, in units
 // of rms(z_FFT)) goes into a padded FP64 field; after its halos, the loss is dᵀ(−V∇²)d with the
 // solver's own (conditional) operator, and its gradient with respect to y is −2·act·V∇²d.
 template <class T> __global__ void __launch_bounds__(256) k_ln_err(DevGrid<T> g, const float *y, const float *tgt, double *d) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         d[idx3(i, j, k, g.sy, g.sz)] = active(g, i, j, k) * ((y ? (double)y[t] : 0.0) - (double)tgt[t]);
-    }
-}
-template <class T> __global__ void __launch_bounds__(256) k_ln_energy(DevGrid<T> g, const double *d, double gscale, float *gout,
-                                                                      double *partial) {
-    const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-    double acc = 0;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k; cell_of(g, t, i, j, k);
-        const int64_t o = idx3(i, j, k, g.sy, g.sz);
-        const double ad = vlap<double>(g, d, o, i, j, k);
-        acc -= d[o] * ad;
-        if (gout) gout[t] = (float)(-2.0 * gscale * active(g, i, j, k) * ad);
-    }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    });
 }
 
-// out[v] = Σ partial[v·kRedBlocks + b] over b, in a fixed order
-static __global__ void __launch_bounds__(kRedThreads) k_cg_finish(const double *partial, int nvals, double *out) {
+// out[v] = Σ partial[v·nper + b] over b < nper, in a fixed order
+static __global__ void __launch_bounds__(kRedThreads) k_cg_finish(const double *partial, int nvals, int nper, double *out) {
     for (int v = 0; v < nvals; ++v) {
         double acc = 0;
-        for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) acc += partial[v * kRedBlocks + b];
+        for (int b = threadIdx.x; b < nper; b += kRedThreads) acc += partial[v * nper + b];
         const double s = block_sum(acc);
         if (threadIdx.x == 0) out[v] = s;
     }
@@ -514,6 +502,15 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_halo_r(DevGrid<T>
 static dim3 red_grid() { return dim3(kRedBlocks); }
 static dim3 red_blk() { return dim3(kRedThreads); }
 
+template <class T, int MODE>
+static void cg_stencil(const DevGrid<T> &g, const double *src, const double *b, double *out, float *gout, double gscale,
+                       double *partial, double *outv, hipStream_t st) {
+    const CgStencilShape s = cg_stencil_shape(g.Nx, g.Ny, g.Nz);
+    hipLaunchKernelGGL((k_cg_stencil<T, MODE>), dim3(s.nblocks), dim3(kCgStTx * kCgStTy), 0, st, g, src, b, out, gout, gscale,
+                       partial, s.kchunk, s.ntx);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, s.nblocks, outv);
+}
+
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_scale<T>, red_grid(), red_blk(), 0, st, g, f, s, mode);
 }
@@ -528,8 +525,7 @@ template <class T> void cg_source(const Ctx<T> &C, const T *dense_rhs, double *b
 }
 template <class T> void cg_residual(const DevGrid<T> &g, const double *x, const double *b, double *r, double *partial, double *out,
                                     hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, r, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    cg_stencil<T, STENCIL_RESIDUAL>(g, x, b, r, (float *)nullptr, 0.0, partial, out, st);
 }
 template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st) {
     const int nxy = g.Nx * g.Ny, nyz = g.Ny * g.Nz, nxz = g.Nx * g.Nz;
@@ -549,7 +545,7 @@ template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, 
 }
 template <class T> void ln_scales(const DevGrid<T> &g, const double *r, const T *z, double *partial, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_scales<T>, red_grid(), red_blk(), 0, st, g, r, z, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 3, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 3, kRedBlocks, out);
 }
 template <class T> void ln_input(const DevGrid<T> &g, const double *r, const T *z, double isr, double isz, float *in3, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_input<T>, red_grid(), red_blk(), 0, st, g, r, z, isr, isz, in3);
@@ -563,12 +559,12 @@ template <class T> void cg_copy_d(const DevGrid<T> &g, const double *src, double
 template <class T> void cg_dot2(const DevGrid<T> &g, const T *z, const double *r, const double *rp, double *partial, double *out,
                                 hipStream_t st) {
     hipLaunchKernelGGL(k_cg_dot2<T>, red_grid(), red_blk(), 0, st, g, z, r, rp, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
 }
 template <class T> void ln_target(const DevGrid<T> &g, const double *xf, const double *xk, const float *in3k, double isz, float *tgt,
                                   double *partial, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_target<T>, red_grid(), red_blk(), 0, st, g, xf, xk, in3k, isz, tgt, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
 }
 template <class T> void ln_center(const DevGrid<T> &g, float *tgt, float mean, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_center<T>, red_grid(), red_blk(), 0, st, g, tgt, mean);
@@ -577,8 +573,7 @@ template <class T> void ln_energy(const DevGrid<T> &g, const float *y, const flo
                                   double *partial, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_err<T>, red_grid(), red_blk(), 0, st, g, y, tgt, dscratch);
     cg_halo_r(g, dscratch, st);
-    hipLaunchKernelGGL(k_ln_energy<T>, red_grid(), red_blk(), 0, st, g, (const double *)dscratch, gscale, gout, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    cg_stencil<T, STENCIL_ENERGY>(g, dscratch, (const double *)nullptr, (double *)nullptr, gout, gscale, partial, out, st);
 }
 template <class T> void ln_mask(const DevGrid<T> &g, float *mask, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_mask<T>, red_grid(), red_blk(), 0, st, g, mask);
@@ -588,34 +583,32 @@ template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hi
 }
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_dot_zr<T>, red_grid(), red_blk(), 0, st, g, z, r, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
 template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_update_p<T>, red_grid(), red_blk(), 0, st, g, p, z, beta, first);
 }
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_apply<T>, red_grid(), red_blk(), 0, st, g, p, q, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    cg_stencil<T, STENCIL_APPLY>(g, p, (const double *)nullptr, q, (float *)nullptr, 0.0, partial, out, st);
 }
 template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, double alpha,
                               double *partial, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, alpha, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
 }
 template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
                                       double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, mx, mr, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
                                          double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_true_residual<T>, red_grid(), red_blk(), 0, st, g, x, b, e, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    cg_stencil<T, STENCIL_TRUE>(g, x, b, e, (float *)nullptr, 0.0, partial, out, st);
 }
 template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, double mean, double *partial, double *out,
                                        hipStream_t st) {
     hipLaunchKernelGGL(k_cg_gauged_norm<T>, red_grid(), red_blk(), 0, st, g, e, mean, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, out);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
 
 #define OM_CG_INSTANTIATE(T)                                                                                     \

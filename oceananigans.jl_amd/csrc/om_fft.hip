@@ -449,9 +449,13 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<
 
 // ---------------------------------------------------------------------------------------------
 // Fourier–tridiagonal z solve (Thomas per (kx, ky) column; diagonals from
-// fourier_tridiagonal_poisson_solver.jl:199-229).  One thread per column; t[k] kept in LDS.
-// The (0,0) column is singular: its last pivot is a rounding residue and is elided as in the
-// reference; the constant this leaves is removed with the column mean (ϕ -= mean(ϕ), :253).
+// fourier_tridiagonal_poisson_solver.jl:199-229).  One thread per column, the columns ℓ = ky·Mx + kx
+// flattened so that a level of a block's columns is one contiguous run (level k of column ℓ at
+// k·(Mx·Ny) + ℓ); t[k] kept in LDS.  Each sweep reads its levels in batches of kTriBatch, all
+// loads of a batch issued before its arithmetic, so a column costs 2·Nz/kTriBatch memory round
+// trips instead of 2·Nz.  The (0,0) column is singular: its last pivot is a rounding residue and
+// is elided as in the reference; the constant this leaves is removed with the column mean
+// (ϕ -= mean(ϕ), :253).
 // ---------------------------------------------------------------------------------------------
 template <class T> struct TriArgs {
     DevGrid<T> g;
@@ -462,52 +466,81 @@ template <class T> struct TriArgs {
     T zscale;   // (1 / L_backward_norm) applied later; unused here
 };
 
-template <class T> __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A) {
+template <class T> constexpr int tri_batch() { return sizeof(T) == 4 ? 16 : 8; }
+
+// The level coefficients and the spectrum come in as restrict-qualified arguments: the sweeps'
+// stores to S then provably leave them alone, so their (column-uniform) loads stay scalar.
+template <class T>
+__global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A, C2<T> *__restrict__ Sg, const T *__restrict__ dzc,
+                                                const T *__restrict__ idzf) {
     extern __shared__ unsigned char smem_raw[];
     T *tt = reinterpret_cast<T *>(smem_raw);   // [Nz][blockDim.x]
+    constexpr int KB = tri_batch<T>();
     const int Ny = A.g.Ny, Nz = A.g.Nz, Mx = A.Mx;
     const int bw = blockDim.x;
-    const int kx = blockIdx.x * bw + threadIdx.x;
-    const int j = blockIdx.y;
-    if (kx >= Mx) return;
+    const int64_t ncol = (int64_t)Mx * Ny;
+    const int64_t col = (int64_t)blockIdx.x * bw + threadIdx.x;
+    if (col >= ncol) return;
+    const int j = (int)(col / Mx), kx = (int)(col - (int64_t)j * Mx);
     const T lam = A.lx[kx] + A.ly[j];
-    const T *idzf = A.g.idzf;
     auto diag = [&](int k) -> T {   // 0-based k
-        const T dzc = A.g.dzc[k];
-        if (Nz == 1) return -dzc * lam;
-        if (k == 0) return -idzf[1] - dzc * lam;
-        if (k == Nz - 1) return -idzf[Nz - 1] - dzc * lam;
-        return -(idzf[k + 1] + idzf[k]) - dzc * lam;
+        const T dz = dzc[k];
+        if (Nz == 1) return -dz * lam;
+        if (k == 0) return -idzf[1] - dz * lam;
+        if (k == Nz - 1) return -idzf[Nz - 1] - dz * lam;
+        return -(idzf[k + 1] + idzf[k]) - dz * lam;
     };
     auto off = [&](int k) -> T { return idzf[k + 1]; };   // a[k] = c[k] = 1/Δzᶠ(k+1) (reference q = k+1)
-    auto at = [&](int k) -> int64_t { return ((int64_t)k * Ny + j) * Mx + kx; };
+    C2<T> *S = Sg + col;
+    auto at = [&](int k) -> int64_t { return (int64_t)k * ncol; };
+    // forward sweep: level 0, then full batches of KB levels (loads first), then the rest
     T beta = diag(0);
-    C2<T> f = A.S[at(0)];
-    C2<T> phi = cmk<T>(f.x / beta, f.y / beta);
-    A.S[at(0)] = phi;
-    for (int k = 1; k < Nz; ++k) {
+    C2<T> phi;
+    {
+        const C2<T> f = S[0];
+        phi = cmk<T>(f.x / beta, f.y / beta);
+        S[0] = phi;
+    }
+    auto fwd = [&](int k, const C2<T> &f) {
         const T c = off(k - 1), a = off(k - 1);
         const T t = c / beta;
         tt[k * bw + threadIdx.x] = t;
         beta = diag(k) - a * t;
-        f = A.S[at(k)];
         if (fabs(beta) > A.eps10) phi = cmk<T>((f.x - a * phi.x) / beta, (f.y - a * phi.y) / beta);
         else phi = cmk<T>(T(0), T(0));
-        A.S[at(k)] = phi;
+        S[at(k)] = phi;
+    };
+    int k = 1;
+    for (; k + KB <= Nz; k += KB) {
+        C2<T> f[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) f[u] = S[at(k + u)];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) fwd(k + u, f[u]);
     }
-    for (int k = Nz - 2; k >= 0; --k) {
-        const T t = tt[(k + 1) * bw + threadIdx.x];
-        C2<T> p = A.S[at(k)];
-        p = cmk<T>(p.x - t * phi.x, p.y - t * phi.y);
-        A.S[at(k)] = p;
-        phi = p;
+    for (; k < Nz; ++k) fwd(k, S[at(k)]);
+    // backward substitution from level Nz − 2 down, in the same batches
+    auto bwd = [&](int kk, const C2<T> &p) {
+        const T t = tt[(kk + 1) * bw + threadIdx.x];
+        const C2<T> q = cmk<T>(p.x - t * phi.x, p.y - t * phi.y);
+        S[at(kk)] = q;
+        phi = q;
+    };
+    k = Nz - 2;
+    for (; k - KB + 1 >= 0; k -= KB) {
+        C2<T> p[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) p[u] = S[at(k - u)];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) bwd(k - u, p[u]);
     }
-    if (kx == 0 && j == 0) {
+    for (; k >= 0; --k) bwd(k, S[at(k)]);
+    if (col == 0) {
         // remove the mean of the (0,0) column: equals removing the volume mean of ϕ
         C2<T> s = cmk<T>(T(0), T(0));
-        for (int k = 0; k < Nz; ++k) s = cadd<T>(s, A.S[at(k)]);
+        for (int q = 0; q < Nz; ++q) s = cadd<T>(s, S[at(q)]);
         s = cmk<T>(s.x / T(Nz), s.y / T(Nz));
-        for (int k = 0; k < Nz; ++k) A.S[at(k)] = csub<T>(A.S[at(k)], s);
+        for (int q = 0; q < Nz; ++q) S[at(q)] = csub<T>(S[at(q)], s);
     }
 }
 
@@ -620,7 +653,9 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
             TriArgs<T> A{g, P.S, Mx, P.lx, P.ly, P.eps10, T(1)};
             int bw = 64;   // columns per block, limited by the LDS holding t[k] (≤ 64 KiB)
             while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
-            hipLaunchKernelGGL(k_tridiag<T>, dim3((Mx + bw - 1) / bw, Ny), dim3(bw), (size_t)Nz * bw * sizeof(T), st, A);
+            const int64_t ncol = (int64_t)Mx * Ny;
+            hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)((ncol + bw - 1) / bw)), dim3(bw), (size_t)Nz * bw * sizeof(T), st, A,
+                               P.S, g.dzc, g.idzf);
         } else {
             const LineGeom Lz = line_geom(P.Lz_line, P.Lz_line + (P.zmode == 1 ? Nz : 0), e, lds);
             ZArgs<T> A{g, P.S, P.tw_z, P.tw_dct, Mx, P.Lz_line, P.zmode, P.lx, P.ly, P.lz, Lz};

@@ -2,6 +2,7 @@
 // used by the C-ABI host code.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "om_internal.h"
 
@@ -226,11 +227,20 @@ template <class T> struct IB {
     __device__ __forceinline__ explicit IB(const DevGrid<T> &gg) : g(gg) {}
     __device__ __forceinline__ bool on() const { return g.kbot != nullptr; }
     __device__ __forceinline__ bool outside(int k) const { return g.tz == OM_BOUNDED && (k < 0 || k >= g.Nz); }
+    // periodic wrap of a column index into [0, n): one add or subtract for the stencil offsets
+    // (|offset| < n), an integer remainder only for points further out (grids narrower than a stencil)
+    static __device__ __forceinline__ int wrap(int i, int n) {
+        if (i < 0) i += n;
+        else if (i >= n) i -= n;
+        if ((unsigned)i >= (unsigned)n) {
+            i %= n;
+            if (i < 0) i += n;
+        }
+        return i;
+    }
     __device__ __forceinline__ bool immersed(int i, int j, int k) const {
         if (!g.kbot) return false;
-        const int ii = i < 0 ? i + g.Nx * ((-i + g.Nx - 1) / g.Nx) : i % g.Nx;
-        const int jj = j < 0 ? j + g.Ny * ((-j + g.Ny - 1) / g.Ny) : j % g.Ny;
-        return k < g.kbot[ii % g.Nx + g.Nx * (jj % g.Ny)];
+        return k < g.kbot[wrap(i, g.Nx) + g.Nx * wrap(j, g.Ny)];
     }
     __device__ __forceinline__ bool inactive_cell(int i, int j, int k) const { return immersed(i, j, k) || outside(k); }
     template <int LX, int LY, int LZ> __device__ __forceinline__ bool all_inactive(int i, int j, int k, bool under) const {
@@ -644,6 +654,30 @@ template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStrea
 // mask_immersed_field! of a field at location loc (0 ccc, 1 fcc, 2 cfc, 3 ccf) on an immersed grid
 template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st);
 // conjugate-gradient Poisson solver kernels (om_cg.hip)
+// Grid-wide reductions of the elementwise kernels use kCgRedBlocks blocks (and partial sums per
+// value).  The 7-point operator sweeps take a 64 × 4 tile of columns and a chunk of kchunk levels
+// per block, with the chunk length chosen so that the grid holds at most kCgStencilPartials blocks
+// where the tiles allow it; cg_partials_needed is the partial-sum buffer a solver allocates.
+constexpr int kCgRedBlocks = 2048;
+constexpr int kCgStTx = 64, kCgStTy = 4;
+constexpr int kCgStencilPartials = 4096;
+struct CgStencilShape {
+    int ntx, nty, kchunk, nblocks;
+};
+inline CgStencilShape cg_stencil_shape(int Nx, int Ny, int Nz) {
+    CgStencilShape s;
+    s.ntx = (Nx + kCgStTx - 1) / kCgStTx;
+    s.nty = (Ny + kCgStTy - 1) / kCgStTy;
+    const int tiles = s.ntx * s.nty;
+    int nch = std::max(1, std::min((Nz + 15) / 16, kCgStencilPartials / tiles));
+    s.kchunk = (Nz + nch - 1) / nch;
+    nch = (Nz + s.kchunk - 1) / s.kchunk;
+    s.nblocks = tiles * nch;
+    return s;
+}
+inline int cg_partials_needed(int Nx, int Ny, int Nz) {
+    return std::max(4 * kCgRedBlocks, cg_stencil_shape(Nx, Ny, Nz).nblocks);
+}
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st);
 template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st);
 template <class T> void cg_xio(const DevGrid<T> &g, T *f, double *x, T s, int store, hipStream_t st);

