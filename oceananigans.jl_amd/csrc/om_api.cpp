@@ -416,7 +416,7 @@ template <class T> struct ModelT : ModelBase {
             cg_x = field_origin(dalloc<double>(fsize)); cg_p = field_origin(dalloc<double>(fsize));
             cg_q = field_origin(dalloc<double>(fsize)); cg_z = field_origin(dalloc<T>(fsize));
             cg_partial = dalloc<double>((size_t)cg_partials_needed(g.Nx, g.Ny, g.Nz));
-            cg_out = dalloc<double>(4);
+            cg_out = dalloc<double>(kCgSlots);
             if (cg_precond() == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_t = dalloc<T>((size_t)g.Nx * g.Ny * g.Nz);
             if (cg_precond() == OM_PRECOND_LEARNED) {
                 cg_rprev = field_origin(dalloc<double>(fsize));
@@ -657,10 +657,9 @@ template <class T> struct ModelT : ModelBase {
         const double res0 = std::sqrt(cg_read(1));
         const double tol = tol_override >= 0 ? tol_override : std::max(reltol * res0, abstol);
         cg_tol_last = tol;
-        cg_gauge_norm(g, x, cg_r, 0.0, 0.0, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
+        cg_gauge_norm(g, x, cg_r, (const double *)nullptr, ncell, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
         double res = std::sqrt(cg_read(1));
         int64_t it = 0;
-        double rho_prev = 0;
         // flexible CG for the learned preconditioner (a nonlinear map): β = zₖ·(rₖ − rₖ₋₁) / (zₖ₋₁·rₖ₋₁);
         // three iterations in a row that do not bring ‖V⁻¹r‖ below 0.999 of its best value so far hand
         // over to the FFT preconditioner for the rest of the solve
@@ -690,29 +689,19 @@ template <class T> struct ModelT : ModelBase {
                 }
             } else if (pk == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_ct(g, cg_r, cg_z, cg_t, st);
             else cg_copy_r(g, cg_r, cg_z, st);
-            double rho, beta;
-            if (flex && it > 0) {
-                cg_dot2(g, cg_z, cg_r, cg_rprev, cg_partial, cg_out, st);
-                cg_read(2);
-                rho = cg_host[0];
-                beta = (cg_host[0] - cg_host[1]) / rho_prev;
-            } else {
-                cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out, st);
-                rho = cg_read(1);
-                beta = it == 0 ? 0.0 : rho / rho_prev;
-            }
-            cg_update_p(g, cg_p, cg_z, beta, it == 0 ? 1 : 0, st);
+            // ρ (and z·r_prev), p·Ap and the sums of x and r stay on the device (kCgSlot*): α, β and the
+            // gauge means are formed by the kernels that use them, and the host reads one norm per iteration
+            const int itp = (int)(it & 1);
+            if (flex && it > 0) cg_dot2(g, cg_z, cg_r, cg_rprev, cg_partial, cg_out + cg_slot_rho(itp), st);
+            else cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out + cg_slot_rho(itp), st);
+            cg_update_p(g, cg_p, cg_z, cg_out, it == 0 ? 0 : 2 + itp, (flex && it > 0) ? 1 : 0, st);
             if (flex) cg_copy_d(g, cg_r, cg_rprev, st);
             cg_halo_r(g, cg_p, st);
-            cg_apply(g, cg_p, cg_q, cg_partial, cg_out, st);
-            const double alpha = rho / cg_read(1);
-            cg_xr(g, x, cg_p, cg_r, cg_q, alpha, cg_partial, cg_out, st);
-            cg_read(2);
-            const double mx = cg_host[0] / ncell, mr = cg_host[1] / ncell;
-            cg_gauge_norm(g, x, cg_r, mx, mr, cg_partial, cg_out, st);
+            cg_apply(g, cg_p, cg_q, cg_partial, cg_out + kCgSlotPQ, st);
+            cg_xr(g, x, cg_p, cg_r, cg_q, cg_out, itp, cg_partial, cg_out + kCgSlotSumX, st);
+            cg_gauge_norm(g, x, cg_r, cg_out, ncell, cg_partial, cg_out, st);
             res = std::sqrt(cg_read(1));
             it += 1;
-            rho_prev = rho;
             if (flex && !fallback) {
                 stall = res < 0.999 * res_best ? 0 : stall + 1;
                 if (stall >= 3) { fallback = true; ln_fell_back = 1; }

@@ -315,8 +315,18 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_dot_zr(DevGrid<T>
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z (first iteration) or p = z + β p
-template <class T> __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__restrict__ p, const T *__restrict__ z, double beta, int first) {
+// p = z (first iteration, it = 0) or p = z + β p with β = ρ / ρ_prev, or the flexible form
+// β = (ρ − z·r_prev) / ρ_prev, from the iteration's scalars on the device (cg_slot_rho); `it` is the
+// iteration index or any number of its parity that is nonzero after the first iteration
+template <class T>
+__global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__restrict__ p, const T *__restrict__ z,
+                                                     const double *__restrict__ sc, int it, int flex) {
+    const int first = it == 0;
+    double beta = 0.0;
+    if (!first) {
+        const double rho = sc[cg_slot_rho(it)], prev = sc[cg_slot_rho(it - 1)];
+        beta = flex ? (rho - sc[cg_slot_rho(it) + 1]) / prev : rho / prev;
+    }
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         p[o] = first ? (double)z[o] : (double)z[o] + beta * p[o];
     });
@@ -324,9 +334,11 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<
 
 
 // x += α p, r −= α q; partial sums of x and of r (for the zero-mean gauge)
-template <class T> __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *__restrict__ x, const double *__restrict__ p, double *__restrict__ r,
-                                                                  const double *__restrict__ q,
-                                                                  double alpha, double *partial) {
+// (α = ρ / p·q from the iteration's scalars on the device)
+template <class T>
+__global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *__restrict__ x, const double *__restrict__ p, double *__restrict__ r,
+                                               const double *__restrict__ q, const double *__restrict__ sc, int it, double *partial) {
+    const double alpha = sc[cg_slot_rho(it)] / sc[kCgSlotPQ];
     double sx = 0, sr = 0;
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double xn = x[o] + alpha * p[o];
@@ -341,9 +353,12 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, 
     if (threadIdx.x == 0) { partial[blockIdx.x] = a; partial[kRedBlocks + blockIdx.x] = b; }
 }
 
-// zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep
-template <class T> __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, double *__restrict__ r, double mx, double mr,
-                                                                          double *partial) {
+// zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep; the
+// means are the sums of x and r on the device over ncell (sc null: no shift, the loop's first norm)
+template <class T>
+__global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, double *__restrict__ r,
+                                                       const double *__restrict__ sc, double ncell, double *partial) {
+    const double mx = sc ? sc[kCgSlotSumX] / ncell : 0.0, mr = sc ? sc[kCgSlotSumX + 1] / ncell : 0.0;
     double acc = 0;
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double act = active(g, i, j, k);
@@ -585,20 +600,20 @@ template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double 
     hipLaunchKernelGGL(k_cg_dot_zr<T>, red_grid(), red_blk(), 0, st, g, z, r, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
-template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_update_p<T>, red_grid(), red_blk(), 0, st, g, p, z, beta, first);
+template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, const double *sc, int it, int flex, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_update_p<T>, red_grid(), red_blk(), 0, st, g, p, z, sc, it, flex);
 }
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st) {
     cg_stencil<T, STENCIL_APPLY>(g, p, (const double *)nullptr, q, (float *)nullptr, 0.0, partial, out, st);
 }
-template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, double alpha,
+template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, const double *sc, int it,
                               double *partial, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, alpha, partial);
+    hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, sc, it, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
 }
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, double *partial,
                                       double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, mx, mr, partial);
+    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, sc, ncell, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
@@ -635,12 +650,12 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void ln_energy<T>(const DevGrid<T> &, const float *, const float *, double *, double, float *, double *, double *, \
                                hipStream_t);                                                                      \
     template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
-    template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, double, int, hipStream_t);            \
+    template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, const double *, int, int, hipStream_t); \
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \
-    template void cg_xr<T>(const DevGrid<T> &, double *, const double *, double *, const double *, double,       \
-                           double *, double *, hipStream_t);                                                     \
-    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, double, double, double *, double *,  \
-                                   hipStream_t);                                                                 \
+    template void cg_xr<T>(const DevGrid<T> &, double *, const double *, double *, const double *, const double *, \
+                           int, double *, double *, hipStream_t);                                                \
+    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, const double *, double, double *,    \
+                                   double *, hipStream_t);                                                       \
     template void cg_true_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *,   \
                                       double *, hipStream_t);                                                    \
     template void cg_gauged_norm<T>(const DevGrid<T> &, const double *, double, double *, double *, hipStream_t);

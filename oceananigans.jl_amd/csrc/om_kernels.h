@@ -678,6 +678,12 @@ inline CgStencilShape cg_stencil_shape(int Nx, int Ny, int Nz) {
 inline int cg_partials_needed(int Nx, int Ny, int Nz) {
     return std::max(4 * kCgRedBlocks, cg_stencil_shape(Nx, Ny, Nz).nblocks);
 }
+// The iteration's scalars stay on the device, in the solver's reduction-output array: slots 0-3
+// hold results the host reads; ρ of iteration k (and, for flexible CG, z·r_prev after it) sit at
+// cg_slot_rho(k) by the parity of k, so that ρ_prev is still there; then p·Ap and the sums of x
+// and r.  The update kernels form α, β and the gauge means from them.
+constexpr int kCgSlotRho = 8, kCgSlotPQ = 12, kCgSlotSumX = 13, kCgSlots = 16;
+__host__ __device__ inline int cg_slot_rho(int it) { return kCgSlotRho + 2 * (it & 1); }
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st);
 template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st);
 template <class T> void cg_xio(const DevGrid<T> &g, T *f, double *x, T s, int store, hipStream_t st);
@@ -706,11 +712,11 @@ template <class T> void ln_energy(const DevGrid<T> &g, const float *y, const flo
 // ColumnwiseTridiagonalPreconditioner: z = Lz⁻¹ r per column; t is an Nx·Ny·Nz scratch array
 template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st);
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);
-template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, double beta, int first, hipStream_t st);
+template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, const double *sc, int it, int flex, hipStream_t st);
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st);
-template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, double alpha,
+template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, const double *sc, int it,
                               double *partial, double *out, hipStream_t st);
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, double mx, double mr, double *partial,
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, double *partial,
                                       double *out, hipStream_t st);
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
                                          double *out, hipStream_t st);
