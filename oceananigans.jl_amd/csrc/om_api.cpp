@@ -310,6 +310,7 @@ template <class T> struct ModelT : ModelBase {
         // GridFittedBottom: kbot = the number of cells of a column whose centre lies at or below
         // the bottom height, compared in FT (grid_fitted_bottom.jl:118-147)
         g.kbot = nullptr;
+        g.kbot_top = 0;
         if (c.bottom_height) {
             std::vector<int> kb((size_t)g.Nx * g.Ny, 0);
             for (int j = 0; j < g.Ny; ++j)
@@ -321,6 +322,7 @@ template <class T> struct ModelT : ModelBase {
                     kb[(size_t)i + (size_t)g.Nx * j] = n;
                 }
             g.kbot = upload(kb);
+            g.kbot_top = *std::max_element(kb.begin(), kb.end());
         }
         // physics
         ph.advection = c.advection; ph.closure = c.closure; ph.buoyancy = c.buoyancy; ph.coriolis = c.coriolis;

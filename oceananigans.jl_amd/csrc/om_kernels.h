@@ -239,7 +239,7 @@ template <class T> struct IB {
         return i;
     }
     __device__ __forceinline__ bool immersed(int i, int j, int k) const {
-        if (!g.kbot) return false;
+        if (!g.kbot || k >= g.kbot_top) return false;   // above the highest bottom: no column lookup
         return k < g.kbot[wrap(i, g.Nx) + g.Nx * wrap(j, g.Ny)];
     }
     __device__ __forceinline__ bool inactive_cell(int i, int j, int k) const { return immersed(i, j, k) || outside(k); }
