@@ -311,6 +311,8 @@ template <class T> struct ModelT : ModelBase {
         // the bottom height, compared in FT (grid_fitted_bottom.jl:118-147)
         g.kbot = nullptr;
         g.kbot_top = 0;
+        g.kfix = nullptr;
+        g.kfix_max = 0;
         if (c.bottom_height) {
             std::vector<int> kb((size_t)g.Nx * g.Ny, 0);
             for (int j = 0; j < g.Ny; ++j)
@@ -323,6 +325,28 @@ template <class T> struct ModelT : ModelBase {
                 }
             g.kbot = upload(kb);
             g.kbot_top = *std::max_element(kb.begin(), kb.end());
+            // kfix: the window maximum of kbot over ±kImmersedReach columns (periodic wrap), plus the
+            // reach, on columns whose window holds an immersed cell; max filter along x, then along y
+            const int R = kImmersedReach, Nx = g.Nx, Ny = g.Ny;
+            std::vector<int> wx(kb.size()), kf(kb.size());
+            auto wrap = [](int a, int n) { return ((a % n) + n) % n; };
+            for (int j = 0; j < Ny; ++j)
+                for (int i = 0; i < Nx; ++i) {
+                    int m = 0;
+                    for (int d = -R; d <= R; ++d) m = std::max(m, kb[(size_t)wrap(i + d, Nx) + (size_t)Nx * j]);
+                    wx[(size_t)i + (size_t)Nx * j] = m;
+                }
+            int kfm = 0;
+            for (int j = 0; j < Ny; ++j)
+                for (int i = 0; i < Nx; ++i) {
+                    int m = 0;
+                    for (int d = -R; d <= R; ++d) m = std::max(m, wx[(size_t)i + (size_t)Nx * wrap(j + d, Ny)]);
+                    const int f = m > 0 ? std::min(g.Nz, m + R) : 0;
+                    kf[(size_t)i + (size_t)Nx * j] = f;
+                    kfm = std::max(kfm, f);
+                }
+            g.kfix = upload(kf);
+            g.kfix_max = kfm;
         }
         // physics
         ph.advection = c.advection; ph.closure = c.closure; ph.buoyancy = c.buoyancy; ph.coriolis = c.coriolis;
@@ -473,8 +497,12 @@ template <class T> struct ModelT : ModelBase {
     void compute_tendencies() {
         PhaseScope ps(this, OM_PHASE_TENDENCIES);
         const Ctx<T> C = ctx();
-        if (tune_tendency_kernel == 0 && tiled_tendencies_supported(C)) launch_tend_tiled(C, tune_kchunk, st);
-        else launch_tendencies(C, st);
+        if (tune_tendency_kernel == 0 && tiled_tendencies_supported(C)) {
+            launch_tend_tiled(C, tune_kchunk, st);
+            if (g.kbot) launch_tendencies(C, st, true);   // cells within reach of the immersed bottom
+        } else {
+            launch_tendencies(C, st);
+        }
     }
     void update_state() {
         if (g.kbot) {   // mask immersed tracers (update_nonhydrostatic_model_state.jl:30-33)
@@ -491,8 +519,12 @@ template <class T> struct ModelT : ModelBase {
             {
                 PhaseScope ps(this, OM_PHASE_CLOSURE);
                 const Ctx<T> C = ctx();
-                if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) launch_closure_tiled(C, tune_kchunk_closure, st);
-                else launch_closure(C, st);
+                if (tune_tendency_kernel == 0 && tiled_closure_supported(C)) {
+                    launch_closure_tiled(C, tune_kchunk_closure, st);
+                    if (g.kbot) launch_closure(C, st, true);   // cells within reach of the immersed bottom
+                } else {
+                    launch_closure(C, st);
+                }
             }
             add(s.nu);
             if (cfg.closure == OM_CLOSURE_AMD) for (int n = 0; n < nt; ++n) add(s.kap[n]);
@@ -842,9 +874,13 @@ template <class T> struct ModelT : ModelBase {
             }
             return pick;
         };
-        // the re-evaluations leave νₑ, κₑ and Gⁿ as they were (same state, same values)
+        // the re-evaluations leave νₑ, κₑ and Gⁿ as they were (same state, same values); on an
+        // immersed grid the cells below kfix are redone by the generic kernels, the closure's before
+        // the tendency kernel is timed (its last tiled launch then reads the final νₑ, κₑ)
         if (clo) tune_kchunk_closure = fastest([&](int kc) { launch_closure_tiled(C, kc, st); });
+        if (clo && g.kbot) launch_closure(C, st, true);
         tune_kchunk = fastest([&](int kc) { launch_tend_tiled(C, kc, st); });
+        if (g.kbot) launch_tendencies(C, st, true);
         drop_graph();
         return tune_kchunk;
     }

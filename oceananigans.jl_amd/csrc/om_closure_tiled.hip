@@ -250,7 +250,7 @@ template <class T> bool tiled_closure_supported(const Ctx<T> &C) {
     const DevGrid<T> &g = C.g;
     return (C.ph.closure == OM_CLOSURE_AMD || C.ph.closure == OM_CLOSURE_SMAGORINSKY_LILLY) && g.tx == OM_PERIODIC &&
            g.ty == OM_PERIODIC && g.tz != OM_FLAT && g.Hx >= 1 && g.Hy >= 1 && g.Hz >= 1 && C.ph.n_tracers <= 2 &&
-           !g.kbot;   // the immersed-boundary conditions live in the generic kernel
+           (!g.kbot || (g.kfix && g.tz == OM_BOUNDED));   // immersed: the generic kernel redoes the cells below kfix
 }
 
 template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStream_t st) {

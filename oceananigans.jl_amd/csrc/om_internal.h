@@ -33,6 +33,11 @@ template <class T> struct DevGrid {
     int stretched;
     const int *kbot;         // immersed bottom: immersed cells per interior column (x fastest), or nullptr
     int kbot_top;            // the largest entry of kbot: no cell at or above this level is immersed
+    // Levels per column (x fastest) on which an immersed-boundary rule can apply to a tendency or
+    // closure evaluation (an immersed cell within kImmersedReach cells in x, y and z), 0 where none
+    // can; elsewhere the evaluation equals that on the grid without the bottom.  nullptr without one.
+    const int *kfix;
+    int kfix_max;            // the largest entry of kfix
     // per-level AMD ratios, 5 entries for each k = -1 .. Nz at (k + 1)·5: 2Δx/Δᶠz, Δᶠz/2Δx, 2Δy/Δᶠz,
     // Δᶠz/2Δy with Δᶠz = 2Δzᶜ(k), and δ² = 3 / (1/(2Δx)² + 1/(2Δy)² + 1/Δᶠz²), rounded in FT as the
     // per-cell expressions (anisotropic_minimum_dissipation.jl:161-191)

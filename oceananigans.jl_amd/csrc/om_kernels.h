@@ -639,11 +639,14 @@ inline int fft_lines_per_block(int n, int tables, size_t elem) {
     return 0;
 }
 
-template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st);
+// The generic kernels over every cell, or (fixup) only over the cells below kfix of an immersed
+// grid, after the tiled kernels have covered the rest
+constexpr int kImmersedReach = 5;   // farthest offset of an immersed-boundary query from its cell, plus one
+template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st, bool fixup = false);
 // z-marching LDS-tiled tendency kernel (om_tend_tiled.hip) and the configurations it covers
 template <class T> bool tiled_tendencies_supported(const Ctx<T> &C);
 template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream_t st);
-template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st);
+template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st, bool fixup = false);
 // z-marching LDS-tiled eddy viscosity / diffusivity kernel (om_closure_tiled.hip)
 template <class T> bool tiled_closure_supported(const Ctx<T> &C);
 template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStream_t st);

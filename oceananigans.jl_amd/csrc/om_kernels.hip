@@ -15,11 +15,12 @@ namespace om {
 // Tendency kernel: one thread per cell computes Gu, Gv, Gw and every tracer tendency.
 // nonhydrostatic_tendency_kernel_functions.jl:71-302
 // ----------------------------------------------------------------------------------------
-template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C) {
+template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C, int fixup) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
+    if (fixup && k >= C.g.kfix[i + C.g.Nx * j]) return;   // the tiled kernel's value stands
     const bool adv = C.ph.advection != OM_ADV_NONE;
     const bool clo = C.ph.closure != OM_CLOSURE_NONE;
     const int64_t o = C.at(i, j, k);
@@ -136,11 +137,12 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C)
 // Smagorinsky-Lilly: smagorinsky.jl:90-104, lilly_coefficient.jl:129-142)
 // ----------------------------------------------------------------------------------------
 
-template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C) {
+template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C, int fixup) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
+    if (fixup && k >= C.g.kfix[i + C.g.Nx * j]) return;   // the tiled kernel's value stands
     const int64_t o = C.at(i, j, k);
     const T dxu = C.dx_u(i, j, k), dyv = C.dy_v(i, j, k), dzw = C.dz_w(i, j, k);
 
@@ -653,13 +655,17 @@ static inline dim3 cell_grid(int Nx, int Ny, int Nz, dim3 blk) {
     return dim3((Nx + blk.x - 1) / blk.x, (Ny + blk.y - 1) / blk.y, Nz);
 }
 
-template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st) {
+template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st, bool fixup) {
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_tendencies<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C);
+    const int nz = fixup ? std::min(C.g.kfix_max, C.g.Nz) : C.g.Nz;
+    if (nz <= 0) return;
+    hipLaunchKernelGGL(k_tendencies<T>, cell_grid(C.g.Nx, C.g.Ny, nz, blk), blk, 0, st, C, fixup ? 1 : 0);
 }
-template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st) {
+template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st, bool fixup) {
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C);
+    const int nz = fixup ? std::min(C.g.kfix_max, C.g.Nz) : C.g.Nz;
+    if (nz <= 0) return;
+    hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, nz, blk), blk, 0, st, C, fixup ? 1 : 0);
 }
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
     if (sizeof(T) == 4 && C.g.Nz >= 8 && C.g.Nz <= 4 * kHySegLevels) {
@@ -724,8 +730,8 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
 }
 
 #define OM_INSTANTIATE(T)                                                                           \
-    template void launch_tendencies<T>(const Ctx<T> &, hipStream_t);                               \
-    template void launch_closure<T>(const Ctx<T> &, hipStream_t);                                  \
+    template void launch_tendencies<T>(const Ctx<T> &, hipStream_t, bool);                         \
+    template void launch_closure<T>(const Ctx<T> &, hipStream_t, bool);                            \
     template void launch_hydrostatic<T>(const Ctx<T> &, hipStream_t);                              \
     template void launch_rk3<T>(const Ctx<T> &, T, T, T, int, int, hipStream_t);                    \
     template void launch_correct<T>(const Ctx<T> &, T, int, hipStream_t);                               \

@@ -433,7 +433,8 @@ template <class T> bool tiled_tendencies_supported(const Ctx<T> &C) {
     const DevGrid<T> &g = C.g;
     return C.ph.advection == OM_ADV_WENO5 && g.tx == OM_PERIODIC && g.ty == OM_PERIODIC &&
            (g.tz == OM_BOUNDED || g.tz == OM_PERIODIC) && g.Hx >= 3 && g.Hy >= 3 && g.Hz >= 3 &&
-           C.ph.n_tracers <= 2 && g.Nz >= 1 && !g.kbot;   // the immersed-boundary conditions live in the generic kernel
+           C.ph.n_tracers <= 2 && g.Nz >= 1 &&
+           (!g.kbot || (g.kfix && g.tz == OM_BOUNDED));   // immersed: the generic kernel redoes the cells below kfix
 }
 
 template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream_t st) {

@@ -55,6 +55,52 @@ def test_immersed_tendencies_identical_inputs_fp64(case):
     assert rel_err(model.pressures.pHY.interior(), orc.get(om._abi.OM_FIELD_PHY)) <= 1e-12
 
 
+def test_immersed_tiled_kernels_with_generic_fixup_fp64():
+    """On an immersed grid the tiled tendency and closure kernels cover every cell and the generic
+    kernels then redo the cells within reach of the bottom (an immersed cell within 5 cells in x, y
+    and z).  Out of that reach every immersed-boundary predicate is false and both kernels evaluate
+    the same expressions.  On a grid wide enough that most cells are out of reach, the tendencies
+    and closure fields agree with the generic kernels' and with the oracle's to 1e-12, also after
+    the autotuner has re-run the tiled kernels."""
+    grid, kw, init = seamount(N=(64, 48, 24))
+    models = {}
+    for kern in ("tiled", "generic"):
+        m = om.NonhydrostaticModel(grid, **kw)
+        m.set_tuning(tendency_kernel=kern)
+        om.set_(m, enforce_incompressibility=False, **init)
+        models[kern] = m
+    from oracle.oracle import OracleModel
+    cfg, keep = om.build_config(grid, **kw)
+    orc = OracleModel(cfg, grid)
+    names = prognostic_names(kw)
+    for name, val in init.items():
+        orc.set(names.index(name), om.models._evaluate(grid, name, val))
+    orc.set_finalize(False)
+    # cells the generic kernels redo: below the window maximum of the bottom index plus the reach
+    kb = _immersed(grid, orc).sum(axis=2)
+    R = 5
+    win = np.zeros_like(kb)
+    for di in range(-R, R + 1):
+        for dj in range(-R, R + 1):
+            win = np.maximum(win, np.roll(np.roll(kb, di, axis=0), dj, axis=1))
+    kfix = np.where(win > 0, np.minimum(win + R, grid.Nz), 0)
+    frac = float(np.minimum(kfix, grid.Nz).sum()) / kb.size / grid.Nz
+    print("cells redone by the generic kernels:", f"{frac:.2f}", flush=True)
+    assert 0.05 < frac < 0.6
+    t, g = models["tiled"], models["generic"]
+    assert t.get_tuning()["tendency_kernel"] == "tiled" and g.get_tuning()["tendency_kernel"] == "generic"
+    compare_state(t, orc, kw, fields=("state", "G"), label="tiled + fix-up vs oracle")
+    compare_state(g, orc, kw, fields=("state", "G"), label="generic vs oracle")
+    for n in names:
+        assert rel_err(t.Gn[n].interior(), g.Gn[n].interior()) <= 1e-12, n
+    assert rel_err(t.closure_fields.nu.interior(), orc.get(om._abi.OM_FIELD_NU)) <= 1e-12
+    assert rel_err(t.closure_fields.kappa_b.interior(), orc.get(om._abi.OM_FIELD_KAPPA0)) <= 1e-12
+    before = {n: t.Gn[n].interior().copy() for n in names}
+    t.autotune()
+    for n in names:
+        assert np.array_equal(t.Gn[n].interior(), before[n]), n
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_immersed_steps_match_oracle_fp64(case):
     """Three RK3 steps with the CG solver (FFT preconditioner of the underlying grid, reltol 1e-13)."""
