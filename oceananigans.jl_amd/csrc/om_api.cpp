@@ -691,7 +691,9 @@ template <class T> struct ModelT : ModelBase {
         const double res0 = std::sqrt(cg_read(1));
         const double tol = tol_override >= 0 ? tol_override : std::max(reltol * res0, abstol);
         cg_tol_last = tol;
-        cg_gauge_norm(g, x, cg_r, (const double *)nullptr, ncell, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
+        // with the FFT or learned preconditioner the gauge sweeps also write its source r V⁻¹
+        T *const fsrc = (pk == OM_PRECOND_FFT || pk == OM_PRECOND_LEARNED) ? rhs_dense : nullptr;
+        cg_gauge_norm(g, x, cg_r, (const double *)nullptr, ncell, fsrc, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
         double res = std::sqrt(cg_read(1));
         int64_t it = 0;
         // flexible CG for the learned preconditioner (a nonlinear map): β = zₖ·(rₖ − rₖ₋₁) / (zₖ₋₁·rₖ₋₁);
@@ -706,8 +708,7 @@ template <class T> struct ModelT : ModelBase {
         while (it < maxiter && !(res <= tol)) {
             if (pk == OM_PRECOND_DIAGONALLY_DOMINANT) { cg_halo_r(g, cg_r, st); cg_dd(g, cg_r, cg_z, st); }
             else if (pk == OM_PRECOND_FFT || pk == OM_PRECOND_LEARNED) {
-                cg_fft_src(g, cg_r, rhs_dense, st);
-                launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);
+                launch_poisson(P, ctx(), rhs_dense, cg_z, 0, T(1), st);   // source written by the last gauge sweep
                 const bool log_this = logging && it < log_iters;
                 const bool learn_this = pk == OM_PRECOND_LEARNED && !fallback &&
                                         (tune_learned_iterations == 0 || it < tune_learned_iterations);
@@ -728,12 +729,11 @@ template <class T> struct ModelT : ModelBase {
             const int itp = (int)(it & 1);
             if (flex && it > 0) cg_dot2(g, cg_z, cg_r, cg_rprev, cg_partial, cg_out + cg_slot_rho(itp), st);
             else cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out + cg_slot_rho(itp), st);
-            cg_update_p(g, cg_p, cg_z, cg_out, it == 0 ? 0 : 2 + itp, (flex && it > 0) ? 1 : 0, st);
+            cg_update_p(g, cg_p, cg_z, cg_out, it == 0 ? 0 : 2 + itp, (flex && it > 0) ? 1 : 0, st);   // with p's halos
             if (flex) cg_copy_d(g, cg_r, cg_rprev, st);
-            cg_halo_r(g, cg_p, st);
             cg_apply(g, cg_p, cg_q, cg_partial, cg_out + kCgSlotPQ, st);
             cg_xr(g, x, cg_p, cg_r, cg_q, cg_out, itp, cg_partial, cg_out + kCgSlotSumX, st);
-            cg_gauge_norm(g, x, cg_r, cg_out, ncell, cg_partial, cg_out, st);
+            cg_gauge_norm(g, x, cg_r, cg_out, ncell, fsrc, cg_partial, cg_out, st);
             res = std::sqrt(cg_read(1));
             it += 1;
             if (flex && !fallback) {

@@ -87,16 +87,18 @@ template <class T> struct ColumnMask {
             c = xm = xp = ym = yp = INT_MIN;
         }
     }
-    __device__ __forceinline__ bool outside(int k) const { return bounded && (k < 0 || k >= Nz); }
-    __device__ __forceinline__ bool inactive(int kb, int k) const { return k < kb || outside(k); }
+    // (bitwise & and | throughout: the predicates are cheap and evaluating all of them keeps the
+    // sweep free of branches)
+    __device__ __forceinline__ bool outside(int k) const { return bounded & ((k < 0) | (k >= Nz)); }
+    __device__ __forceinline__ bool inactive(int kb, int k) const { return (k < kb) | outside(k); }
     // immersed_inactive_node of a cell centre, of the x or y face between two cells, and of the
     // z face below level k
-    __device__ __forceinline__ bool iin(int kb, int k) const { return inactive(kb, k) && !outside(k); }
+    __device__ __forceinline__ bool iin(int kb, int k) const { return inactive(kb, k) & !outside(k); }
     __device__ __forceinline__ bool iin_face(int ka, int kb, int k) const {
-        return inactive(ka, k) && inactive(kb, k) && !outside(k);
+        return inactive(ka, k) & inactive(kb, k) & !outside(k);
     }
     __device__ __forceinline__ bool iin_zface(int k) const {
-        return inactive(c, k) && inactive(c, k - 1) && !(outside(k) && outside(k - 1));
+        return inactive(c, k) & inactive(c, k - 1) & !(outside(k) & outside(k - 1));
     }
 };
 
@@ -113,15 +115,15 @@ __device__ __forceinline__ double vlap_col(const DevGrid<T> &g, const ColumnMask
     double gyp = (yp - c0) * idy, gym = (c0 - ym) * idy;
     double gzp = (zp - c0) * (1.0 / (double)g.dzf[k + 1]), gzm = (c0 - zm) * (1.0 / (double)g.dzf[k]);
     const bool ic = m.iin(m.c, k);
-    if (ic || m.iin(m.xp, k)) gxp = 0.0;
-    if (ic || m.iin(m.xm, k)) gxm = 0.0;
-    if (ic || m.iin(m.yp, k)) gyp = 0.0;
-    if (ic || m.iin(m.ym, k)) gym = 0.0;
-    if (ic || m.iin(m.c, k + 1)) gzp = 0.0;
-    if (ic || m.iin(m.c, k - 1)) gzm = 0.0;
-    const bool ox = m.iin_face(m.c, m.xm, k) || m.iin_face(m.xp, m.c, k);
-    const bool oy = m.iin_face(m.c, m.ym, k) || m.iin_face(m.yp, m.c, k);
-    const bool oz = m.iin_zface(k) || m.iin_zface(k + 1);
+    gxp = (ic | m.iin(m.xp, k)) ? 0.0 : gxp;
+    gxm = (ic | m.iin(m.xm, k)) ? 0.0 : gxm;
+    gyp = (ic | m.iin(m.yp, k)) ? 0.0 : gyp;
+    gym = (ic | m.iin(m.ym, k)) ? 0.0 : gym;
+    gzp = (ic | m.iin(m.c, k + 1)) ? 0.0 : gzp;
+    gzm = (ic | m.iin(m.c, k - 1)) ? 0.0 : gzm;
+    const bool ox = m.iin_face(m.c, m.xm, k) | m.iin_face(m.xp, m.c, k);
+    const bool oy = m.iin_face(m.c, m.ym, k) | m.iin_face(m.yp, m.c, k);
+    const bool oz = m.iin_zface(k) | m.iin_zface(k + 1);
     const double tx = ox ? 0.0 : Ax * gxp - Ax * gxm;
     const double ty = oy ? 0.0 : Ay * gyp - Ay * gym;
     const double tz = oz ? 0.0 : Az * gzp - Az * gzm;
@@ -157,22 +159,19 @@ __global__ void __launch_bounds__(kCgStTx * kCgStTy)
         const ColumnMask<T> m(g, i, j);
         const int64_t col = idx3(i, j, 0, g.sy, g.sz), sy = g.sy, sz = g.sz;
         const double *s = src + col;
-        double zm = s[(int64_t)(k0 - 1) * sz], c0 = s[(int64_t)k0 * sz];
-        for (int k = k0; k < k1; ++k) {
-            const int64_t o = (int64_t)k * sz;
-            const double zp = s[o + sz];
-            const double v = vlap_col(g, m, k, c0, s[o - 1], s[o + 1], s[o - sy], s[o + sy], zm, zp);
+        auto emit = [&](int k, double c0, double v) {
+            const int64_t o = col + (int64_t)k * sz;
             if (MODE == STENCIL_APPLY) {
-                out[col + o] = v;
+                out[o] = v;
                 acc += c0 * v;
             } else if (MODE == STENCIL_RESIDUAL) {
-                const double rv = b[col + o] - v;
+                const double rv = b[o] - v;
                 const double rs = rv * (double)(T(1) / vol(g, k));
                 acc += rs * rs;
-                out[col + o] = rs * (double)vol(g, k);
+                out[o] = rs * (double)vol(g, k);
             } else if (MODE == STENCIL_TRUE) {
-                const double e = b[col + o] - v;
-                out[col + o] = e;
+                const double e = b[o] - v;
+                out[o] = e;
                 acc += e;
             } else {
                 acc -= c0 * v;
@@ -181,6 +180,32 @@ __global__ void __launch_bounds__(kCgStTx * kCgStTy)
                     gout[(int64_t)i + (int64_t)g.Nx * ((int64_t)j + (int64_t)g.Ny * k)] = (float)(-2.0 * gscale * act * v);
                 }
             }
+        };
+        // the column's values at k − 1 and k carried in registers; each batch of kCgStBatch levels
+        // loads its levels' four horizontal neighbours and the levels above (all loads issued first)
+        double zm = s[(int64_t)(k0 - 1) * sz], c0 = s[(int64_t)k0 * sz];
+        int k = k0;
+        for (; k + kCgStBatch <= k1; k += kCgStBatch) {
+            double xm[kCgStBatch], xp[kCgStBatch], ym[kCgStBatch], yp[kCgStBatch], zc[kCgStBatch];
+#pragma unroll
+            for (int u = 0; u < kCgStBatch; ++u) {
+                const int64_t o = (int64_t)(k + u) * sz;
+                xm[u] = s[o - 1]; xp[u] = s[o + 1]; ym[u] = s[o - sy]; yp[u] = s[o + sy];
+                zc[u] = s[o + sz];   // level k + u + 1 ≤ k1 ≤ Nz: at most the top halo layer
+            }
+#pragma unroll
+            for (int u = 0; u < kCgStBatch; ++u) {
+                const double v = vlap_col(g, m, k + u, c0, xm[u], xp[u], ym[u], yp[u], zm, zc[u]);
+                emit(k + u, c0, v);
+                zm = c0;
+                c0 = zc[u];
+            }
+        }
+        for (; k < k1; ++k) {
+            const int64_t o = (int64_t)k * sz;
+            const double zp = s[o + sz];
+            const double v = vlap_col(g, m, k, c0, s[o - 1], s[o + 1], s[o - sy], s[o + sy], zm, zp);
+            emit(k, c0, v);
             zm = c0;
             c0 = zp;
         }
@@ -292,13 +317,6 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_ct(DevGrid<T> g, 
     }
 }
 
-// FFT preconditioner source: r · V⁻¹ into a dense interior array (conjugate_gradient_poisson_solver.jl:169-211)
-template <class T> __global__ void __launch_bounds__(256) k_cg_fft_src(DevGrid<T> g, const double *__restrict__ r, T *__restrict__ dense) {
-    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
-        dense[t] = (T)(r[o] * (double)(T(1) / vol(g, k)));
-    });
-}
-
 template <class T> __global__ void __launch_bounds__(256) k_cg_copy_r(DevGrid<T> g, const double *__restrict__ r, T *__restrict__ z) {
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         z[o] = (T)r[o];
@@ -317,7 +335,9 @@ template <class T> __global__ void __launch_bounds__(256) k_cg_dot_zr(DevGrid<T>
 
 // p = z (first iteration, it = 0) or p = z + β p with β = ρ / ρ_prev, or the flexible form
 // β = (ρ − z·r_prev) / ρ_prev, from the iteration's scalars on the device (cg_slot_rho); `it` is the
-// iteration index or any number of its parity that is nonzero after the first iteration
+// iteration index or any number of its parity that is nonzero after the first iteration.  The
+// cells next to the domain's faces also write p's one-layer halos (the fill of cg_halo_r: periodic
+// x and y, z by its topology), which the operator sweep that follows reads.
 template <class T>
 __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__restrict__ p, const T *__restrict__ z,
                                                      const double *__restrict__ sc, int it, int flex) {
@@ -327,8 +347,17 @@ __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__res
         const double rho = sc[cg_slot_rho(it)], prev = sc[cg_slot_rho(it - 1)];
         beta = flex ? (rho - sc[cg_slot_rho(it) + 1]) / prev : rho / prev;
     }
+    const int64_t xs = g.Nx, ys = (int64_t)g.Ny * g.sy, zs = (int64_t)g.Nz * g.sz;
+    const bool zper = g.tz == OM_PERIODIC;
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
-        p[o] = first ? (double)z[o] : (double)z[o] + beta * p[o];
+        const double pv = first ? (double)z[o] : (double)z[o] + beta * p[o];
+        p[o] = pv;
+        if (i == 0) p[o + xs] = pv;
+        if (i == g.Nx - 1) p[o - xs] = pv;
+        if (j == 0) p[o + ys] = pv;
+        if (j == g.Ny - 1) p[o - ys] = pv;
+        if (k == 0) p[zper ? o + zs : o - g.sz] = pv;
+        if (k == g.Nz - 1) p[zper ? o - zs : o + g.sz] = pv;
     });
 }
 
@@ -354,10 +383,12 @@ __global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *__restrict_
 }
 
 // zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep; the
-// means are the sums of x and r on the device over ncell (sc null: no shift, the loop's first norm)
+// means are the sums of x and r on the device over ncell (sc null: no shift, the loop's first norm).
+// With `dense`, the FFT preconditioner's source r V⁻¹ of the next iteration is written as well.
 template <class T>
 __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, double *__restrict__ r,
-                                                       const double *__restrict__ sc, double ncell, double *partial) {
+                                                       const double *__restrict__ sc, double ncell, T *__restrict__ dense,
+                                                       double *partial) {
     const double mx = sc ? sc[kCgSlotSumX] / ncell : 0.0, mr = sc ? sc[kCgSlotSumX + 1] / ncell : 0.0;
     double acc = 0;
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
@@ -366,7 +397,9 @@ __global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__r
         const double rg = (r[o] - mr) * act;
         const double rs = rg * (double)(T(1) / vol(g, k));
         acc += rs * rs;
-        r[o] = rs * (double)vol(g, k);
+        const double rn = rs * (double)vol(g, k);
+        r[o] = rn;
+        if (dense) dense[t] = (T)(rn * (double)(T(1) / vol(g, k)));   // r V⁻¹ of the stored r (conjugate_gradient_poisson_solver.jl:169-211)
     });
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -551,9 +584,6 @@ template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st
 template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_dd<T>, red_grid(), red_blk(), 0, st, g, r, z);
 }
-template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_fft_src<T>, red_grid(), red_blk(), 0, st, g, r, dense);
-}
 template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, hipStream_t st) {
     const int64_t ncol = (int64_t)g.Nx * g.Ny;
     hipLaunchKernelGGL(k_cg_ct<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g, r, z, t);
@@ -611,9 +641,9 @@ template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, d
     hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, sc, it, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
 }
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, double *partial,
-                                      double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, sc, ncell, partial);
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, T *dense,
+                                      double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, sc, ncell, dense, partial);
     hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
 }
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
@@ -635,7 +665,6 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
                                  hipStream_t);                                                                   \
     template void cg_halo_r<T>(const DevGrid<T> &, double *, hipStream_t);                                      \
     template void cg_dd<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                               \
-    template void cg_fft_src<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                          \
     template void cg_copy_r<T>(const DevGrid<T> &, const double *, T *, hipStream_t);                           \
     template void cg_ct<T>(const DevGrid<T> &, const double *, T *, T *, hipStream_t);                          \
     template void ln_scales<T>(const DevGrid<T> &, const double *, const T *, double *, double *, hipStream_t);    \
@@ -654,7 +683,7 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \
     template void cg_xr<T>(const DevGrid<T> &, double *, const double *, double *, const double *, const double *, \
                            int, double *, double *, hipStream_t);                                                \
-    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, const double *, double, double *,    \
+    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, const double *, double, T *, double *, \
                                    double *, hipStream_t);                                                       \
     template void cg_true_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *,   \
                                       double *, hipStream_t);                                                    \

@@ -664,6 +664,8 @@ template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStrea
 constexpr int kCgRedBlocks = 2048;
 constexpr int kCgStTx = 64, kCgStTy = 4;
 constexpr int kCgStencilPartials = 4096;
+constexpr int kCgStBatch = 4;      // levels per batch of loads in a sweep
+constexpr int kCgStMinChunk = 32;  // levels per thread where the grid allows it
 struct CgStencilShape {
     int ntx, nty, kchunk, nblocks;
 };
@@ -672,7 +674,7 @@ inline CgStencilShape cg_stencil_shape(int Nx, int Ny, int Nz) {
     s.ntx = (Nx + kCgStTx - 1) / kCgStTx;
     s.nty = (Ny + kCgStTy - 1) / kCgStTy;
     const int tiles = s.ntx * s.nty;
-    int nch = std::max(1, std::min((Nz + 15) / 16, kCgStencilPartials / tiles));
+    int nch = std::max(1, std::min((Nz + kCgStMinChunk - 1) / kCgStMinChunk, kCgStencilPartials / tiles));
     s.kchunk = (Nz + nch - 1) / nch;
     nch = (Nz + s.kchunk - 1) / s.kchunk;
     s.nblocks = tiles * nch;
@@ -695,7 +697,6 @@ template <class T> void cg_residual(const DevGrid<T> &g, const double *x, const 
                                     double *out, hipStream_t st);
 template <class T> void cg_halo_r(const DevGrid<T> &g, double *r, hipStream_t st);
 template <class T> void cg_dd(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
-template <class T> void cg_fft_src(const DevGrid<T> &g, const double *r, T *dense, hipStream_t st);
 template <class T> void cg_copy_r(const DevGrid<T> &g, const double *r, T *z, hipStream_t st);
 // learned correction to the FFT preconditioner and flexible CG (om_cg.hip)
 template <class T> void ln_scales(const DevGrid<T> &g, const double *r, const T *z, double *partial, double *out, hipStream_t st);
@@ -719,8 +720,10 @@ template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, 
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st);
 template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, const double *sc, int it,
                               double *partial, double *out, hipStream_t st);
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, double *partial,
-                                      double *out, hipStream_t st);
+// (dense: also write r V⁻¹ in FP32 cell order, the FFT preconditioner's source; may be null)
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, T *dense,
+                                      double *partial, double *out, hipStream_t st);
+// (the p update also fills p's one-layer halos)
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
                                          double *out, hipStream_t st);
 template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, double mean, double *partial, double *out,
