@@ -573,8 +573,9 @@ template <class T> struct ModelT : ModelBase {
     }
 
     // ---- conjugate-gradient solve of V∇²x = b (conjugate_gradient_solver.jl:177-284) with the
-    // zero-mean gauge; x holds the initial guess, cg_b the right-hand side.  The FP64 scalars are
-    // read back after each reduction, as the reference's host loop does.
+    // zero-mean gauge; x holds the initial guess, cg_b the right-hand side.  ρ, p·Ap and the gauge
+    // sums stay on the device (the kernels that use them form α, β and the means); the host reads
+    // the norm once per iteration for the convergence test.
     double cg_read(int n) {
         HIPCHECK(hipMemcpyAsync(cg_host, cg_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
