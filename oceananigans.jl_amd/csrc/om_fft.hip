@@ -451,8 +451,8 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<
 // Fourier–tridiagonal z solve (Thomas per (kx, ky) column; diagonals from
 // fourier_tridiagonal_poisson_solver.jl:199-229).  One thread per column, the columns ℓ = ky·Mx + kx
 // flattened so that a level of a block's columns is one contiguous run (level k of column ℓ at
-// k·(Mx·Ny) + ℓ); t[k] kept in LDS.  Each sweep reads its levels in batches of kTriBatch, all
-// loads of a batch issued before its arithmetic, so a column costs 2·Nz/kTriBatch memory round
+// k·(Mx·Ny) + ℓ); t[k] kept in LDS.  Each sweep reads its levels in batches of tri_batch (16 FP32, 8 FP64), all
+// loads of a batch issued before its arithmetic, so a column costs 2·Nz/batch memory round
 // trips instead of 2·Nz.  The (0,0) column is singular: its last pivot is a rounding residue and
 // is elided as in the reference; the constant this leaves is removed with the column mean
 // (ϕ -= mean(ϕ), :253).
