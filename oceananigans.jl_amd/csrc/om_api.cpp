@@ -576,6 +576,11 @@ template <class T> struct ModelT : ModelBase {
     // zero-mean gauge; x holds the initial guess, cg_b the right-hand side.  ρ, p·Ap and the gauge
     // sums stay on the device (the kernels that use them form α, β and the means); the host reads
     // the norm once per iteration for the convergence test.
+    double cg_read_slot(int slot) {
+        HIPCHECK(hipMemcpyAsync(cg_host, cg_out + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return cg_host[0];
+    }
     double cg_read(int n) {
         HIPCHECK(hipMemcpyAsync(cg_host, cg_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -694,8 +699,9 @@ template <class T> struct ModelT : ModelBase {
         cg_tol_last = tol;
         // with the FFT or learned preconditioner the gauge sweeps also write its source r V⁻¹
         T *const fsrc = (pk == OM_PRECOND_FFT || pk == OM_PRECOND_LEARNED) ? rhs_dense : nullptr;
-        cg_gauge_norm(g, x, cg_r, (const double *)nullptr, ncell, fsrc, cg_partial, cg_out, st);   // the loop's first VolumeInverseNorm
-        double res = std::sqrt(cg_read(1));
+        // the loop's first VolumeInverseNorm (step 0: no update, so p, q and the scalars are not read)
+        cg_gauge_norm(g, x, cg_p, cg_r, cg_q, cg_out, 0, 0, ncell, fsrc, cg_partial, cg_out + kCgSlotGauge, st);
+        double res = std::sqrt(cg_read_slot(kCgSlotGauge));
         int64_t it = 0;
         // flexible CG for the learned preconditioner (a nonlinear map): β = zₖ·(rₖ − rₖ₋₁) / (zₖ₋₁·rₖ₋₁);
         // three iterations in a row that do not bring ‖V⁻¹r‖ below 0.999 of its best value so far hand
@@ -725,17 +731,18 @@ template <class T> struct ModelT : ModelBase {
                 }
             } else if (pk == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_ct(g, cg_r, cg_z, cg_t, st);
             else cg_copy_r(g, cg_r, cg_z, st);
-            // ρ (and z·r_prev), p·Ap and the sums of x and r stay on the device (kCgSlot*): α, β and the
-            // gauge means are formed by the kernels that use them, and the host reads one norm per iteration
+            // ρ (and z·r_prev), p·Ap with Σp and Σq, and the gauge sweep's sums of x and r stay on the
+            // device (kCgSlot*): α, β and the gauge means are formed by the kernels that use them, and
+            // the host reads one norm per iteration
             const int itp = (int)(it & 1);
             if (flex && it > 0) cg_dot2(g, cg_z, cg_r, cg_rprev, cg_partial, cg_out + cg_slot_rho(itp), st);
             else cg_dot_zr(g, cg_z, cg_r, cg_partial, cg_out + cg_slot_rho(itp), st);
             cg_update_p(g, cg_p, cg_z, cg_out, it == 0 ? 0 : 2 + itp, (flex && it > 0) ? 1 : 0, st);   // with p's halos
             if (flex) cg_copy_d(g, cg_r, cg_rprev, st);
             cg_apply(g, cg_p, cg_q, cg_partial, cg_out + kCgSlotPQ, st);
-            cg_xr(g, x, cg_p, cg_r, cg_q, cg_out, itp, cg_partial, cg_out + kCgSlotSumX, st);
-            cg_gauge_norm(g, x, cg_r, cg_out, ncell, fsrc, cg_partial, cg_out, st);
-            res = std::sqrt(cg_read(1));
+            // x += αp, r −= αq, the zero-mean gauge and the norm in one sweep
+            cg_gauge_norm(g, x, cg_p, cg_r, cg_q, cg_out, itp, 1, ncell, fsrc, cg_partial, cg_out + kCgSlotGauge, st);
+            res = std::sqrt(cg_read_slot(kCgSlotGauge));
             it += 1;
             if (flex && !fallback) {
                 stall = res < 0.999 * res_best ? 0 : stall + 1;

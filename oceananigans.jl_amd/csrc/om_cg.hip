@@ -136,7 +136,7 @@ __device__ __forceinline__ double vlap_col(const DevGrid<T> &g, const ColumnMask
 // registers, so a level costs five loads (the level above and the four horizontal neighbours),
 // and the immersed-boundary rules use the five bottom indices of ColumnMask.  Blocks are dealt to
 // the XCDs in contiguous runs of work items (xcd_work_item), the chunks of one tile consecutive.
-//   STENCIL_APPLY     q = V∇²p; partial sums of p·q
+//   STENCIL_APPLY     q = V∇²p; partial sums of p·q, Σp and Σq
 //   STENCIL_RESIDUAL  r = b − V∇²x, through the initial VolumeInverseNorm; partial sums of (r V⁻¹)²
 //   STENCIL_TRUE      e = b − V∇²x; partial sums of e
 //   STENCIL_ENERGY    partial sums of −d·V∇²d and, when gout is given, gout = −2·s·active·V∇²d
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kCgStTx * kCgStTy)
     const int ty = tile / ntx, tx = tile - ty * ntx;
     const int i = tx * kCgStTx + a, j = ty * kCgStTy + c;
     const int k0 = chunk * kchunk, k1 = min(k0 + kchunk, g.Nz);
-    double acc = 0;
+    double acc = 0, accp = 0, accq = 0;   // (STENCIL_APPLY: p·q, Σp, Σq)
     if (i < g.Nx && j < g.Ny) {
         const ColumnMask<T> m(g, i, j);
         const int64_t col = idx3(i, j, 0, g.sy, g.sz), sy = g.sy, sz = g.sz;
@@ -164,6 +164,8 @@ __global__ void __launch_bounds__(kCgStTx * kCgStTy)
             if (MODE == STENCIL_APPLY) {
                 out[o] = v;
                 acc += c0 * v;
+                accp += c0;
+                accq += v;
             } else if (MODE == STENCIL_RESIDUAL) {
                 const double rv = b[o] - v;
                 const double rs = rv * (double)(T(1) / vol(g, k));
@@ -211,7 +213,16 @@ __global__ void __launch_bounds__(kCgStTx * kCgStTy)
         }
     }
     const double sum = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = sum;
+    if (MODE == STENCIL_APPLY) {
+        const double sp = block_sum(accp), sq = block_sum(accq);
+        if (threadIdx.x == 0) {
+            partial[blockIdx.x] = sum;
+            partial[gridDim.x + blockIdx.x] = sp;
+            partial[2 * gridDim.x + blockIdx.x] = sq;
+        }
+    } else if (threadIdx.x == 0) {
+        partial[blockIdx.x] = sum;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -362,47 +373,45 @@ __global__ void __launch_bounds__(256) k_cg_update_p(DevGrid<T> g, double *__res
 }
 
 
-// x += α p, r −= α q; partial sums of x and of r (for the zero-mean gauge)
-// (α = ρ / p·q from the iteration's scalars on the device)
+// The gauge and norm sweep (conjugate_gradient_poisson_solver.jl:87-96, then the VolumeInverseNorm
+// of :13-34): x ← (x − mean x)·active, r ← (r − mean r)·active, partial sums of (r V⁻¹)², and of the
+// new x and r (the next iteration's means start from them).  With `dense`, the FFT preconditioner's
+// source r V⁻¹ of the next iteration is written as well.
+//   step = false: the solve's first sweep, means 0 (no update);
+//   step = true:  the iteration's update x += α p, r −= α q comes first, α = ρ / p·Ap, and the means
+//                 of the updated x and r come from sums taken before it: Σx + α Σp and Σr − α Σq
+//                 (Σx, Σr from the previous sweep, Σp, Σq from the operator sweep).  This is the
+//                 same mean the reference takes after the update, summed in another order, so
+//                 the update and the gauge share one pass over x, p, r and q.
 template <class T>
-__global__ void __launch_bounds__(256) k_cg_xr(DevGrid<T> g, double *__restrict__ x, const double *__restrict__ p, double *__restrict__ r,
-                                               const double *__restrict__ q, const double *__restrict__ sc, int it, double *partial) {
-    const double alpha = sc[cg_slot_rho(it)] / sc[kCgSlotPQ];
-    double sx = 0, sr = 0;
-    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
-        const double xn = x[o] + alpha * p[o];
-        const double rn = r[o] - alpha * q[o];
-        x[o] = xn;
-        r[o] = rn;
-        sx += xn;
-        sr += rn;
-    });
-    const double a = block_sum(sx);
-    const double b = block_sum(sr);
-    if (threadIdx.x == 0) { partial[blockIdx.x] = a; partial[kRedBlocks + blockIdx.x] = b; }
-}
-
-// zero-mean gauge (conjugate_gradient_poisson_solver.jl:87-96), then the VolumeInverseNorm sweep; the
-// means are the sums of x and r on the device over ncell (sc null: no shift, the loop's first norm).
-// With `dense`, the FFT preconditioner's source r V⁻¹ of the next iteration is written as well.
-template <class T>
-__global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, double *__restrict__ r,
-                                                       const double *__restrict__ sc, double ncell, T *__restrict__ dense,
-                                                       double *partial) {
-    const double mx = sc ? sc[kCgSlotSumX] / ncell : 0.0, mr = sc ? sc[kCgSlotSumX + 1] / ncell : 0.0;
-    double acc = 0;
+__global__ void __launch_bounds__(256) k_cg_gauge_norm(DevGrid<T> g, double *__restrict__ x, const double *__restrict__ p,
+                                                       double *__restrict__ r, const double *__restrict__ q,
+                                                       const double *__restrict__ sc, int it, int step, double ncell,
+                                                       T *__restrict__ dense, double *partial) {
+    double alpha = 0.0, mx = 0.0, mr = 0.0;
+    if (step) {
+        alpha = sc[cg_slot_rho(it)] / sc[kCgSlotPQ];
+        mx = (sc[kCgSlotGauge + 1] + alpha * sc[kCgSlotPQ + 1]) / ncell;
+        mr = (sc[kCgSlotGauge + 2] - alpha * sc[kCgSlotPQ + 2]) / ncell;
+    }
+    double acc = 0, sx = 0, sr = 0;
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
         const double act = active(g, i, j, k);
-        x[o] = (x[o] - mx) * act;
-        const double rg = (r[o] - mr) * act;
+        const double xu = step ? x[o] + alpha * p[o] : x[o];
+        const double ru = step ? r[o] - alpha * q[o] : r[o];
+        const double xg = (xu - mx) * act;
+        x[o] = xg;
+        const double rg = (ru - mr) * act;
         const double rs = rg * (double)(T(1) / vol(g, k));
         acc += rs * rs;
         const double rn = rs * (double)vol(g, k);
         r[o] = rn;
+        sx += xg;
+        sr += rn;
         if (dense) dense[t] = (T)(rn * (double)(T(1) / vol(g, k)));   // r V⁻¹ of the stored r (conjugate_gradient_poisson_solver.jl:169-211)
     });
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    const double s0 = block_sum(acc), s1 = block_sum(sx), s2 = block_sum(sr);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = s0; partial[kRedBlocks + blockIdx.x] = s1; partial[2 * kRedBlocks + blockIdx.x] = s2; }
 }
 
 
@@ -556,7 +565,8 @@ static void cg_stencil(const DevGrid<T> &g, const double *src, const double *b, 
     const CgStencilShape s = cg_stencil_shape(g.Nx, g.Ny, g.Nz);
     hipLaunchKernelGGL((k_cg_stencil<T, MODE>), dim3(s.nblocks), dim3(kCgStTx * kCgStTy), 0, st, g, src, b, out, gout, gscale,
                        partial, s.kchunk, s.ntx);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, s.nblocks, outv);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, MODE == STENCIL_APPLY ? 3 : 1, s.nblocks,
+                       outv);
 }
 
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st) {
@@ -636,15 +646,11 @@ template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, 
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st) {
     cg_stencil<T, STENCIL_APPLY>(g, p, (const double *)nullptr, q, (float *)nullptr, 0.0, partial, out, st);
 }
-template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, const double *sc, int it,
-                              double *partial, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_xr<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, sc, it, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 2, kRedBlocks, out);
-}
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, T *dense,
-                                      double *partial, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, r, sc, ncell, dense, partial);
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 1, kRedBlocks, out);
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q,
+                                      const double *sc, int it, int step, double ncell, T *dense, double *partial, double *out,
+                                      hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_gauge_norm<T>, red_grid(), red_blk(), 0, st, g, x, p, r, q, sc, it, step, ncell, dense, partial);
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), red_blk(), 0, st, (const double *)partial, 3, kRedBlocks, out);
 }
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
                                          double *out, hipStream_t st) {
@@ -681,10 +687,8 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void cg_dot_zr<T>(const DevGrid<T> &, const T *, const double *, double *, double *, hipStream_t);  \
     template void cg_update_p<T>(const DevGrid<T> &, double *, const T *, const double *, int, int, hipStream_t); \
     template void cg_apply<T>(const DevGrid<T> &, const double *, double *, double *, double *, hipStream_t);   \
-    template void cg_xr<T>(const DevGrid<T> &, double *, const double *, double *, const double *, const double *, \
-                           int, double *, double *, hipStream_t);                                                \
-    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, double *, const double *, double, T *, double *, \
-                                   double *, hipStream_t);                                                       \
+    template void cg_gauge_norm<T>(const DevGrid<T> &, double *, const double *, double *, const double *,      \
+                                   const double *, int, int, double, T *, double *, double *, hipStream_t);     \
     template void cg_true_residual<T>(const DevGrid<T> &, const double *, const double *, double *, double *,   \
                                       double *, hipStream_t);                                                    \
     template void cg_gauged_norm<T>(const DevGrid<T> &, const double *, double, double *, double *, hipStream_t);

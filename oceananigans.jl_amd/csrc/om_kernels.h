@@ -681,13 +681,14 @@ inline CgStencilShape cg_stencil_shape(int Nx, int Ny, int Nz) {
     return s;
 }
 inline int cg_partials_needed(int Nx, int Ny, int Nz) {
-    return std::max(4 * kCgRedBlocks, cg_stencil_shape(Nx, Ny, Nz).nblocks);
+    return std::max(4 * kCgRedBlocks, 3 * cg_stencil_shape(Nx, Ny, Nz).nblocks);
 }
 // The iteration's scalars stay on the device, in the solver's reduction-output array: slots 0-3
 // hold results the host reads; ρ of iteration k (and, for flexible CG, z·r_prev after it) sit at
-// cg_slot_rho(k) by the parity of k, so that ρ_prev is still there; then p·Ap and the sums of x
-// and r.  The update kernels form α, β and the gauge means from them.
-constexpr int kCgSlotRho = 8, kCgSlotPQ = 12, kCgSlotSumX = 13, kCgSlots = 16;
+// cg_slot_rho(k) by the parity of k, so that ρ_prev is still there; then p·Ap with the sums of p
+// and A·p, and the gauge sweep's ‖V⁻¹r‖² with the sums of x and r.  The update kernels form α, β and
+// the gauge means from them.
+constexpr int kCgSlotRho = 8, kCgSlotPQ = 12, kCgSlotGauge = 16, kCgSlots = 24;   // PQ: p·Ap, Σp, Σq; Gauge: ‖V⁻¹r‖², Σx, Σr
 __host__ __device__ inline int cg_slot_rho(int it) { return kCgSlotRho + 2 * (it & 1); }
 template <class T> void cg_scale(const DevGrid<T> &g, T *f, T s, int mode, hipStream_t st);
 template <class T> void cg_gather(const DevGrid<T> &g, const double *x, T *dense, hipStream_t st);
@@ -718,11 +719,12 @@ template <class T> void cg_ct(const DevGrid<T> &g, const double *r, T *z, T *t, 
 template <class T> void cg_dot_zr(const DevGrid<T> &g, const T *z, const double *r, double *partial, double *out, hipStream_t st);
 template <class T> void cg_update_p(const DevGrid<T> &g, double *p, const T *z, const double *sc, int it, int flex, hipStream_t st);
 template <class T> void cg_apply(const DevGrid<T> &g, const double *p, double *q, double *partial, double *out, hipStream_t st);
-template <class T> void cg_xr(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q, const double *sc, int it,
-                              double *partial, double *out, hipStream_t st);
-// (dense: also write r V⁻¹ in FP32 cell order, the FFT preconditioner's source; may be null)
-template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, double *r, const double *sc, double ncell, T *dense,
-                                      double *partial, double *out, hipStream_t st);
+// The gauge and norm sweep (step = 0: the solve's first; step = 1: with the iteration's update of
+// x and r folded in); writes ‖V⁻¹r‖², Σx, Σr to out.  dense: also write r V⁻¹ in FP32 cell order,
+// the FFT preconditioner's source (may be null).
+template <class T> void cg_gauge_norm(const DevGrid<T> &g, double *x, const double *p, double *r, const double *q,
+                                      const double *sc, int it, int step, double ncell, T *dense, double *partial, double *out,
+                                      hipStream_t st);
 // (the p update also fills p's one-layer halos)
 template <class T> void cg_true_residual(const DevGrid<T> &g, const double *x, const double *b, double *e, double *partial,
                                          double *out, hipStream_t st);
