@@ -494,19 +494,21 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A, C2<T> *__restrict_
     C2<T> *S = Sg + col;
     auto at = [&](int k) -> int64_t { return (int64_t)k * ncol; };
     // forward sweep: level 0, then full batches of KB levels (loads first), then the rest
+    // (ft_div: IEEE division in FP64, as the reference's Thomas sweep; the hardware reciprocal in
+    // FP32, which shortens the serial chain through β at each level)
     T beta = diag(0);
     C2<T> phi;
     {
         const C2<T> f = S[0];
-        phi = cmk<T>(f.x / beta, f.y / beta);
+        phi = cmk<T>(ft_div<T>(f.x, beta), ft_div<T>(f.y, beta));
         S[0] = phi;
     }
     auto fwd = [&](int k, const C2<T> &f) {
         const T c = off(k - 1), a = off(k - 1);
-        const T t = c / beta;
+        const T t = ft_div<T>(c, beta);
         tt[k * bw + threadIdx.x] = t;
         beta = diag(k) - a * t;
-        if (fabs(beta) > A.eps10) phi = cmk<T>((f.x - a * phi.x) / beta, (f.y - a * phi.y) / beta);
+        if (fabs(beta) > A.eps10) phi = cmk<T>(ft_div<T>(f.x - a * phi.x, beta), ft_div<T>(f.y - a * phi.y, beta));
         else phi = cmk<T>(T(0), T(0));
         S[at(k)] = phi;
     };
@@ -519,12 +521,15 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A, C2<T> *__restrict_
         for (int u = 0; u < KB; ++u) fwd(k + u, f[u]);
     }
     for (; k < Nz; ++k) fwd(k, S[at(k)]);
-    // backward substitution from level Nz − 2 down, in the same batches
+    // backward substitution from level Nz − 2 down, in the same batches; the column's sum is
+    // accumulated on the way (used by the (0,0) column only, top level first)
+    C2<T> colsum = phi;
     auto bwd = [&](int kk, const C2<T> &p) {
         const T t = tt[(kk + 1) * bw + threadIdx.x];
         const C2<T> q = cmk<T>(p.x - t * phi.x, p.y - t * phi.y);
         S[at(kk)] = q;
         phi = q;
+        colsum = cadd<T>(colsum, q);
     };
     k = Nz - 2;
     for (; k - KB + 1 >= 0; k -= KB) {
@@ -536,11 +541,21 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A, C2<T> *__restrict_
     }
     for (; k >= 0; --k) bwd(k, S[at(k)]);
     if (col == 0) {
-        // remove the mean of the (0,0) column: equals removing the volume mean of ϕ
-        C2<T> s = cmk<T>(T(0), T(0));
-        for (int q = 0; q < Nz; ++q) s = cadd<T>(s, S[at(q)]);
-        s = cmk<T>(s.x / T(Nz), s.y / T(Nz));
-        for (int q = 0; q < Nz; ++q) S[at(q)] = csub<T>(S[at(q)], s);
+        // remove the mean of the (0,0) column: equals removing the volume mean of ϕ (the reference
+        // subtracts the mean of the real field; the sum here runs down the column as the backward
+        // sweep produced it).  The column is one thread's work at the end of the kernel, so its
+        // read-modify-write pass goes in batches too (one round trip per level would set the
+        // kernel's duration).
+        const C2<T> s = cmk<T>(colsum.x / T(Nz), colsum.y / T(Nz));
+        int q = 0;
+        for (; q + KB <= Nz; q += KB) {
+            C2<T> v[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) v[u] = S[at(q + u)];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) S[at(q + u)] = csub<T>(v[u], s);
+        }
+        for (; q < Nz; ++q) S[at(q)] = csub<T>(S[at(q)], s);
     }
 }
 
@@ -651,9 +666,13 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     if (z_active) {
         if (P.tridiagonal) {
             TriArgs<T> A{g, P.S, Mx, P.lx, P.ly, P.eps10, T(1)};
-            int bw = 64;   // columns per block, limited by the LDS holding t[k] (≤ 64 KiB)
-            while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
+            // columns per block (one wave, partly filled when narrow): each column is a serial sweep,
+            // so the launch wants waves on every SIMD (≥ 2048 of them) more than full waves; 16
+            // columns still read whole 128-byte lines of the spectrum.  Limited by the LDS holding t[k].
             const int64_t ncol = (int64_t)Mx * Ny;
+            int bw = 64;
+            while (bw > 16 && ncol / bw < 2048) bw >>= 1;
+            while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
             hipLaunchKernelGGL(k_tridiag<T>, dim3((unsigned)((ncol + bw - 1) / bw)), dim3(bw), (size_t)Nz * bw * sizeof(T), st, A,
                                P.S, g.dzc, g.idzf);
         } else {
