@@ -169,7 +169,7 @@ template <class T> struct XFwdArgs {
     const C2<T> *post;    // exp(-2πi k / Nx), k = 0 .. Nx/2
     int scale_dzc;
     LineGeom L;
-    FastDiv fNx, fNy, fMx;
+    FastDiv fNy, fMx;
 };
 
 template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const T *src, int i, int j, int k) {
@@ -202,23 +202,39 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArg
     const int nrows = Ny * C.g.Nz;
     load_table<T>(twl, A.tw, M);
     T *ap = reinterpret_cast<T *>(a);
-    staged_copy<4>(R * Nx,
-        [&](int t) {
-            const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
-            const int row = row0 + r;
-            T x = T(0);
-            if (row < nrows) {
-                const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
-                x = source_at<T>(C, A.src, i, j, k);
-                if (A.scale_dzc) x = C.g.dzc[k] * x;
+    {
+        // blockDim.x / R threads per row: a thread's row (j, k), its metric factors and addresses are
+        // fixed, and its elements i = l, l + tpr, … differ by constant offsets; the loads of four
+        // elements are issued before their LDS stores.  Rows past the last stage zeros.
+        const int lgt = __builtin_ctz((unsigned)blockDim.x) - A.L.lgR;   // log2 of threads per row
+        const int tpr = 1 << lgt;
+        const int r = threadIdx.x >> lgt, l = threadIdx.x & (tpr - 1);
+        const int row = row0 + r;
+        const bool live = row < nrows;
+        const int k = live ? (int)fdiv((uint32_t)row, A.fNy) : 0, j = live ? row - k * Ny : 0;
+        const T zs = A.scale_dzc && live ? C.g.dzc[k] : T(1);
+        constexpr int B = 4;
+        for (int i0 = l; i0 < Nx; i0 += B * tpr) {
+            T x[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int i = i0 + u * tpr;
+                x[u] = T(0);
+                if (live && i < Nx) {
+                    x[u] = source_at<T>(C, A.src, i, j, k);
+                    if (A.scale_dzc) x[u] = zs * x[u];
+                }
             }
-            return x;
-        },
-        [&](int t, T x) {
-            const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
-            if (half) ap[2 * (r * ld + (i >> 1)) + (i & 1)] = x;
-            else a[r * ld + i] = cmk<T>(x, T(0));
-        });
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int i = i0 + u * tpr;
+                if (i < Nx) {
+                    if (half) ap[2 * (r * ld + (i >> 1)) + (i & 1)] = x[u];
+                    else a[r * ld + i] = cmk<T>(x[u], T(0));
+                }
+            }
+        }
+    }
     __syncthreads();
     const C2<T> *Z = lds_fft<T>(a, b, R, M, ld, twl, false);
     for (int t = threadIdx.x; t < R * Mx; t += blockDim.x) {
@@ -248,7 +264,7 @@ template <class T> struct XBwdArgs {
     T scale;           // 1 / (Nx · Ny · Nz-normalisations)
     T divisor;         // the written value is (x · scale) / divisor
     LineGeom L;
-    FastDiv fNx, fNy, fM;
+    FastDiv fNy;
 };
 
 template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArgs<T> A) {
@@ -264,43 +280,54 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArg
     const int row0 = blockIdx.x * R;
     const int nrows = Ny * A.g.Nz;
     load_table<T>(twl, A.tw, M);
-    staged_copy<4>(R * M,
-        [&](int t) {
-            const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
-            const int row = row0 + r;
-            C2<T> Zk = cmk<T>(T(0), T(0));
-            if (row < nrows) {
-                if (!half) {
-                    Zk = kk < Mx ? A.S[(int64_t)row * Mx + kk] : cconj<T>(A.S[(int64_t)row * Mx + (Nx - kk)]);
-                } else {
-                    const C2<T> Xk = A.S[(int64_t)row * Mx + kk];
-                    const C2<T> Xc = cconj<T>(A.S[(int64_t)row * Mx + (M - kk)]);
-                    const C2<T> E = cmk<T>(T(0.5) * (Xk.x + Xc.x), T(0.5) * (Xk.y + Xc.y));
-                    const C2<T> d = csub<T>(Xk, Xc);
-                    // O = (Xk - conj(X_{M-k})) · e^{+2πik/N} / 2 ;  Z = E + i O
-                    const C2<T> O = cmul<T>(cmk<T>(T(0.5) * d.x, T(0.5) * d.y), cconj<T>(A.post[kk]));
-                    Zk = cmk<T>(E.x - O.y, E.y + O.x);
+    // blockDim.x / R threads per row, as in k_xfwd: a thread's row and its addresses are fixed
+    const int lgt = __builtin_ctz((unsigned)blockDim.x) - A.L.lgR;   // log2 of threads per row
+    const int tpr = 1 << lgt;
+    const int r = threadIdx.x >> lgt, l = threadIdx.x & (tpr - 1);
+    const int row = row0 + r;
+    const bool live = row < nrows;
+    {
+        const C2<T> *Srow = A.S + (int64_t)(live ? row : 0) * Mx;
+        constexpr int B = 4;
+        for (int k0 = l; k0 < M; k0 += B * tpr) {
+            C2<T> v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int kk = k0 + u * tpr;
+                C2<T> Zk = cmk<T>(T(0), T(0));
+                if (live && kk < M) {
+                    if (!half) {
+                        Zk = kk < Mx ? Srow[kk] : cconj<T>(Srow[Nx - kk]);
+                    } else {
+                        const C2<T> Xk = Srow[kk];
+                        const C2<T> Xc = cconj<T>(Srow[M - kk]);
+                        const C2<T> E = cmk<T>(T(0.5) * (Xk.x + Xc.x), T(0.5) * (Xk.y + Xc.y));
+                        const C2<T> d = csub<T>(Xk, Xc);
+                        // O = (Xk - conj(X_{M-k})) · e^{+2πik/N} / 2 ;  Z = E + i O
+                        const C2<T> O = cmul<T>(cmk<T>(T(0.5) * d.x, T(0.5) * d.y), cconj<T>(A.post[kk]));
+                        Zk = cmk<T>(E.x - O.y, E.y + O.x);
+                    }
                 }
+                v[u] = Zk;
             }
-            return Zk;
-        },
-        [&](int t, C2<T> v) {
-            const int r = (int)fdiv((uint32_t)t, A.fM), kk = t - r * M;
-            a[r * ld + kk] = v;
-        });
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int kk = k0 + u * tpr;
+                if (kk < M) a[r * ld + kk] = v[u];
+            }
+        }
+    }
     __syncthreads();
     const C2<T> *z = lds_fft<T>(a, b, R, M, ld, twl, true);
-    for (int t = threadIdx.x; t < R * Nx; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, A.fNx), i = t - r * Nx;
-        const int row = row0 + r;
-        if (row >= nrows) continue;
+    if (live) {
         const int k = (int)fdiv((uint32_t)row, A.fNy), j = row - k * Ny;
-        T x;
-        if (half) { const C2<T> zz = z[r * ld + (i >> 1)]; x = (i & 1) ? zz.y : zz.x; }
-        else x = z[r * ld + i].x;
-        x = (x * A.scale) / A.divisor;
-        if (A.dense) A.out[(int64_t)i + (int64_t)Nx * ((int64_t)j + (int64_t)Ny * k)] = x;
-        else A.out[idx3(i, j, k, A.g.sy, A.g.sz)] = x;
+        T *dst = A.dense ? A.out + (int64_t)Nx * ((int64_t)j + (int64_t)Ny * k) : A.out + idx3(0, j, k, A.g.sy, A.g.sz);
+        for (int i = l; i < Nx; i += tpr) {
+            T x;
+            if (half) { const C2<T> zz = z[r * ld + (i >> 1)]; x = (i & 1) ? zz.y : zz.x; }
+            else x = z[r * ld + i].x;
+            dst[i] = ft_div<T>(x * A.scale, A.divisor);   // IEEE in FP64; FP32 by the reciprocal of the divisor
+        }
     }
 }
 
@@ -650,8 +677,7 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     size_t lds = 0;
     {
         const LineGeom L = line_geom(M, M, e, lds);
-        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L,
-                      make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx)};
+        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L, make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx)};
         hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
     const bool z_active = g.tz != OM_FLAT;
@@ -686,8 +712,7 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     }
     {
         const LineGeom L = line_geom(M, M, e, lds);
-        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L,
-                      make_fastdiv((uint32_t)Nx), make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)M)};
+        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L, make_fastdiv((uint32_t)Ny)};
         hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
 }
