@@ -137,7 +137,8 @@ def run_cg_seamount(args):
         m.time_step(20.0, 1)
         ph = m.phase_times()
         it, r, true_r = m.last_solve_info()
-        res[name] = {"ms_per_solve": ph["poisson"][0] / 3, "iterations_last_solve": it, "recomputed_residual": true_r}
+        res[name] = {"ms_per_solve": ph["poisson"][0] / 3, "iterations_last_solve": it, "recomputed_residual": true_r,
+                     "step_phases_ms": {k: round(v[0], 4) for k, v in ph.items()}}
     res["learned"]["held_out_energy_ratio"] = learned.learned_evaluate()
     res["learned"]["network_iterations_last_solve"], res["learned"]["fell_back"] = learned.learned_last_solve()
     fft.close()
