@@ -313,6 +313,8 @@ template <class T> struct ModelT : ModelBase {
         g.kbot_top = 0;
         g.kfix = nullptr;
         g.kfix_max = 0;
+        g.fix_list = nullptr;
+        g.fix_count = 0;
         if (c.bottom_height) {
             std::vector<int> kb((size_t)g.Nx * g.Ny, 0);
             for (int j = 0; j < g.Ny; ++j)
@@ -347,6 +349,22 @@ template <class T> struct ModelT : ModelBase {
                 }
             g.kfix = upload(kf);
             g.kfix_max = kfm;
+            // the fix-up launches' blocks: every (block, level) holding a cell below kfix, level-major
+            const int nbx = (Nx + kGenericBlockX - 1) / kGenericBlockX, nby = (Ny + kGenericBlockY - 1) / kGenericBlockY;
+            if (nbx > 0xffff || nby > 0x7fff) throw ArgError{"grid too wide for the immersed fix-up list"};
+            std::vector<int> bmax((size_t)nbx * nby, 0);
+            for (int j = 0; j < Ny; ++j)
+                for (int i = 0; i < Nx; ++i) {
+                    int &m = bmax[(size_t)(i / kGenericBlockX) + (size_t)nbx * (j / kGenericBlockY)];
+                    m = std::max(m, kf[(size_t)i + (size_t)Nx * j]);
+                }
+            std::vector<int2> fl;
+            for (int k = 0; k < kfm; ++k)
+                for (int by = 0; by < nby; ++by)
+                    for (int bx = 0; bx < nbx; ++bx)
+                        if (bmax[(size_t)bx + (size_t)nbx * by] > k) fl.push_back(make_int2(bx | (by << 16), k));
+            g.fix_count = (int)fl.size();
+            if (!fl.empty()) g.fix_list = upload(fl);
         }
         // physics
         ph.advection = c.advection; ph.closure = c.closure; ph.buoyancy = c.buoyancy; ph.coriolis = c.coriolis;

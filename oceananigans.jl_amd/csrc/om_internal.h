@@ -38,6 +38,10 @@ template <class T> struct DevGrid {
     // can; elsewhere the evaluation equals that on the grid without the bottom.  nullptr without one.
     const int *kfix;
     int kfix_max;            // the largest entry of kfix
+    // the (64 × 4-column block, level) pairs of the generic kernels' launch that hold a cell below
+    // kfix: x = block x | block y << 16, y = level; the fix-up launches visit only these
+    const int2 *fix_list;
+    int fix_count;
     // per-level AMD ratios, 5 entries for each k = -1 .. Nz at (k + 1)·5: 2Δx/Δᶠz, Δᶠz/2Δx, 2Δy/Δᶠz,
     // Δᶠz/2Δy with Δᶠz = 2Δzᶜ(k), and δ² = 3 / (1/(2Δx)² + 1/(2Δy)² + 1/Δᶠz²), rounded in FT as the
     // per-cell expressions (anisotropic_minimum_dissipation.jl:161-191)

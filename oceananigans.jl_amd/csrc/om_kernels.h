@@ -642,6 +642,7 @@ inline int fft_lines_per_block(int n, int tables, size_t elem) {
 // The generic kernels over every cell, or (fixup) only over the cells below kfix of an immersed
 // grid, after the tiled kernels have covered the rest
 constexpr int kImmersedReach = 5;   // farthest offset of an immersed-boundary query from its cell, plus one
+constexpr int kGenericBlockX = 64, kGenericBlockY = 4;   // block shape of k_tendencies / k_closure
 template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st, bool fixup = false);
 // z-marching LDS-tiled tendency kernel (om_tend_tiled.hip) and the configurations it covers
 template <class T> bool tiled_tendencies_supported(const Ctx<T> &C);

@@ -15,10 +15,23 @@ namespace om {
 // Tendency kernel: one thread per cell computes Gu, Gv, Gw and every tracer tendency.
 // nonhydrostatic_tendency_kernel_functions.jl:71-302
 // ----------------------------------------------------------------------------------------
+// (fixup: the block's columns and level come from the fix-up list, see DevGrid::fix_list)
+template <class T> __device__ __forceinline__ void cell_block(const DevGrid<T> &g, int fixup, int &i, int &j, int &k) {
+    int bx = blockIdx.x, by = blockIdx.y;
+    k = blockIdx.z;
+    if (fixup) {
+        const int2 e = g.fix_list[blockIdx.x];
+        bx = e.x & 0xffff;
+        by = e.x >> 16;
+        k = e.y;
+    }
+    i = bx * blockDim.x + threadIdx.x;
+    j = by * blockDim.y + threadIdx.y;
+}
+
 template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C, int fixup) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    cell_block(C.g, fixup, i, j, k);
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     if (fixup && k >= C.g.kfix[i + C.g.Nx * j]) return;   // the tiled kernel's value stands
     const bool adv = C.ph.advection != OM_ADV_NONE;
@@ -138,9 +151,8 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C,
 // ----------------------------------------------------------------------------------------
 
 template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C, int fixup) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    cell_block(C.g, fixup, i, j, k);
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     if (fixup && k >= C.g.kfix[i + C.g.Nx * j]) return;   // the tiled kernel's value stands
     const int64_t o = C.at(i, j, k);
@@ -655,17 +667,23 @@ static inline dim3 cell_grid(int Nx, int Ny, int Nz, dim3 blk) {
     return dim3((Nx + blk.x - 1) / blk.x, (Ny + blk.y - 1) / blk.y, Nz);
 }
 
+// the generic kernels' block shape; the fix-up list is built for it
+static_assert(kGenericBlockX == 64 && kGenericBlockY == 4, "fix-up list geometry");
 template <class T> void launch_tendencies(const Ctx<T> &C, hipStream_t st, bool fixup) {
-    dim3 blk(64, 4, 1);
-    const int nz = fixup ? std::min(C.g.kfix_max, C.g.Nz) : C.g.Nz;
-    if (nz <= 0) return;
-    hipLaunchKernelGGL(k_tendencies<T>, cell_grid(C.g.Nx, C.g.Ny, nz, blk), blk, 0, st, C, fixup ? 1 : 0);
+    dim3 blk(kGenericBlockX, kGenericBlockY, 1);
+    if (fixup) {
+        if (C.g.fix_count > 0) hipLaunchKernelGGL(k_tendencies<T>, dim3(C.g.fix_count), blk, 0, st, C, 1);
+        return;
+    }
+    hipLaunchKernelGGL(k_tendencies<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, 0);
 }
 template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st, bool fixup) {
-    dim3 blk(64, 4, 1);
-    const int nz = fixup ? std::min(C.g.kfix_max, C.g.Nz) : C.g.Nz;
-    if (nz <= 0) return;
-    hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, nz, blk), blk, 0, st, C, fixup ? 1 : 0);
+    dim3 blk(kGenericBlockX, kGenericBlockY, 1);
+    if (fixup) {
+        if (C.g.fix_count > 0) hipLaunchKernelGGL(k_closure<T>, dim3(C.g.fix_count), blk, 0, st, C, 1);
+        return;
+    }
+    hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, 0);
 }
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
     if (sizeof(T) == 4 && C.g.Nz >= 8 && C.g.Nz <= 4 * kHySegLevels) {
