@@ -294,6 +294,11 @@ template <class T> struct ModelT : ModelBase {
         g.dzf = upload(hg.dzf) + g.Hz;
         g.idzc = upload(idzc) + g.Hz;
         g.idzf = upload(idzf) + g.Hz;
+        {
+            std::vector<double> idzf64(hg.dzf.size());
+            for (size_t q = 0; q < idzf64.size(); ++q) idzf64[q] = 1.0 / (double)hg.dzf[q];
+            g.idzf64 = upload(idzf64) + g.Hz;
+        }
         g.stretched = hg.stretched;
         g.amdr = nullptr;
         if (g.tz != OM_FLAT) {

@@ -113,7 +113,8 @@ __device__ __forceinline__ double vlap_col(const DevGrid<T> &g, const ColumnMask
     const double Az = (double)g.dx * (double)g.dy;
     double gxp = (xp - c0) * idx, gxm = (c0 - xm) * idx;
     double gyp = (yp - c0) * idy, gym = (c0 - ym) * idy;
-    double gzp = (zp - c0) * (1.0 / (double)g.dzf[k + 1]), gzm = (c0 - zm) * (1.0 / (double)g.dzf[k]);
+    // 1 / (double)Δzᶠ from the grid's FP64 table: the same correctly rounded quotient as dividing here
+    double gzp = (zp - c0) * g.idzf64[k + 1], gzm = (c0 - zm) * g.idzf64[k];
     const bool ic = m.iin(m.c, k);
     gxp = (ic | m.iin(m.xp, k)) ? 0.0 : gxp;
     gxm = (ic | m.iin(m.xm, k)) ? 0.0 : gxm;

@@ -30,6 +30,7 @@ template <class T> struct DevGrid {
     T Az, iAz;               // Δx·Δy
     const T *dzc, *dzf;      // Δzᶜ[k], Δzᶠ[k] for k = -Hz .. Nz+Hz (pointer offset so [k] is valid)
     const T *idzc, *idzf;    // reciprocals
+    const double *idzf64;    // 1 / (double)Δzᶠ[k], IEEE-rounded (the FP64 operator sweeps' factor)
     int stretched;
     const int *kbot;         // immersed bottom: immersed cells per interior column (x fastest), or nullptr
     int kbot_top;            // the largest entry of kbot: no cell at or above this level is immersed
