@@ -403,7 +403,12 @@ template <class T> struct ModelT : ModelBase {
         rhs_dense = dalloc<T>(ncell);
         phi_dense = dalloc<T>(ncell);
         const int Mx = g.Nx / 2 + 1;
-        P.S = dalloc<typename Vec2<T>::type>((size_t)Mx * g.Ny * g.Nz);
+        {   // spectral rows padded to whole 128-byte lines, so the y, z and tridiagonal passes' groups
+            // of consecutive kx read aligned lines (rows narrower than a line stay unpadded)
+            const int line = (int)(128 / sizeof(typename Vec2<T>::type));
+            P.Ms = Mx >= line ? (Mx + line - 1) / line * line : Mx;
+        }
+        P.S = dalloc<typename Vec2<T>::type>((size_t)P.Ms * g.Ny * g.Nz);
         const int xline = (g.Nx % 2 == 0) ? g.Nx / 2 : g.Nx;
         P.tw_xh = upload(twiddles<T>(xline));
         {

@@ -1,7 +1,8 @@
 // om_fft.hip — pressure Poisson solvers on gfx950.
 //
-// Spectral array S (complex, x fastest): S[(k·Ny + j)·Mx + kx] with Mx = Nx/2 + 1 (real-to-complex
-// along a Periodic x) or 1 (Flat x).  Passes, each one HBM read + one write of S:
+// Spectral array S (complex, x fastest): S[(k·Ny + j)·Ms + kx], kx < Mx = Nx/2 + 1 (real-to-complex
+// along a Periodic x) or 1 (Flat x); the row stride Ms rounds Mx up to a 128-byte line, and the
+// passes skip the padding entries kx ≥ Mx.  Passes, each one HBM read + one write of S:
 //   X-forward  : source term (∇·U* formed on the fly, optionally ×Δzᶜ) → real-to-complex FFT
 //   Y-forward  : complex FFT along y                    (skipped when y is the last axis)
 //   last axis  : forward transform · eigenvalue division · backward transform in one kernel
@@ -170,6 +171,7 @@ template <class T> struct XFwdArgs {
     int scale_dzc;
     LineGeom L;
     FastDiv fNy, fMx;
+    int Ms;
 };
 
 template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const T *src, int i, int j, int k) {
@@ -241,14 +243,14 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArg
         const int r = (int)fdiv((uint32_t)t, A.fMx), kk = t - r * Mx;
         const int row = row0 + r;
         if (row >= nrows) continue;
-        if (!half) { A.S[(int64_t)row * Mx + kk] = Z[r * ld + kk]; continue; }
+        if (!half) { A.S[(int64_t)row * A.Ms + kk] = Z[r * ld + kk]; continue; }
         const C2<T> zk = Z[r * ld + (kk == M ? 0 : kk)];
         const C2<T> zc = cconj<T>(Z[r * ld + (kk == 0 ? 0 : M - kk)]);
         const C2<T> E = cmk<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
         // O = (zk - zc) / (2i) = (-i/2)(zk - zc)
         const C2<T> d = csub<T>(zk, zc);
         const C2<T> O = cmk<T>(T(0.5) * d.y, T(-0.5) * d.x);
-        A.S[(int64_t)row * Mx + kk] = cadd<T>(E, cmul<T>(A.post[kk], O));
+        A.S[(int64_t)row * A.Ms + kk] = cadd<T>(E, cmul<T>(A.post[kk], O));
     }
 }
 
@@ -265,6 +267,7 @@ template <class T> struct XBwdArgs {
     T divisor;         // the written value is (x · scale) / divisor
     LineGeom L;
     FastDiv fNy;
+    int Ms;
 };
 
 template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArgs<T> A) {
@@ -287,7 +290,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArg
     const int row = row0 + r;
     const bool live = row < nrows;
     {
-        const C2<T> *Srow = A.S + (int64_t)(live ? row : 0) * Mx;
+        const C2<T> *Srow = A.S + (int64_t)(live ? row : 0) * A.Ms;
         constexpr int B = 4;
         for (int k0 = l; k0 < M; k0 += B * tpr) {
             C2<T> v[B];
@@ -345,7 +348,7 @@ template <class T> struct YArgs {
     DevGrid<T> g;
     C2<T> *S;
     const C2<T> *tw;
-    int Mx;
+    int Mx, Ms;
     int inverse;
     int fused_divide;
     const T *lx, *ly;   // eigenvalues (fused)
@@ -359,16 +362,17 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<
     C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *a = twl + Ny;
     C2<T> *b = a + R * ld;
-    const int nlines = Mx * A.g.Nz;
+    const int Ms = A.Ms;
+    const int nlines = Ms * A.g.Nz;
     // blockDim.x is a multiple of R, so every element a thread stages belongs to the same line
     const int r = threadIdx.x & (R - 1);
     const int line = blockIdx.x * R + r;
-    const bool valid = line < nlines;
-    const int k = (int)fdiv((uint32_t)line, A.fMx), kx = line - k * Mx;
-    C2<T> *col = A.S + (int64_t)k * Ny * Mx + kx;
+    const int k = (int)fdiv((uint32_t)line, A.fMx), kx = line - k * Ms;   // (fMx divides by Ms)
+    const bool valid = line < nlines && kx < Mx;                         // padding lines: nothing
+    C2<T> *col = A.S + (int64_t)k * Ny * Ms + kx;
     load_table<T>(twl, A.tw, Ny);
     staged_copy<8>(R * Ny,
-        [&](int t) { return valid ? col[(int64_t)(t >> lgR) * Mx] : cmk<T>(T(0), T(0)); },
+        [&](int t) { return valid ? col[(int64_t)(t >> lgR) * Ms] : cmk<T>(T(0), T(0)); },
         [&](int t, C2<T> v) { a[r * ld + (t >> lgR)] = v; });
     __syncthreads();
     C2<T> *res = lds_fft<T>(a, b, R, Ny, ld, twl, A.inverse != 0);
@@ -384,7 +388,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<
     if (!valid) return;
     for (int t = threadIdx.x; t < R * Ny; t += blockDim.x) {
         const int j = t >> lgR;
-        col[(int64_t)j * Mx] = res[r * ld + j];
+        col[(int64_t)j * Ms] = res[r * ld + j];
     }
 }
 
@@ -406,7 +410,7 @@ template <class T> struct ZArgs {
     C2<T> *S;
     const C2<T> *tw;         // table for the FFT length L
     const C2<T> *dct;        // c_k, k < Nz (zmode 1)
-    int Mx, L, zmode;        // L = Nz (zmodes 0, 1) or 2Nz (zmode 2)
+    int Mx, Ms, L, zmode;    // L = Nz (zmodes 0, 1) or 2Nz (zmode 2); Ms: row stride of S
     const T *lx, *ly, *lz;   // lz: Nz entries (Periodic), 2Nz (Bounded, symmetric)
     LineGeom G;
 };
@@ -419,12 +423,13 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<
     C2<T> *dctl = twl + L;
     C2<T> *a = dctl + (zmode == 1 ? Nz : 0);
     C2<T> *b = a + R * ld;
-    const int nlines = Mx * Ny;
+    const int Ms = A.Ms;
+    const int nlines = Ms * Ny;
     const int r = threadIdx.x & (R - 1);   // blockDim.x is a multiple of R
     const int line = blockIdx.x * R + r;
-    const bool valid = line < nlines;
-    const int j = valid ? line / Mx : 0, kx = line - j * Mx;
-    const int64_t plane = (int64_t)Ny * Mx;
+    const int j = line < nlines ? line / Ms : 0, kx = line - j * Ms;
+    const bool valid = line < nlines && kx < Mx;   // padding lines: nothing
+    const int64_t plane = (int64_t)Ny * Ms;
     C2<T> *col = A.S + line;
     load_table<T>(twl, A.tw, L);
     if (zmode == 1) load_table<T>(dctl, A.dct, Nz);
@@ -487,7 +492,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<
 template <class T> struct TriArgs {
     DevGrid<T> g;
     C2<T> *S;
-    int Mx;
+    int Mx, Ms;
     const T *lx, *ly;
     T eps10;
     T zscale;   // (1 / L_backward_norm) applied later; unused here
@@ -505,10 +510,11 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs<T> A, C2<T> *__restrict_
     constexpr int KB = tri_batch<T>();
     const int Ny = A.g.Ny, Nz = A.g.Nz, Mx = A.Mx;
     const int bw = blockDim.x;
-    const int64_t ncol = (int64_t)Mx * Ny;
+    const int64_t ncol = (int64_t)A.Ms * Ny;   // columns ℓ = ky·Ms + kx, padding columns kx ≥ Mx skipped
     const int64_t col = (int64_t)blockIdx.x * bw + threadIdx.x;
     if (col >= ncol) return;
-    const int j = (int)(col / Mx), kx = (int)(col - (int64_t)j * Mx);
+    const int j = (int)(col / A.Ms), kx = (int)(col - (int64_t)j * A.Ms);
+    if (kx >= Mx) return;
     const T lam = A.lx[kx] + A.ly[j];
     auto diag = [&](int k) -> T {   // 0-based k
         const T dz = dzc[k];
@@ -677,25 +683,25 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     size_t lds = 0;
     {
         const LineGeom L = line_geom(M, M, e, lds);
-        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L, make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx)};
+        XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L, make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx), P.Ms};
         hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
     const bool z_active = g.tz != OM_FLAT;
     const LineGeom Ly = line_geom(Ny, Ny, e, lds);
     const size_t lds_y = lds;
-    const int ygrid = (Mx * Nz + Ly.R - 1) / Ly.R;
+    const int ygrid = (P.Ms * Nz + Ly.R - 1) / Ly.R;
     {
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 0, z_active ? 0 : 1, P.lx, P.ly, Ly, make_fastdiv((uint32_t)Mx)};
+        YArgs<T> A{g, P.S, P.tw_y, Mx, P.Ms, 0, z_active ? 0 : 1, P.lx, P.ly, Ly, make_fastdiv((uint32_t)P.Ms)};
         hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     T scale = T(1) / (T(M) * T(Ny));
     if (z_active) {
         if (P.tridiagonal) {
-            TriArgs<T> A{g, P.S, Mx, P.lx, P.ly, P.eps10, T(1)};
+            TriArgs<T> A{g, P.S, Mx, P.Ms, P.lx, P.ly, P.eps10, T(1)};
             // columns per block (one wave, partly filled when narrow): each column is a serial sweep,
             // so the launch wants waves on every SIMD (≥ 2048 of them) more than full waves; 16
             // columns still read whole 128-byte lines of the spectrum.  Limited by the LDS holding t[k].
-            const int64_t ncol = (int64_t)Mx * Ny;
+            const int64_t ncol = (int64_t)P.Ms * Ny;
             int bw = 64;
             while (bw > 16 && ncol / bw < 2048) bw >>= 1;
             while (bw > 1 && (size_t)Nz * bw * sizeof(T) > budget) bw >>= 1;
@@ -703,16 +709,16 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
                                P.S, g.dzc, g.idzf);
         } else {
             const LineGeom Lz = line_geom(P.Lz_line, P.Lz_line + (P.zmode == 1 ? Nz : 0), e, lds);
-            ZArgs<T> A{g, P.S, P.tw_z, P.tw_dct, Mx, P.Lz_line, P.zmode, P.lx, P.ly, P.lz, Lz};
-            hipLaunchKernelGGL(k_zpass<T>, dim3((Mx * Ny + Lz.R - 1) / Lz.R), dim3(kFftThreads), lds, st, A);
+            ZArgs<T> A{g, P.S, P.tw_z, P.tw_dct, Mx, P.Ms, P.Lz_line, P.zmode, P.lx, P.ly, P.lz, Lz};
+            hipLaunchKernelGGL(k_zpass<T>, dim3((P.Ms * Ny + Lz.R - 1) / Lz.R), dim3(kFftThreads), lds, st, A);
             scale = T(1) / ((T(M) * T(Ny)) * T(P.Lz_norm));
         }
-        YArgs<T> A{g, P.S, P.tw_y, Mx, 1, 0, P.lx, P.ly, Ly, make_fastdiv((uint32_t)Mx)};
+        YArgs<T> A{g, P.S, P.tw_y, Mx, P.Ms, 1, 0, P.lx, P.ly, Ly, make_fastdiv((uint32_t)P.Ms)};
         hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     {
         const LineGeom L = line_geom(M, M, e, lds);
-        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L, make_fastdiv((uint32_t)Ny)};
+        XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L, make_fastdiv((uint32_t)Ny), P.Ms};
         hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
 }

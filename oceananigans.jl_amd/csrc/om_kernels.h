@@ -612,7 +612,8 @@ template <class T> struct RKArgs {
 
 // Poisson solver plan: spectral storage and per-axis tables (device pointers).
 template <class T> struct PoissonPlan {
-    typename Vec2<T>::type *S;          // complex spectral array, Mx · Ny · Nz
+    typename Vec2<T>::type *S;          // complex spectral array, Ms · Ny · Nz (x rows padded to Ms)
+    int Ms;                              // row stride of S: Mx = Nx/2 + 1 rounded up to a 128-byte line
     const typename Vec2<T>::type *tw_xh; // twiddles for the x line (length Nx/2 when Nx is even, Nx otherwise)
     const typename Vec2<T>::type *post;  // exp(-2πi k / Nx), k = 0 .. Nx/2
     const typename Vec2<T>::type *tw_y;  // length Ny
