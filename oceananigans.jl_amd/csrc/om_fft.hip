@@ -353,7 +353,7 @@ template <class T> struct YArgs {
     int fused_divide;
     const T *lx, *ly;   // eigenvalues (fused)
     LineGeom L;
-    FastDiv fMx;
+    FastDiv fMs;        // division by the row stride Ms
 };
 
 template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<T> A) {
@@ -367,7 +367,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<
     // blockDim.x is a multiple of R, so every element a thread stages belongs to the same line
     const int r = threadIdx.x & (R - 1);
     const int line = blockIdx.x * R + r;
-    const int k = (int)fdiv((uint32_t)line, A.fMx), kx = line - k * Ms;   // (fMx divides by Ms)
+    const int k = (int)fdiv((uint32_t)line, A.fMs), kx = line - k * Ms;
     const bool valid = line < nlines && kx < Mx;                         // padding lines: nothing
     C2<T> *col = A.S + (int64_t)k * Ny * Ms + kx;
     load_table<T>(twl, A.tw, Ny);
