@@ -7,7 +7,8 @@ Protocol (benchmarking/src/utils.jl:84-101): `--warmup` untimed steps, device sy
 timed steps, device sync.  Inputs are resident in HBM before the timed region.  Per-phase device
 times are HIP events recorded on the model's stream inside the timed region; the roofline entry
 is the dominant kernel's.  The CPU baseline is the FP64 oracle (a CPU restatement of the
-reference algorithm) timed on a bounded sample of the same workload on this host.
+reference algorithm) timed on this host: the full size on all physical cores, and one thread on
+a bounded sample, with the CPU model.
 
 Prints one JSON line.
 """
@@ -78,24 +79,61 @@ def run_gpu(args):
                 dom_bytes=dom_bytes, achieved=achieved, bytes_step=bytes_step, word=word, div=div, finite=finite)
 
 
+def host_cpu():
+    """(CPU model, physical cores usable by this process) from lscpu and the CPU affinity mask."""
+    import subprocess
+    model, cores = "unknown", None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        allowed = os.sched_getaffinity(0)
+        table = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        phys = {(int(c), int(s)) for cpu, c, s in (l.split(",") for l in table.splitlines() if l and l[0] != "#")
+                if int(cpu) in allowed}
+        cores = len(phys)
+    except Exception:
+        pass
+    return model, cores or len(os.sched_getaffinity(0))
+
+
 def run_cpu_baseline(args):
-    """FP64 oracle on a bounded sample of the same workload (test infrastructure, not the product)."""
+    """The FP64 oracle (a CPU restatement of the reference algorithm, test infrastructure, not the
+    product) timed on this host per benchmarking/src/utils.jl:84-101: one warm-up step, then timed
+    steps.  All physical cores usable by the process on the benchmark's full size; one thread on a
+    bounded sample of the same workload (the per-cell cost of a step is size-independent to ~10%)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from cases import oracle_for
+    from oracle import oracle as O
     from oceananigans_mi355x import workloads
 
-    N = tuple(int(x) for x in args.cpu_sample.split(","))
-    grid, kw, init = workloads.free_convection(N=N, float_type=np.float64)
-    orc = oracle_for(grid, kw, init)
-    orc.time_step(1.0, 1)                     # warm-up step
-    t0 = time.perf_counter()
-    orc.time_step(1.0, args.cpu_steps)
-    t1 = time.perf_counter()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ncell = N[0] * N[1] * N[2]
-    return {"value": ncell * args.cpu_steps / (t1 - t0), "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle (CPU restatement of the reference algorithm), same physics on {N[0]}x{N[1]}x{N[2]}, "
-                      f"{args.cpu_steps} RK3 steps after 1 warm-up step, {t1 - t0:.1f} s, OpenMP threads={cores}"}
+    model_name, phys = host_cpu()
+    cap = int(os.environ.get("OMP_NUM_THREADS", phys) or phys)
+    threads = max(1, min(phys, cap))
+
+    def timed(N, nthreads, steps):
+        O.set_threads(nthreads)
+        grid, kw, init = workloads.free_convection(N=N, float_type=np.float64)
+        orc = oracle_for(grid, kw, init)
+        orc.time_step(1.0, 1)                     # warm-up step
+        t0 = time.perf_counter()
+        orc.time_step(1.0, steps)
+        dt = time.perf_counter() - t0
+        orc.close()
+        return N[0] * N[1] * N[2] * steps / dt, dt
+
+    Nfull = tuple(int(x) for x in args.size.split(","))
+    full, t_full = timed(Nfull, threads, args.cpu_steps)
+    Ns = tuple(int(x) for x in args.cpu_sample.split(","))
+    one, t_one = timed(Ns, 1, 2)
+    return {"value": full, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (CPU restatement of the reference algorithm), the benchmark's physics at "
+                      f"{'x'.join(map(str, Nfull))}, {args.cpu_steps} RK3 step(s) after 1 warm-up step in {t_full:.1f} s, "
+                      f"OpenMP threads = {threads} (the physical cores of this process's CPU set)",
+            "cpu_model": model_name, "physical_cores": phys,
+            "one_thread": {"value": one, "unit": "cell-updates/s",
+                           "sample": f"{'x'.join(map(str, Ns))}, 2 RK3 steps after 1 warm-up step in {t_one:.1f} s, 1 thread"}}
 
 
 def run_cg_seamount(args):
@@ -244,8 +282,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", default="256,256,128")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
-    ap.add_argument("--cpu-sample", default="128,128,128")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", default="64,64,64", help="the one-thread CPU baseline's sample size")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="timed steps of the all-cores CPU baseline (full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the seamount CG measurement (learned vs FFT preconditioner)")
     ap.add_argument("--cg-size", default="256,256,128")

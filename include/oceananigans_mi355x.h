@@ -55,7 +55,10 @@ extern "C" {
 #define OM_EHIP         -2   /* HIP runtime failure; the handle is unusable afterwards */
 #define OM_ENOMEM       -3
 #define OM_ESTATE       -4   /* call not valid in the current state */
-#define OM_ENOTCONV     -5   /* iterative solver did not converge */
+#define OM_ENOTCONV     -5   /* a conjugate-gradient solve of the call stopped at maxiter above its tolerance;
+                                 the call's work completed, as the reference's solver stops there and the
+                                 step goes on (conjugate_gradient_solver.jl:279-284); the handle stays usable
+                                 and om_last_solve_info describes the last solve */
 
 /* topology per dimension (src/Grids/topology ... Periodic, Bounded, Flat) */
 #define OM_PERIODIC 0
@@ -234,7 +237,10 @@ int om_cell_advection_timescale(om_model_t *m, double *tau);
  *   om_learned_backward  mean-square loss over the mask, weight gradients and input gradient
  *                        (3 channels) for a given input, target and mask (test hook)
  *   om_learned_last_solve  iterations that used the network in the last solve, and whether the
- *                        solve handed over to the FFT preconditioner */
+ *                        solve handed over to the FFT preconditioner
+ *   om_learned_energy_loss  the training loss's energy and its gradient for a given network output
+ *                        y and target t (test hook): d = active·(y − t), E = dᵀ(−V∇²)d with the
+ *                        CG solver's conditional operator, dE/dy = −2·active·V∇²d (dense, x fastest) */
 #define OM_LEARNED_NWEIGHTS (4 * 27 * 16 * 16)
 int om_learned_set_weights(om_model_t *m, const float *weights);
 int om_learned_get_weights(om_model_t *m, float *weights);
@@ -246,6 +252,7 @@ int om_learned_forward(om_model_t *m, const float *input3, float *output);
 int om_learned_backward(om_model_t *m, const float *input3, const float *target, const float *mask, double *loss,
                         float *dweights, float *dinput3);
 int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *fell_back);
+int om_learned_energy_loss(om_model_t *m, const float *output, const float *target, double *energy, float *doutput);
 int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
                         int64_t *nan_count);
 

@@ -43,6 +43,13 @@ struct ArgError {
     std::string msg;
 };
 
+// A conjugate-gradient solve stopped at maxiter above its tolerance.  The reference stops there
+// silently and the step goes on (conjugate_gradient_solver.jl:279-284); the library does the
+// same and reports it when the call returns, as OM_ENOTCONV, without poisoning the handle.
+struct NotConverged {
+    std::string msg;
+};
+
 // ---------------------------------------------------------------------------------------------
 // Grid metrics in FT (src/Grids/grid_generation.jl:34-95 stretched, :117-156 uniform)
 // ---------------------------------------------------------------------------------------------
@@ -176,6 +183,8 @@ template <class T> struct ModelT : ModelBase {
     double *cg_partial = nullptr, *cg_out = nullptr, *cg_host = nullptr;
     int cg_iterations = 0;
     double cg_res_last = 0, cg_true_res_last = 0;
+    int cg_unconverged = 0;            // solves of the current call that stopped at maxiter above tolerance
+    std::string cg_unconverged_msg;
     // learned correction to the FFT preconditioner (OM_PRECOND_LEARNED) and its training data
     NetState net;
     float *ln_in3 = nullptr, *ln_maskd = nullptr;  // current network input (3 channels, dense), active mask
@@ -697,6 +706,17 @@ template <class T> struct ModelT : ModelBase {
         }
     }
 
+    void note_unconverged(int64_t it, double res, double tol) {
+        ++cg_unconverged;
+        char buf[256];
+        snprintf(buf, sizeof buf, "conjugate-gradient solve stopped at maxiter = %lld with |r| = %.3e above the tolerance %.3e "
+                 "(%d solve(s) of this call; the step completed as the reference's does)", (long long)it, res, tol, cg_unconverged);
+        cg_unconverged_msg = buf;
+    }
+    // throws NotConverged (after the call's work is done) when a solve of the call stopped at maxiter
+    void report_unconverged() {
+        if (cg_unconverged) { cg_unconverged = 0; throw NotConverged{cg_unconverged_msg}; }
+    }
     void cg_solve() {
         const int pk = cg_precond();
         cg_solve_with(pk);
@@ -711,6 +731,7 @@ template <class T> struct ModelT : ModelBase {
             cg_solve_with(OM_PRECOND_FFT, tol0);   // the original absolute tolerance, not one relative to the small residual left
             cg_iterations += it0;
         }
+        if (pk == OM_PRECOND_LEARNED && !(cg_res_last <= cg_tol_last)) note_unconverged(cg_iterations, cg_res_last, cg_tol_last);
     }
     double cg_tol_last = 0;
     void cg_solve_with(const int pk, const double tol_override = -1.0) {
@@ -780,6 +801,7 @@ template <class T> struct ModelT : ModelBase {
         }
         cg_iterations = (int)it;
         cg_res_last = res;
+        if (!(res <= tol) && tol_override < 0 && !(pk == OM_PRECOND_LEARNED)) note_unconverged(it, res, tol);
         if (logging && nlogged > 0 && res <= tol) ln_log_pairs(nlogged);
         cg_halo_r(g, x, st);
         cg_true_residual(g, x, cg_b, cg_q, cg_partial, cg_out, st);
@@ -998,6 +1020,9 @@ template <class F> static int guarded(om_model *m, F &&fn) {
     } catch (const ArgError &e) {
         m->err = e.msg;
         return OM_EINVAL;
+    } catch (const NotConverged &e) {
+        m->err = e.msg;
+        return OM_ENOTCONV;
     } catch (const std::bad_alloc &) {
         m->err = "host allocation failed";
         return OM_ENOMEM;
@@ -1167,9 +1192,11 @@ int om_compute_tendencies(om_model_t *m) {
 int om_rk3_substep(om_model_t *m, double dt, double gamma, double zeta, int32_t first_stage) {
     return WITH_MODEL(m, {
         using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        M->cg_unconverged = 0;
         M->rk3_substep(dt, (TT)gamma, (TT)zeta, first_stage != 0);
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
+        M->report_unconverged();
     });
 }
 
@@ -1178,9 +1205,11 @@ int om_cache_previous_tendencies(om_model_t *m) { return WITH_MODEL(m, M->cache_
 int om_pressure_correction(om_model_t *m, double dt) {
     return WITH_MODEL(m, {
         using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
+        M->cg_unconverged = 0;
         M->pressure_correction((TT)dt);
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
+        M->report_unconverged();
     });
 }
 
@@ -1188,6 +1217,7 @@ int om_time_step(om_model_t *m, double dt, int32_t nsteps) {
     if (nsteps < 0) return OM_EINVAL;
     return WITH_MODEL(m, {
         M->evused = 0;
+        M->cg_unconverged = 0;
         HIPCHECK(hipEventRecord(M->ev0, M->st));
         for (int q = 0; q < nsteps; ++q) M->time_step(dt);
         HIPCHECK(hipEventRecord(M->ev1, M->st));
@@ -1197,6 +1227,7 @@ int om_time_step(om_model_t *m, double dt, int32_t nsteps) {
         HIPCHECK(hipEventElapsedTime(&ms, M->ev0, M->ev1));
         M->last_ms = ms;
         if (M->phase_timing) M->collect_phase_times();
+        M->report_unconverged();
     });
 }
 
@@ -1226,6 +1257,7 @@ int om_poisson_solve(om_model_t *m, const double *rhs, double *phi) {
         std::vector<TT> h(n);
         for (size_t q = 0; q < n; ++q) h[q] = (TT)rhs[q];
         HIPCHECK(hipMemcpy(M->rhs_dense, h.data(), n * sizeof(TT), hipMemcpyHostToDevice));
+        M->cg_unconverged = 0;
         if (M->cg) {
             // ∇²ϕ = rhs as V∇²ϕ = V·rhs from a zero initial guess
             cg_source(M->ctx(), M->rhs_dense, M->cg_b, M->st);
@@ -1243,6 +1275,7 @@ int om_poisson_solve(om_model_t *m, const double *rhs, double *phi) {
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpy(h.data(), M->phi_dense, n * sizeof(TT), hipMemcpyDeviceToHost));
         for (size_t q = 0; q < n; ++q) phi[q] = (double)h[q];
+        M->report_unconverged();
     });
 }
 
@@ -1431,6 +1464,22 @@ int om_learned_backward(om_model_t *m, const float *input3, const float *target,
         if (dinput3) HIPCHECK(hipMemcpyAsync(dinput3, dev + 2 * nc, 3 * nc * sizeof(float), hipMemcpyDeviceToHost, M->st));
         HIPCHECK(hipStreamSynchronize(M->st));
         if (loss) *loss = h * (double)inv_n;
+        M->dfree(dev);
+    });
+}
+
+int om_learned_energy_loss(om_model_t *m, const float *output, const float *target, double *energy, float *doutput) {
+    if (!m || !output || !target || !energy) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        if (!M->cg) throw ArgError{"the energy uses the conjugate-gradient solver's operator"};
+        const size_t nc = (size_t)M->g.Nx * M->g.Ny * M->g.Nz;
+        float *dev = M->template dalloc<float>(3 * nc);   // y, t, dE/dy (a test hook: allocated per call)
+        HIPCHECK(hipMemcpyAsync(dev, output, nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        HIPCHECK(hipMemcpyAsync(dev + nc, target, nc * sizeof(float), hipMemcpyHostToDevice, M->st));
+        ln_energy(M->g, dev, dev + nc, M->cg_q, 1.0, dev + 2 * nc, M->cg_partial, M->cg_out, M->st);
+        *energy = M->cg_read(1);
+        if (doutput) HIPCHECK(hipMemcpyAsync(doutput, dev + 2 * nc, nc * sizeof(float), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
         M->dfree(dev);
     });
 }

@@ -54,13 +54,22 @@ template <class T> struct RC {
     static constexpr T eps = T(1e-8f);
 };
 
-// Smoothness quadratic form.  FP64 evaluates it on raw stencil values, where it cancels
-// heavily when the values carry a large offset; it is therefore evaluated without FMA
-// contraction and in the reference's left-to-right order, so that identical inputs give
-// identically rounded indicators.
+// The reference evaluates the smoothness indicators, the WENO weighted sum and the Centered
+// interpolants under @muladd (weno_interpolants.jl:265,279,516; centered_reconstruction.jl:56-62):
+// a sum of products p₁ + p₂ + … + pₙ, evaluated left to right, becomes
+// muladd(pₙ, muladd(…, muladd(p₂, p₁))), i.e. the first product rounded on its own and every later
+// one fused into the running sum, and a product a·b·c splits as (a·b, c).  The generic templates
+// below (the FP64 path) spell these fused multiply-adds out and contract nothing else, so that
+// identical inputs give identically rounded values; FP64 smoothness indicators on raw stencil
+// values cancel heavily when the values carry a large offset, which makes the order observable.
+
+// Smoothness quadratic form ψ₁(k₀ψ₁ + k₁ψ₂ + k₂ψ₃) + ψ₂(k₃ψ₂ + k₄ψ₃) + ψ₃ψ₃k₅
+// (weno_interpolants.jl:172-174, 228-265)
 template <class T> __device__ __forceinline__ T beta_q(T a, T b, T c, T k0, T k1, T k2, T k3, T k4, T k5) {
 #pragma clang fp contract(off)
-    return a * (k0 * a + k1 * b + k2 * c) + b * (k3 * b + k4 * c) + c * (k5 * c);
+    const T s1 = fma(k2, c, fma(k1, b, k0 * a));
+    const T s2 = fma(k4, c, k3 * b);
+    return fma(c * c, k5, fma(b, s2, a * s1));
 }
 
 // WENO(order=5) value at face i from s0..s5 = ψ[i-3..i+2] (weno_interpolants.jl:348-354, 520-531)
@@ -84,10 +93,11 @@ template <class T> __device__ __forceinline__ T weno5(T s0, T s1, T s2, T s3, T 
     const T qa = wdiv<T>(tau, ba + K::eps), qb = wdiv<T>(tau, bb + K::eps), qc = wdiv<T>(tau, bc + K::eps);
     const T al0 = K::C50 * (T(1) + qa * qa), al1 = K::C51 * (T(1) + qb * qb), al2 = K::C52 * (T(1) + qc * qc);
     const T sinv = T(1) / ((al0 + al1) + al2);
+    // candidate values sum(coeff .* ψ) (no @muladd), then the weighted sum under @muladd
     const T pa = K::a0 * a0 + K::a1 * a1 + K::a2 * a2;
     const T pb = K::b0 * b0 + K::b1 * b1 + K::b2 * b2;
     const T pc = K::c0 * c0 + K::c1 * c1 + K::c2 * c2;
-    return (al0 * sinv) * pa + (al1 * sinv) * pb + (al2 * sinv) * pc;
+    return fma(al2 * sinv, pc, fma(al1 * sinv, pb, (al0 * sinv) * pa));
 }
 
 // FP32 WENO(order=5): the same stencils, smoothness indicators on central-value-shifted values
@@ -126,9 +136,10 @@ template <> __device__ __forceinline__ float recip<float>(float x) { return __bu
 template <class T> __device__ __forceinline__ T ft_div(T a, T b) { return a / b; }
 template <> __device__ __forceinline__ float ft_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// WENO(order=3) smoothness ψ₁(ψ₁ − 2ψ₂) + ψ₂ψ₂·1 under @muladd
 template <class T> __device__ __forceinline__ T beta2_q(T a, T b) {
 #pragma clang fp contract(off)
-    return a * (T(1) * a + T(-2) * b) + b * (T(1) * b);
+    return fma(b * b, T(1), a * fma(T(-2), b, T(1) * a));
 }
 
 // WENO(order=3) value at face i from s0..s3 = ψ[i-2..i+1]
@@ -149,15 +160,16 @@ template <class T> __device__ __forceinline__ T weno3(T s0, T s1, T s2, T s3, bo
     const T qa = wdiv<T>(tau, ba + K::eps), qb = wdiv<T>(tau, bb + K::eps);
     const T al0 = K::C30 * (T(1) + qa * qa), al1 = K::C31 * (T(1) + qb * qb);
     const T sinv = recip<T>(al0 + al1);
-    return (al0 * sinv) * (K::d0 * a0 + K::d1 * a1) + (al1 * sinv) * (K::e0 * b0 + K::e1 * b1);
+    return fma(al1 * sinv, K::e0 * b0 + K::e1 * b1, (al0 * sinv) * (K::d0 * a0 + K::d1 * a1));
 }
 
 // ℑ of four values, as the reference's nested two-point averages
 template <class T> __device__ __forceinline__ T avg4(T a, T b, T c, T d) { return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d)); }
 
+// Centered(4) on offsets (−2, −1, 0, +1) from the face, under @muladd
 template <class T> __device__ __forceinline__ T cen4(T m2, T m1, T z0, T p1) {
     typedef RC<T> K;
-    return K::q_m2 * m2 + K::q_m1 * m1 + K::q_0 * z0 + K::q_p1 * p1;
+    return fma(K::q_p1, p1, fma(K::q_0, z0, fma(K::q_m1, m1, K::q_m2 * m2)));
 }
 // cen4 of four values that share one area factor A: in FP64 the reference's order,
 // cen4(A·m2, A·m1, A·z0, A·p1); in FP32 the factor applied once to the interpolated value.
@@ -165,9 +177,10 @@ template <class T> __device__ __forceinline__ T acen4(T A, T m2, T m1, T z0, T p
     if (sizeof(T) == 4) return A * cen4<T>(m2, m1, z0, p1);
     return cen4<T>(A * m2, A * m1, A * z0, A * p1);
 }
+// Centered(2) under @muladd (the same value as unfused: both products are exact halvings)
 template <class T> __device__ __forceinline__ T cen2(T m1, T z0) {
     typedef RC<T> K;
-    return K::h_m1 * m1 + K::h_0 * z0;
+    return fma(K::h_0, z0, K::h_m1 * m1);
 }
 
 // XCD-aware work order for a 1-D grid of nb blocks.  The dispatcher hands consecutive block ids to

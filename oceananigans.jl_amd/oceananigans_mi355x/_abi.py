@@ -31,6 +31,7 @@ OM_TUNE_POISSON_REFINEMENT = 3
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
 
+OM_EINVAL, OM_EHIP, OM_ENOMEM, OM_ESTATE, OM_ENOTCONV = -1, -2, -3, -4, -5
 ERRORS = {-1: "OM_EINVAL", -2: "OM_EHIP", -3: "OM_ENOMEM", -4: "OM_ESTATE", -5: "OM_ENOTCONV"}
 
 
@@ -131,6 +132,8 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "om_learned_last_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "om_learned_energy_loss": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
     "om_field_statistics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
 }
@@ -158,11 +161,20 @@ class OmError(RuntimeError):
     pass
 
 
+class SolverNotConvergedWarning(RuntimeWarning):
+    """A conjugate-gradient pressure solve stopped at maxiter above its tolerance; the call's work
+    completed (the reference's solver stops there silently, conjugate_gradient_solver.jl:279-284)."""
+
+
 def check(rc, handle=None):
     if rc != 0:
         lib = load_library()
         msg = lib.om_last_error(handle)
         msg = msg.decode() if msg else ""
+        if rc == OM_ENOTCONV:
+            import warnings
+            warnings.warn(msg, SolverNotConvergedWarning, stacklevel=3)
+            return
         raise OmError(f"{ERRORS.get(rc, rc)}: {msg}")
 
 

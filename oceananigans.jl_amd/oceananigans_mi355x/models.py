@@ -323,6 +323,10 @@ class NonhydrostaticModel:
     def update_state(self):
         _abi.check(self._lib.om_update_state(self._h), self._h)
 
+    def compute_tendencies(self):
+        """compute_tendencies!(model) on the current state, closure fields and pHY′ (no halo fill)."""
+        _abi.check(self._lib.om_compute_tendencies(self._h), self._h)
+
     def compute_pressure_correction(self, dt):
         _abi.check(self._lib.om_pressure_correction(self._h, float(dt)), self._h)
 
@@ -458,6 +462,17 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_learned_backward(self._h, self._fp(x), self._fp(t), self._fp(m), ctypes.byref(loss),
                                                  self._fp(dw), self._fp(din)), self._h)
         return loss.value, dw.reshape(4, 27, 16, 16), np.transpose(din, (2, 1, 0, 3))
+
+    def learned_energy_loss(self, output, target):
+        """(E, dE/dy) of the training loss's energy for a network output y and target t, both
+        (Nx, Ny, Nz): d = active·(y − t), E = dᵀ(−V∇²)d with the CG solver's operator."""
+        g = self.grid
+        tr = lambda a: np.ascontiguousarray(np.transpose(np.asarray(a, dtype=np.float32), (2, 1, 0)))
+        y, t = tr(output), tr(target)
+        grad = np.empty((g.Nz, g.Ny, g.Nx), dtype=np.float32)
+        e = ctypes.c_double()
+        _abi.check(self._lib.om_learned_energy_loss(self._h, self._fp(y), self._fp(t), ctypes.byref(e), self._fp(grad)), self._h)
+        return e.value, np.transpose(grad, (2, 1, 0))
 
     def learned_last_solve(self):
         a, b = ctypes.c_int32(), ctypes.c_int32()

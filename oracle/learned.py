@@ -124,6 +124,34 @@ def backward(acts, weights, dout, precise=False):
     return np.stack(dws), gin
 
 
+def energy(d, dx, dy, dzc, dzf, active):
+    """The training loss's energy E = dᵀ(−V∇²)d of a masked error d (Nx, Ny, Nz), with the CG
+    solver's conditional operator: a sum over the faces between two active cells of
+    (face area / spacing)·(d_a − d_b)², x and y periodic, no flux through the z walls or into
+    inactive cells.  dzc[k] are the cell heights, dzf[k] (k = 1 .. Nz−1) the spacings between
+    cell centres k−1 and k.  Returns (E, dE/dd) in FP64."""
+    d = np.where(active, np.asarray(d, dtype=np.float64), 0.0)
+    Nx, Ny, Nz = d.shape
+    E = 0.0
+    g = np.zeros_like(d)
+    dzc = np.asarray(dzc, dtype=np.float64)
+    for axis, coef in ((0, dy * dzc / dx), (1, dx * dzc / dy)):
+        nb = np.roll(d, 1, axis=axis)
+        both = active & np.roll(active, 1, axis=axis)
+        c = np.where(both, np.broadcast_to(coef[None, None, :], d.shape), 0.0)
+        diff = d - nb
+        E += float((c * diff * diff).sum())
+        g += 2 * c * diff
+        g -= np.roll(2 * c * diff, -1, axis=axis)
+    for k in range(1, Nz):
+        c = np.where(active[:, :, k] & active[:, :, k - 1], dx * dy / dzf[k], 0.0)
+        diff = d[:, :, k] - d[:, :, k - 1]
+        E += float((c * diff * diff).sum())
+        g[:, :, k] += 2 * c * diff
+        g[:, :, k - 1] -= 2 * c * diff
+    return E, np.where(active, g, 0.0)
+
+
 def adam(params, grads, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     """One Adam step (Kingma & Ba 2015, Algorithm 1) in FP32, as the device applies it."""
     p, g = params.astype(np.float32), grads.astype(np.float32)

@@ -33,8 +33,20 @@
 #include <limits>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/oceananigans_mi355x.h"
+
+// Planted mistakes, for tests that show a pin fails on a plausible error (built only as the
+// separate oracle/liboracle_plant<N>.so libraries of those tests; 0 in every other build):
+//   1  one AMD interpolation misplaced: the ν·Σ₂₂ term's ffc quantity averaged with ℑxz instead of ℑxy
+//   2  pHY′ started at the top with Δzᶠ(Nz) instead of Δzᶠ(Nz+1)
+//   3  the z buffer band of WENO5 one face too wide at the bottom (WENO5 at face 3)
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 
 namespace orc {
 
@@ -178,18 +190,31 @@ template <class T> struct Coeffs {
 
 template <class T> static inline bool is_fp32() { return sizeof(T) == 4; }
 
-// β for WENO{3} (weno_interpolants.jl:172-174, 264-283): un-normalised quadratic forms.
+// The reference wraps the smoothness indicators, the WENO weighted sum and the Centered
+// interpolants in @muladd (weno_interpolants.jl:265,279,516; centered_reconstruction.jl:56-62),
+// which Julia lowers to fused multiply-adds on FMA hardware.  MuladdMacro 0.2 keeps Julia's
+// left-to-right evaluation of a sum of products: p₁ + p₂ + … + pₙ becomes
+// muladd(pₙ, muladd(…, muladd(p₂, p₁))) — the first product is rounded on its own and every
+// later one is fused into the running sum — and a product of three factors a·b·c is split as
+// (a·b, c).  Everything outside these sites is evaluated without contraction.
+
+// β for WENO{3} (weno_interpolants.jl:172-174, 228-283): with C = (C₁..C₆) the reference's
+// unrolled form ψ₁(C₁ψ₁ + C₂ψ₂ + C₃ψ₃) + ψ₂(C₄ψ₂ + C₅ψ₃) + ψ₃ψ₃C₆ under @muladd; FP32 first
+// subtracts the stencil's middle value ψ̂ = ψ₂ from every entry.
 template <class T> static inline T beta5(const T p[3], int r) {
     static const double C[3][6] = {{10, -31, 11, 25, -19, 4}, {4, -13, 5, 13, -13, 4}, {4, -19, 11, 25, -31, 10}};
     T a = p[0], b = p[1], c = p[2];
     if (is_fp32<T>()) { T h = p[1]; a -= h; b -= h; c -= h; }
     const double *K = C[r];
-    return a * ((T)K[0] * a + (T)K[1] * b + (T)K[2] * c) + b * ((T)K[3] * b + (T)K[4] * c) + c * ((T)K[5] * c);
+    const T s1 = std::fma((T)K[2], c, std::fma((T)K[1], b, (T)K[0] * a));
+    const T s2 = std::fma((T)K[4], c, (T)K[3] * b);
+    return std::fma(c * c, (T)K[5], std::fma(b, s2, a * s1));
 }
+// β for WENO{2}: ψ₁(C₁ψ₁ + C₂ψ₂) + ψ₂ψ₂C₃ with C = (1, −2, 1)
 template <class T> static inline T beta3(const T p[2]) {
     T a = p[0], b = p[1];
     if (is_fp32<T>()) { T h = p[1]; a -= h; b -= h; }
-    return a * (T(1) * a + T(-2) * b) + b * (T(1) * b);
+    return std::fma(b * b, T(1), a * std::fma(T(-2), b, T(1) * a));
 }
 
 // WENO(order=5) biased face value from S = (ψ[i-3], ..., ψ[i+2]); left = (ũ > 0)
@@ -209,12 +234,13 @@ template <class T> static T weno5(const Coeffs<T> &K, const T S[6], bool left) {
     T tau = std::abs(beta[0] - beta[2]);
     for (int r = 0; r < 3; ++r) { T q = tau / (beta[r] + K.eps); alpha[r] = K.Cs5[r] * (T(1) + q * q); }
     T sinv = T(1) / ((alpha[0] + alpha[1]) + alpha[2]);
-    T out = 0;
+    // ω = α .* Σα⁻¹; candidate values sum(coeff .* ψ) (no @muladd); the weighted sum under @muladd
+    T w[3], p[3];
     for (int r = 0; r < 3; ++r) {
-        T p = K.w5[r][0] * ps[r][0] + K.w5[r][1] * ps[r][1] + K.w5[r][2] * ps[r][2];
-        out += (alpha[r] * sinv) * p;
+        w[r] = alpha[r] * sinv;
+        p[r] = K.w5[r][0] * ps[r][0] + K.w5[r][1] * ps[r][1] + K.w5[r][2] * ps[r][2];
     }
-    return out;
+    return std::fma(w[2], p[2], std::fma(w[1], p[1], w[0] * p[0]));
 }
 
 // WENO(order=3) biased face value from S = (ψ[i-2], ψ[i-1], ψ[i], ψ[i+1])
@@ -227,9 +253,8 @@ template <class T> static T weno3(const Coeffs<T> &K, const T S[4], bool left) {
     T alpha[2];
     for (int r = 0; r < 2; ++r) { T q = tau / (beta[r] + K.eps); alpha[r] = K.Cs3[r] * (T(1) + q * q); }
     T sinv = T(1) / (alpha[0] + alpha[1]);
-    T out = 0;
-    for (int r = 0; r < 2; ++r) out += (alpha[r] * sinv) * (K.w3[r][0] * ps[r][0] + K.w3[r][1] * ps[r][1]);
-    return out;
+    const T p0 = K.w3[0][0] * ps[0][0] + K.w3[0][1] * ps[0][1], p1 = K.w3[1][0] * ps[1][0] + K.w3[1][1] * ps[1][1];
+    return std::fma(alpha[1] * sinv, p1, (alpha[0] * sinv) * p0);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -409,7 +434,7 @@ enum ZScheme { Z5, Z3, Z2, Z4 };
 template <class T> static inline ZScheme zb_face(const Model<T> &m, int k) {
     if (!m.bz()) return Z5;
     const int Nz = m.Nz();
-    if (k >= 4 && k <= Nz - 2) return Z5;
+    if (k >= (ORC_PLANT == 3 ? 3 : 4) && k <= Nz - 2) return Z5;
     if (k >= 3 && k <= Nz - 1) return Z3;
     return Z2;
 }
@@ -436,11 +461,12 @@ template <class T> static inline ZScheme zs_center(const Model<T> &m, int k) {
 template <class T, class Q> static inline T rec_biased(const Coeffs<T> &K, ZScheme s, bool left, Q q) {
     if (s == Z5) { T S[6] = {q(-3), q(-2), q(-1), q(0), q(1), q(2)}; return weno5(K, S, left); }
     if (s == Z3) { T S[4] = {q(-2), q(-1), q(0), q(1)}; return weno3(K, S, left); }
-    return K.c2[0] * q(-1) + K.c2[1] * q(0);
+    return std::fma(K.c2[1], q(0), K.c2[0] * q(-1));
 }
+// Centered(4) / Centered(2) under @muladd (centered_reconstruction.jl:56-62)
 template <class T, class Q> static inline T rec_symmetric(const Coeffs<T> &K, ZScheme s, Q q) {
-    if (s == Z4) return K.c4[0] * q(-2) + K.c4[1] * q(-1) + K.c4[2] * q(0) + K.c4[3] * q(1);
-    return K.c2[0] * q(-1) + K.c2[1] * q(0);
+    if (s == Z4) return std::fma(K.c4[3], q(1), std::fma(K.c4[2], q(0), std::fma(K.c4[1], q(-1), K.c4[0] * q(-2))));
+    return std::fma(K.c2[1], q(0), K.c2[0] * q(-1));
 }
 
 template <class T> static inline bool bias_left(T u) { return u > T(0); }
@@ -739,7 +765,8 @@ template <class T> struct Clo {
     T amd_r(int i, int j, int k) const {
         T S11 = dx_u(i, j, k), S22 = dy_v(i, j, k), S33 = dz_w(i, j, k);
         T t1 = S11 * (dx_u(i, j, k) * dx_u(i, j, k))
-             + S22 * Ixy_cc(i, j, k, L(n_dx_v(a, b, c) * n_dx_v(a, b, c)))
+             + S22 * (ORC_PLANT == 1 ? Ixz_cc(i, j, k, L(n_dx_v(a, b, c) * n_dx_v(a, b, c)))
+                                     : Ixy_cc(i, j, k, L(n_dx_v(a, b, c) * n_dx_v(a, b, c))))
              + S33 * Ixz_cc(i, j, k, L(n_dx_w(a, b, c) * n_dx_w(a, b, c)))
              + T(2) * dx_u(i, j, k) * Ixy_cc(i, j, k, L(n_dx_v(a, b, c) * n_S12(a, b, c)))
              + T(2) * dx_u(i, j, k) * Ixz_cc(i, j, k, L(n_dx_w(a, b, c) * n_S13(a, b, c)))
@@ -965,7 +992,7 @@ template <class T> static void update_hydrostatic_pressure(Model<T> &m) {
     auto Izb = [&](int i, int j, int k) { return T(0.5) * (buoyancy_pert(m, i, j, k - 1) + buoyancy_pert(m, i, j, k)); };
 #pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
-        T p = -Izb(i, j, Nz + 1) * m.g.dzF(Nz + 1);
+        T p = -Izb(i, j, Nz + 1) * m.g.dzF(ORC_PLANT == 2 ? Nz : Nz + 1);
         m.pHY(i, j, Nz) = p;
         for (int k = Nz - 1; k >= 1; --k) {
             p -= Izb(i, j, k + 1) * m.g.dzF(k + 1);
@@ -1823,6 +1850,26 @@ void orc_get_clock(void *hp, double *time, int64_t *it, int *stage) {
     auto *h = (orc_model *)hp;
     DISPATCH(h, { *time = m->time; *it = m->iteration; *stage = m->stage; });
 }
+// OpenMP threads of the oracle's later parallel regions started from the calling thread
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+// restore_prognostic_state! of the clock (nonhydrostatic_model.jl:354-375, clock.jl:147-173)
+void orc_set_clock(void *hp, double time, int64_t it, int stage, double last_dt, double last_stage_dt) {
+    auto *h = (orc_model *)hp;
+    DISPATCH(h, { m->time = time; m->iteration = it; m->stage = stage; m->last_dt = last_dt; m->last_stage_dt = last_stage_dt; });
+}
 
 // ∇·U at cell centres (interior)
 void orc_divergence(void *hp, double *out) {
@@ -1868,6 +1915,13 @@ double orc_weno5(const double *S, int left, int fp32) {
 double orc_weno3(const double *S, int left, int fp32) {
     if (fp32) { Coeffs<float> K; float s[4]; for (int i = 0; i < 4; ++i) s[i] = (float)S[i]; return weno3<float>(K, s, left != 0); }
     Coeffs<double> K; return weno3<double>(K, S, left != 0);
+}
+// Centered(4) symmetric interpolation at face i from S = (ψ[i-2], ψ[i-1], ψ[i], ψ[i+1])
+double orc_centered4(const double *S, int fp32) {
+    if (fp32) { Coeffs<float> K; float s[4]; for (int i = 0; i < 4; ++i) s[i] = (float)S[i];
+                return rec_symmetric(K, Z4, [&](int o) { return s[o + 2]; }); }
+    Coeffs<double> K;
+    return rec_symmetric(K, Z4, [&](int o) { return S[o + 2]; });
 }
 void orc_weno5_smoothness(const double *S, int left, int fp32, double *beta_out, double *weights_out) {
     auto run = [&](auto tag) {

@@ -16,20 +16,26 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB):
+def lib(plant=0):
+    """The oracle library; plant > 0 loads the build with that planted mistake (om_oracle.cpp
+    ORC_PLANT), which only the tests that show a pin fails on it use."""
+    if plant in _libs:
+        return _libs[plant]
+    # ORACLE_LIBRARY names another build of the oracle (the sanitizer build in its own test)
+    default = os.environ.get("ORACLE_LIBRARY", LIB)
+    path = default if plant == 0 else os.path.join(HERE, f"liboracle_plant{plant}.so")
+    if plant == 0 and path == LIB and not os.path.exists(LIB):
         build()
-    L = ctypes.CDLL(LIB)
+    if plant != 0:
+        subprocess.run(["make", "-s", "-C", HERE, os.path.basename(path)], check=True)
+    L = ctypes.CDLL(path)
     P = ctypes.POINTER
     d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
     pd = P(d)
@@ -51,6 +57,9 @@ def lib():
         "orc_time_step": (None, [vp, d, i]),
         "orc_set_finalize": (None, [vp, i]),
         "orc_get_clock": (None, [vp, pd, P(ctypes.c_int64), P(i)]),
+        "orc_set_clock": (None, [vp, d, ctypes.c_int64, i, d, d]),
+        "orc_set_threads": (None, [i]),
+        "orc_max_threads": (i, []),
         "orc_divergence": (None, [vp, pd]),
         "orc_poisson_solve": (None, [vp, pd, pd]),
         "orc_last_solve_info": (None, [vp, P(i), pd, pd]),
@@ -59,6 +68,7 @@ def lib():
         "orc_grid_z": (None, [vp, pd, pd, pd, pd]),
         "orc_weno5": (d, [pd, i, i]),
         "orc_weno3": (d, [pd, i, i]),
+        "orc_centered4": (d, [pd, i]),
         "orc_weno5_smoothness": (None, [pd, i, i, pd, pd]),
         "orc_coefficients": (None, [i, pd, pd, pd, pd]),
         "orc_thomas": (None, [i, i, pd, pd, pd, pd, pd]),
@@ -67,7 +77,7 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
+    _libs[plant] = L
     return L
 
 
@@ -82,8 +92,8 @@ def _f64(a):
 class OracleModel:
     """CPU restatement of NonhydrostaticModel on the same om_config_t."""
 
-    def __init__(self, config, grid):
-        self.L = lib()
+    def __init__(self, config, grid, plant=0):
+        self.L = lib(plant)
         self.config = config
         self.grid = grid
         self.h = self.L.orc_create(ctypes.byref(config))
@@ -179,6 +189,25 @@ class OracleModel:
         self.L.orc_get_clock(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(st))
         return t.value, it.value, st.value
 
+    def set_clock(self, time, iteration, stage=1, last_dt=float("inf"), last_stage_dt=float("inf")):
+        self.L.orc_set_clock(self.h, float(time), int(iteration), int(stage), float(last_dt), float(last_stage_dt))
+
+    def load_state(self, state, names):
+        """restore_prognostic_state! from a checkpoint dictionary {group path: parent array} (the
+        reference's JLD2 group names): prognostic fields, Gⁿ, G⁻, pNHS and the clock."""
+        g = self.grid
+        lo = tuple(1 - h for h in g.H)
+        for q, n in enumerate(names):
+            group = "velocities" if q < 3 else "tracers"
+            for key, fid in ((f"{group}/{n}/data", q), (f"timestepper/Gⁿ/{n}/data", 100 + q),
+                             (f"timestepper/G⁻/{n}/data", 200 + q)):
+                a = np.asarray(state[key])
+                self.set_range(fid, lo, tuple(l + s - 1 for l, s in zip(lo, a.shape)), a)
+        p = np.asarray(state["pressures/pNHS/data"])
+        self.set_range(300, lo, tuple(l + s - 1 for l, s in zip(lo, p.shape)), p)
+        self.set_clock(state["clock/time"], state["clock/iteration"], state["clock/stage"],
+                       state["clock/last_Δt"], state["clock/last_stage_Δt"])
+
     def divergence(self):
         g = self.grid
         out = np.empty(g.Nx * g.Ny * g.Nz)
@@ -217,9 +246,21 @@ class OracleModel:
         return a   # dzC, dzF, zC, zF for k = 1-Hz .. Nz+Hz
 
 
+def set_threads(n):
+    """OpenMP thread count of the oracle's later parallel regions (called from this thread)."""
+    lib().orc_set_threads(int(n))
+    return lib().orc_max_threads()
+
+
 def weno5(S, left=True, fp32=False):
     S = np.ascontiguousarray(S, dtype=np.float64)
     return lib().orc_weno5(_p(S), 1 if left else 0, 1 if fp32 else 0)
+
+
+def centered4(S, fp32=False):
+    """Centered(4) interpolation at face i of S = (ψ[i-2], ψ[i-1], ψ[i], ψ[i+1])."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    return lib().orc_centered4(_p(S), 1 if fp32 else 0)
 
 
 def weno3(S, left=True, fp32=False):

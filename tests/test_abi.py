@@ -153,3 +153,80 @@ def test_tuning_keys_match_the_header():
     text = open(HEADER).read()
     keys = dict(re.findall(r"#define\s+(OM_TUNE_\w+)\s+(\d+)", text))
     assert keys and all(getattr(_abi, k) == int(v) for k, v in keys.items()), keys
+
+
+JULIA_EXT = os.path.join(ROOT, "oceananigans.jl_amd", "julia", "OceananigansMI355XExt.jl")
+# size and alignment of the Julia field types used in the mirrored structs (C ABI on x86-64)
+JULIA_TYPES = {"Int32": (4, 4), "UInt32": (4, 4), "Int64": (8, 8), "Float32": (4, 4), "Float64": (8, 8), "Ptr": (8, 8)}
+
+
+def _julia_struct(text, name, consts):
+    """[(field, size, align)] of `struct name ... end` in the extension, in declaration order."""
+    body = re.search(r"^struct %s\n(.*?)^end" % name, text, flags=re.M | re.S)
+    assert body, name
+    out = []
+    for line in body.group(1).splitlines():
+        m = re.match(r"\s*(\w+)\s*::\s*(.+?)\s*$", line)
+        if not m:
+            continue
+        field, ty = m.groups()
+        t = re.match(r"NTuple\{(\w+),\s*(\w+)\}", ty)
+        if t:
+            n = int(consts.get(t.group(1), t.group(1)))
+            size, align = JULIA_TYPES[t.group(2)]
+            out.append((field, n * size, align))
+        else:
+            out.append((field,) + JULIA_TYPES[re.sub(r"\{.*", "", ty)])
+    return out
+
+
+def _julia_offsets(fields):
+    off, offs, maxa = 0, [], 1
+    for name, size, align in fields:
+        off = (off + align - 1) // align * align
+        offs.append((name, off))
+        off += size
+        maxa = max(maxa, align)
+    return offs, (off + maxa - 1) // maxa * maxa
+
+
+def test_julia_struct_mirrors_the_header():
+    """The Julia extension's OmConfig / OmFieldInfo declare the C structs' fields in the header's
+    order with the header's types: every field offset and the struct size agree (gcc offsetof on
+    the header; Julia isbits structs follow the C layout rules), and the constants it defines
+    equal the header's."""
+    text = open(JULIA_EXT).read()
+    header = open(HEADER).read()
+    hdefs = dict(re.findall(r"#define\s+(OM_\w+)\s+\(?(-?\d+)", header))
+    consts = {"OM_MAX_FIELDS": int(hdefs["OM_MAX_TRACERS"]) + 3}
+    # constants: "const A, B = Int32.((0, 1))" and "const A = Int32(3)" forms
+    jl = {}
+    for names, vals in re.findall(r"^const ([A-Z0-9_, \n]+?)\s*=\s*(?:Int32\.)?\(\(?([-\d, ]+)\)?\)\s*$", text, flags=re.M):
+        ns = [n.strip() for n in names.replace("\n", " ").split(",") if n.strip()]
+        vs = [int(v) for v in vals.split(",") if v.strip()]
+        jl.update(zip(ns, vs))
+    for names, vals in re.findall(r"^const ([A-Z0-9_, ]+?) = ((?:Int32\(-?\d+\),?\s*)+)$", text, flags=re.M):
+        ns = [n.strip() for n in names.split(",")]
+        vs = [int(v) for v in re.findall(r"Int32\((-?\d+)\)", vals)]
+        jl.update(zip(ns, vs))
+    jl.update({k: int(v) for k, v in re.findall(r"^const (OM_MAX_TRACERS) = (\d+)$", text, flags=re.M)})
+    assert len(jl) >= 20, jl
+    for k, v in jl.items():
+        assert k in hdefs and int(hdefs[k]) == v, (k, v, hdefs.get(k))
+    for cname, jname in (("om_config_t", "OmConfig"), ("om_field_info_t", "OmFieldInfo")):
+        fields = _julia_struct(text, jname, consts)
+        offs, size = _julia_offsets(fields)
+        prog = "#include <stddef.h>\n#include <stdio.h>\n#include \"oceananigans_mi355x.h\"\nint main(void) {\n"
+        prog += '  printf("%%zu\\n", sizeof(%s));\n' % cname
+        for name, _ in offs:
+            prog += '  printf("%%zu\\n", offsetof(%s, %s));\n' % (cname, name)
+        prog += "  return 0;\n}\n"
+        with tempfile.TemporaryDirectory() as d:
+            c = os.path.join(d, "layout.c")
+            exe = os.path.join(d, "layout")
+            open(c, "w").write(prog)
+            subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+            vals = [int(x) for x in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+        assert vals[0] == size, (cname, "size", vals[0], size)
+        for (name, off), coff in zip(offs, vals[1:]):
+            assert off == coff, (cname, name, off, coff)

@@ -97,3 +97,35 @@ def test_cg_fp32_model_reaches_fp64_tolerance():
     assert true_res <= 10 * res + 1e-30
     assert max(errs.values()) <= 1e-5
     assert np.abs(b.divergence()).max() <= 1e-4 * max(np.abs(b.velocities.w.interior()).max(), 1e-30)
+
+
+def test_cg_maxiter_stop_reports_and_matches_oracle():
+    """The reference's solver stops at maxiter and the step goes on (conjugate_gradient_solver.jl:
+    279-284).  With maxiter = 2 and an unreachable tolerance every solve stops early: the step
+    completes, the call reports OM_ENOTCONV (a SolverNotConvergedWarning in Python, the handle
+    stays usable), and the state equals the oracle's, which stops after the same two iterations."""
+    import warnings
+    grid, kw, init = convection(N=(16, 16, 16), noise_velocity=1e-2)
+    kw = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=None, reltol=1e-15, abstol=0, maxiter=2))
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        model = product_for(grid, kw, init)          # set!'s projection is a solve too
+        orc = oracle_for(grid, kw, init)
+        om.time_step_(model, 1.0)
+    assert any(issubclass(w.category, om._abi.SolverNotConvergedWarning) for w in caught)
+    assert any("maxiter = 2" in str(w.message) for w in caught)
+    orc.time_step(1.0, 1)
+    it, res, _ = model.last_solve_info()
+    assert it == 2 and orc.last_solve_info()[0] == 2
+    assert model.clock.iteration == 1
+    names = prognostic_names(kw)
+    errs = {n: rel_err(model.velocities[n].interior() if q < 3 else model.tracers[n].interior(), orc.get(q))
+            for q, n in enumerate(names)}
+    print("maxiter = 2", {k: f"{v:.2e}" for k, v in errs.items()}, "residual", res, flush=True)
+    assert max(errs.values()) <= 1e-11
+    # a raw call returns the status code; the next call on the handle works
+    lib = om._abi.load_library()
+    assert lib.om_time_step(model._h, 1.0, 1) == om._abi.OM_ENOTCONV
+    assert b"maxiter" in lib.om_last_error(model._h)
+    assert model.clock.iteration == 2
+    model.close()

@@ -114,14 +114,43 @@ def test_training_gain_on_held_out_solves_and_flexible_cg_matches_fft_cg():
     assert true_res <= 100 * res + 1e-30 or true_res <= 1e-8
 
 
-def test_untrained_large_weights_still_converge():
-    """A network far from useful (large random weights): flexible CG either converges or hands
-    over to the FFT preconditioner; either way the state matches FFT-preconditioned CG."""
-    learned, fft = _seamount_models(om.LearnedPreconditioner(weights=L.init_weights(11, scale=5.0)),
+def test_untrained_large_weights_hand_over_to_fft():
+    """A network far from useful (large random weights, output layer scaled up): flexible CG stalls,
+    hands over to the FFT preconditioner (three iterations without progress, or the recomputed FP64
+    residual above a hundred times the tolerance), and the state still matches FFT-preconditioned CG."""
+    learned, fft = _seamount_models(om.LearnedPreconditioner(weights=L.init_weights(11, scale=50.0)),
                                     om.FourierTridiagonalPoissonSolver(), N=(16, 16, 12))
     for _ in range(2):
         om.time_step_(learned, 20.0)
         om.time_step_(fft, 20.0)
-    print("learned", learned.last_solve_info(), learned.learned_last_solve(), "fft", fft.last_solve_info(), flush=True)
+        its, fell_back = learned.learned_last_solve()
+        print("learned", learned.last_solve_info(), (its, fell_back), "fft", fft.last_solve_info(), flush=True)
+        assert fell_back and its >= 1
     for n in ("u", "v", "w"):
         assert rel_err(learned.velocities[n].interior(), fft.velocities[n].interior()) <= 1e-8
+
+
+def test_energy_loss_and_gradient_match_oracle():
+    """The loss the training back-propagates: E = dᵀ(−V∇²)d, d = active·(y − t), with the CG
+    solver's conditional operator on the immersed seamount, and dE/dy, against the NumPy
+    restatement (oracle/learned.py energy, itself pinned to the oracle CG operator's quadratic
+    form and to finite differences)."""
+    grid, kw, init = seamount(N=(32, 32, 16))
+    kw = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=om.LearnedPreconditioner()))
+    model = om.NonhydrostaticModel(grid, **kw)
+    from oracle import oracle as O
+    cfg, keep = om.build_config(grid, **kw)
+    dzc, dzf, zc, zf = O.OracleModel(cfg, grid).grid_z()
+    H, Nz = grid.H[2], grid.Nz
+    active = ~(zc[H:H + Nz][None, None, :] <= grid.bottom_heights()[:, :, None])
+    rng = np.random.default_rng(21)
+    y = rng.normal(size=grid.N).astype(np.float32)
+    t = rng.normal(size=grid.N).astype(np.float32)
+    E, g = model.learned_energy_loss(y, t)
+    dx, dy = grid.extent[0] / grid.Nx, grid.extent[1] / grid.Ny
+    Eo, go = L.energy(y.astype(np.float64) - t.astype(np.float64), dx, dy, dzc[H:H + Nz], dzf[H:H + Nz], active)
+    e = _rel(g, go)
+    print("energy", E, Eo, "gradient rel err", e, flush=True)
+    assert E == pytest.approx(Eo, rel=1e-12)
+    assert e <= 1e-6
+    model.close()

@@ -11,9 +11,10 @@ Conditioning.  In FP64 the reference evaluates the WENO smoothness indicators on
 stencil values (only its FP32 path subtracts the central value first).  With a large offset
 (T ≈ 20 °C varying by 1e-3, or u = 0.1 + 1e-3 noise) the quadratic forms cancel to ~1e-6 of
 their terms, so two FP64 evaluations that round differently differ by far more than 1e-12.  The
-FP64 kernels are therefore compiled without FMA contraction and evaluate every flux in the
-reference's order, which makes their tendencies bit-identical to the oracle's on identical
-inputs (test_tendencies_identical_inputs_fp64).  The remaining differences after a step come
+FP64 kernels therefore evaluate every flux in the reference's order, with fused multiply-adds
+exactly at the reference's @muladd sites (smoothness indicators, WENO weighted sums, Centered
+interpolants) and no other contraction, which makes their tendencies bit-identical to the
+oracle's on identical inputs (test_tendencies_identical_inputs_fp64).  The remaining differences after a step come
 from the Poisson solves (different FFT algorithms round differently) amplified by that
 conditioning: the offset cases carry the bound stated beside them.
 """

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oceananigans_mi355x as om
-from cases import convection, product_for, seamount
+from cases import convection, product_for, rel_err, seamount
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +98,35 @@ def test_checkpoint_pickup_continues_bit_for_bit(case, tmp_path):
     for n in a.tracers:
         np.testing.assert_array_equal(b.tracers[n].interior(), a.tracers[n].interior())
     np.testing.assert_array_equal(b.pressures.pNHS.interior(), a.pressures.pNHS.interior())
+
+
+@pytest.mark.parametrize("case", ["convection_fp64", "seamount_cg_fp64"])
+def test_gpu_checkpoint_resumes_in_the_oracle(case, tmp_path):
+    """A checkpoint written by the GPU model (parent arrays under the reference's JLD2 group names,
+    checkpointer.jl:7-165, nonhydrostatic_model.jl:354-375) is read by an independent reader, the
+    oracle, which continues from it; its state after 3 more steps equals the GPU model's."""
+    from cases import oracle_for
+    from oceananigans_mi355x.simulations import load_checkpoint_state
+    if case == "convection_fp64":
+        grid, kw, init = convection(N=(16, 16, 16), noise_velocity=1e-2)
+        dt, tol = 1.0, 1e-11
+    else:
+        grid, kw, init = seamount(N=(16, 16, 12))
+        dt, tol = 20.0, 1e-10
+    a = product_for(grid, kw, init)
+    sim = om.Simulation(a, Δt=dt, stop_iteration=3)
+    sim.output_writers["checkpointer"] = om.Checkpointer(a, om.IterationInterval(3), dir=str(tmp_path), prefix="cp")
+    om.run_(sim)
+    state = load_checkpoint_state(str(tmp_path / "cp_iteration3.npz"))
+    orc = oracle_for(grid, kw, {})
+    names = ("u", "v", "w") + tuple(kw["tracers"])
+    orc.load_state(state, names)
+    assert orc.clock()[0] == a.clock.time and orc.clock()[1] == 3
+    for _ in range(3):
+        om.time_step_(a, dt)
+    orc.time_step(dt, 3)
+    errs = {n: rel_err((a.velocities[n] if q < 3 else a.tracers[n]).interior(), orc.get(q)) for q, n in enumerate(names)}
+    print(case, "GPU checkpoint → oracle, 3 more steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) <= tol
+    assert orc.clock()[0] == pytest.approx(a.clock.time, rel=1e-14) and orc.clock()[1] == 6
+    a.close()

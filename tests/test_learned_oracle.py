@@ -75,3 +75,53 @@ def test_first_adam_step_closed_form():
     z = np.zeros(3, dtype=np.float32)
     new, m, v = L.adam(p, g, z, z, 1, 0.01)
     np.testing.assert_allclose(new, p - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-6)
+
+
+def _seamount_geometry():
+    import oceananigans_mi355x as om
+    from cases import seamount
+    from oracle import oracle as O
+    grid, kw, _ = seamount(N=(12, 10, 8))
+    cfg, keep = om.build_config(grid, advection=om.WENO(),
+                                pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=None, reltol=1e-14, abstol=0))
+    o = O.OracleModel(cfg, grid)
+    dzc, dzf, zc, zf = o.grid_z()
+    H, Nz = grid.H[2], grid.Nz
+    active = ~(zc[H:H + Nz][None, None, :] <= grid.bottom_heights()[:, :, None])
+    dx, dy = grid.extent[0] / grid.Nx, grid.extent[1] / grid.Ny
+    return grid, o, active, dx, dy, dzc[H:H + Nz], dzf[H:H + Nz]
+
+
+def test_energy_gradient_matches_finite_differences():
+    grid, o, active, dx, dy, dzc, dzf = _seamount_geometry()
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=grid.N)
+    E, g = L.energy(d, dx, dy, dzc, dzf, active)
+    assert E > 0
+    for _ in range(12):
+        idx = tuple(rng.integers(n) for n in grid.N)
+        h = 1e-6
+        dp, dm = d.copy(), d.copy()
+        dp[idx] += h
+        dm[idx] -= h
+        fd = (L.energy(dp, dx, dy, dzc, dzf, active)[0] - L.energy(dm, dx, dy, dzc, dzf, active)[0]) / (2 * h)
+        assert fd == pytest.approx(g[idx], rel=1e-6, abs=1e-9 * abs(E))
+    # constants have zero energy (the Neumann null space), inactive cells take no gradient
+    assert L.energy(np.full(grid.N, 3.0), dx, dy, dzc, dzf, active)[0] == pytest.approx(0, abs=1e-9)
+    assert np.all(g[~active] == 0)
+
+
+def test_energy_is_the_cg_operators_quadratic_form():
+    """For x solving the oracle's conditional CG problem V∇²x = V·rhs (immersed cells held at 0),
+    xᵀ(−V∇²)x = −xᵀV·rhs: the energy of the restatement equals the form of the operator the
+    oracle's CG inverts."""
+    grid, o, active, dx, dy, dzc, dzf = _seamount_geometry()
+    rng = np.random.default_rng(8)
+    rhs = rng.normal(size=grid.N)
+    rhs[~active] = 0
+    V = dx * dy * np.broadcast_to(dzc[None, None, :], grid.N)
+    rhs[active] -= (V[active] * rhs[active]).sum() / V[active].sum()
+    x = o.poisson_solve(rhs)
+    E, _ = L.energy(x, dx, dy, dzc, dzf, active)
+    want = -float((x * V * rhs)[active].sum())
+    assert E == pytest.approx(want, rel=1e-10)
