@@ -53,11 +53,19 @@ def run_gpu(args):
         kchunk = args.kchunk
     else:
         kchunk = model.autotune()            # z-chunk length of the tiled kernels, timed on this state
+    # per-phase device times: HIP events around every phase (this run issues every launch directly)
     model.set_phase_timing(True)
+    model.time_step(dt, args.steps)
+    phases = model.phase_times()
+    phased_ms = model.last_step_ms()
+    model.set_phase_timing(False)
+    # the headline: time_step as a caller runs it, pairs of steps replaying the model's captured
+    # two-step hipGraph (captured by the untimed pair first)
+    model.set_tuning(step_graphs=not args.no_step_graphs)
+    model.time_step(dt, 2)
     t0 = time.perf_counter()
     model.time_step(dt, args.steps)          # synchronous: returns after the device has finished
     t1 = time.perf_counter()
-    phases = model.phase_times()
     ev_ms = model.last_step_ms()
     wall = t1 - t0
     ncell = N[0] * N[1] * N[2]
@@ -75,7 +83,8 @@ def run_gpu(args):
     div = float(np.abs(model.divergence()).max())
     finite = all(np.isfinite(model.velocities[v].interior()).all() for v in "uvw")
     model.close()
-    return dict(N=N, kchunk=kchunk, ncell=ncell, wall=wall, ev_ms=ev_ms, phases=phases, dom=dom, dom_avg_ms=dom_avg_s * 1e3,
+    return dict(N=N, kchunk=kchunk, ncell=ncell, wall=wall, ev_ms=ev_ms, phased_ms=phased_ms, phases=phases, dom=dom,
+                dom_avg_ms=dom_avg_s * 1e3,
                 dom_bytes=dom_bytes, achieved=achieved, bytes_step=bytes_step, word=word, div=div, finite=finite)
 
 
@@ -189,10 +198,12 @@ def run_cg_seamount(args):
 
 
 def run_ensemble(args):
-    """Batched ensemble of uneven-size free-convection LES (FP32, WENO5 + AMD): members advanced one
-    after another (om_time_step per member, as the reference would) against all members advanced
-    together (one captured two-step hipGraph per member, members' streams in flight at once), after
-    the autotuner has picked each size's z-chunk length.  Total cell-updates/s of both."""
+    """Batched ensemble of uneven-size free-convection LES (FP32, WENO5 + AMD), after the autotuner
+    has picked each size's z-chunk length: all members advanced together in the batched form (one
+    launch per kernel of the step over the table of all members, each pair of steps one hipGraph
+    replay) for the configuration's 500 steps, against members advanced one after another
+    (om_time_step per member, as the reference would) and against per-member graphs on concurrent
+    streams, each timed on `--ensemble-compare-steps` steps.  Total cell-updates/s of each."""
     import oceananigans_mi355x as om
     from oceananigans_mi355x import workloads
 
@@ -214,19 +225,35 @@ def run_ensemble(args):
         else:
             m.set_tuning(kchunk=tuned[key])
     ncell = sum(int(np.prod(m.grid.N)) for m in members)
-    steps, dt = args.ensemble_steps, 1.0
+    steps, cmp_steps, dt = args.ensemble_steps, args.ensemble_compare_steps, 1.0
     for m in members:                       # warm-up: first steps and captured graphs
         m.time_step(dt, 1)
     ens = om.Ensemble(members)
-    ens.time_step(dt, 4)
+    ens.time_step(dt, 4, mode="member_graphs")
+    ens.time_step(dt, 4, mode="batched")
     t0 = time.perf_counter()
     for m in members:
-        m.time_step(dt, steps)
+        m.time_step(dt, cmp_steps)
     t_seq = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ens.time_step(dt, steps)
+    ens.time_step(dt, cmp_steps, mode="member_graphs")
+    t_graphs = time.perf_counter() - t0
+    by_groups = {}
+    for groups in (1, 2, 4, 8):             # member groups of the batched form, each graph on its own stream
+        ens.set_batch_groups(groups)
+        ens.time_step(dt, 2, mode="batched")  # records and captures
+        t0 = time.perf_counter()
+        ens.time_step(dt, cmp_steps, mode="batched")
+        by_groups[groups] = ncell * cmp_steps / (time.perf_counter() - t0)
+    best = max(by_groups, key=by_groups.get)
+    print(f"bench.py: ensemble compared over {cmp_steps} steps; batched run of {steps} steps in {best} groups next",
+          file=sys.stderr, flush=True)
+    ens.set_batch_groups(best)
+    ens.time_step(dt, 2, mode="batched")
+    t0 = time.perf_counter()
+    ens.time_step(dt, steps, mode="batched")
     t_ens = time.perf_counter() - t0
-    finite = all(np.isfinite(m.velocities.w.interior()).all() for m in members[:4])
+    finite = all(not m.velocities[n].hasnan() for m in members for n in ("u", "v", "w"))
     sizes = {}
     for m in members:
         k = "x".join(map(str, m.grid.N))
@@ -236,9 +263,13 @@ def run_ensemble(args):
         m.close()
     return {"members": len(members), "sizes": sizes, "cells": ncell, "steps": steps,
             "kchunk_by_size": {"x".join(map(str, k)): v for k, v in tuned.items()},
-            "one_after_another_cell_updates_per_s": ncell * steps / t_seq,
-            "together_cell_updates_per_s": ncell * steps / t_ens,
-            "speedup": t_seq / t_ens, "finite": bool(finite)}
+            "batched_cell_updates_per_s": ncell * steps / t_ens, "batched_wall_s": t_ens, "batched_groups": best,
+            "batched_cell_updates_per_s_by_groups": by_groups,
+            "compare_steps": cmp_steps,
+            "one_after_another_cell_updates_per_s": ncell * cmp_steps / t_seq,
+            "member_graphs_cell_updates_per_s": ncell * cmp_steps / t_graphs,
+            "batched_over_one_after_another": (ncell * steps / t_ens) / (ncell * cmp_steps / t_seq),
+            "finite": bool(finite)}
 
 
 def run_mixed(args):
@@ -294,11 +325,14 @@ def main():
                     help="leading CG iterations that apply the network (0: all)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble measurement")
+    ap.add_argument("--no-step-graphs", action="store_true",
+                    help="time the headline steps with every launch issued directly (no captured graph)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 512x512x256 mixed-precision wind-and-convection run")
     ap.add_argument("--mixed-size", default="512,512,256")
     ap.add_argument("--mixed-steps", type=int, default=10)
     ap.add_argument("--ensemble-members", type=int, default=64)
-    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-compare-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_free_convection_256x256x128_fp32.json"),
@@ -376,6 +410,8 @@ def main():
                           "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in r["phases"].items()},
         "device_ms_per_step": r["ev_ms"] / args.steps,
+        "step_graphs": not args.no_step_graphs,
+        "phase_timed_run_ms_per_step": r["phased_ms"] / args.steps,
         "max_abs_divergence": r["div"],
         "finite": r["finite"],
         "cpu_baseline": cpu,

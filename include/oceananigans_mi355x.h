@@ -289,6 +289,13 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
                                         correction, p += correction.  0 (default): off.  Not with
                                         the conjugate-gradient solver (its residuals are FP64
                                         already) or an immersed bottom. */
+#define OM_TUNE_STEP_GRAPHS 4      /* 1 (default): om_time_step advances pairs of steps by replaying the
+                                       model's two-step launch sequence captured once into a hipGraph
+                                       (the same launches; recaptured after a change of Δt or of a
+                                       launch choice); 0: every launch issued directly.  Phase timing
+                                       and the conjugate-gradient solver always launch directly. */
+#define OM_TUNE_CONV_TILED 5       /* convolutions of the learned preconditioner: 1 (default) input tiles
+                                       staged in LDS; 0 the input read per tap.  Identical results. */
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 
@@ -302,15 +309,29 @@ int om_autotune(om_model_t *m, int32_t *kchunk);
  * reference advances such members one after another, `time_step!` per model).  Members are
  * created with om_create, may differ in size and configuration, and stay owned by the caller;
  * the conjugate-gradient solver is not allowed (its steps read scalars on the host).
- * om_ensemble_time_step advances every member nsteps steps of Δt: each member's two-step launch
- * sequence is recorded once into a hipGraph on its own stream (use_graphs = 1) and replayed,
- * members issued in rounds so that their streams run concurrently; it returns when every member
- * is done.  use_graphs = 0 issues the same launches directly. */
+ * om_ensemble_time_step advances every member nsteps steps of Δt; `use_graphs` picks the form
+ * (OM_ENSEMBLE_*): member graphs (1) record each member's two-step launch sequence once into a
+ * hipGraph on its own stream and replay it, members issued in rounds so that their streams run
+ * concurrently; direct (0) issues the same launches member by member; batched (2) records every
+ * member's sequence, merges position p of all sequences into one launch over a device table of the
+ * members' arguments (each member's blocks a contiguous range of one grid), captures the merged
+ * sequence into one hipGraph on the ensemble's stream and replays it per pair of steps.  Members at
+ * iteration 0 take their first step on their own first.  It returns when every member is done. */
+#define OM_ENSEMBLE_DIRECT        0   /* every member's launches issued directly, member by member      */
+#define OM_ENSEMBLE_MEMBER_GRAPHS 1   /* each member's two-step sequence replayed as its own hipGraph    */
+#define OM_ENSEMBLE_BATCHED       2   /* one launch per kernel of the step covering every member (member
+                                         tables), the whole two-step sequence one hipGraph; members must
+                                         share one launch sequence (same precision, closure, buoyancy,
+                                         tracer count and an FFT solver on a uniform grid; sizes may differ) */
 typedef struct om_ensemble om_ensemble_t;
 int         om_ensemble_create(om_model_t *const *members, int32_t n, om_ensemble_t **out);
 int         om_ensemble_time_step(om_ensemble_t *e, double dt, int32_t nsteps, int32_t use_graphs);
 int         om_ensemble_destroy(om_ensemble_t *e);
 const char *om_ensemble_last_error(const om_ensemble_t *e);   /* NULL: the last create error */
+/* the batched form's member groups (1 .. 64, default 1): members are split in order into groups of
+ * about equal work, each group's member-table graph on its own stream, the groups' graphs running
+ * concurrently (no reference counterpart) */
+int         om_ensemble_set_batch_groups(om_ensemble_t *e, int32_t groups);
 
 #ifdef __cplusplus
 }

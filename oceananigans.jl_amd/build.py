@@ -17,11 +17,15 @@ FLAGS = [
 ]
 
 
-# (source, extra flags, object suffix): each .hip source is compiled once per precision; the FP64
-# object is built without FMA contraction so that its rounding follows the reference's.
+# (source, extra flags, object suffix): each .hip source is compiled once per precision.  The FP64
+# object is built without FMA contraction (fused multiply-adds only where the source writes fma, at
+# the reference's @muladd sites), so that its rounding follows the reference's.  The FP32 object
+# contracts a·b + c within one expression only: whether a product and a sum fuse is then decided
+# by the source, not by how a kernel's code is laid out, so one kernel body compiled into the
+# single-model launch and into the ensemble's member-table launch rounds identically.
 UNITS = [("om_api.cpp", [], "")]
 for _s in SOURCES[1:]:
-    UNITS.append((_s, ["-DOM_ONLY_FLOAT"], "_f32"))
+    UNITS.append((_s, ["-DOM_ONLY_FLOAT", "-ffp-contract=on"], "_f32"))
     UNITS.append((_s, ["-DOM_ONLY_DOUBLE", "-ffp-contract=off"], "_f64"))
 
 

@@ -19,10 +19,18 @@
 #include <vector>
 
 #include "om_internal.h"
+#include "om_batch.h"
 #include "om_kernels.h"
 #include "om_learned.h"
 
 using namespace om;
+
+namespace om {
+Recorder *&thread_recorder() {
+    static thread_local Recorder *r = nullptr;
+    return r;
+}
+}  // namespace om
 
 namespace {
 
@@ -152,6 +160,7 @@ template <class T> struct ModelT : ModelBase {
     StateView<T> s;
     PoissonPlan<T> P;
     hipStream_t st = nullptr;
+    int device = 0;                    // the device current at create; the handle's stream lives there
     int nt = 0, nf = 0;
     bool has_pHY = false;
     int pnx = 0, pny = 0, pnz = 0;     // parent array size (pnz: one more for w on a Bounded z)
@@ -203,6 +212,7 @@ template <class T> struct ModelT : ModelBase {
     int tune_kchunk = 16;              // z levels per block of the tiled tendency kernel
     int tune_kchunk_closure = 16;      // z levels per block of the tiled closure kernel
     int tune_learned_iterations = 0;   // leading CG iterations that apply the learned correction (0: all)
+    int tune_step_graphs = 1;          // om_time_step replays captured two-step graphs
     int tune_refine = 0;               // FP64 iterative refinement of FFT / Fourier–tridiagonal solves
     RefineOp refop{};                  // the solver's operator for the refinement residual
     T *ref_r = nullptr, *ref_d = nullptr;   // dense residual and correction
@@ -281,6 +291,7 @@ template <class T> struct ModelT : ModelBase {
         nt = c.n_tracers;
         nf = 3 + nt;
         make_grid(hg, c);
+        HIPCHECK(hipGetDevice(&device));
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&ev0));
         HIPCHECK(hipEventCreate(&ev1));
@@ -879,7 +890,9 @@ template <class T> struct ModelT : ModelBase {
     int graph_parity = 0;
     int swap_parity = 0;   // parity of the G⁻/Gⁿ exchanges so far
     bool capturable() const { return !cg; }
+    int launch_version = 0;   // changes with every change of a launch choice (tuning, autotuner)
     void drop_graph() {
+        ++launch_version;
         if (graph_exec) { HIPCHECK(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
     }
     void capture(double dt) {
@@ -906,6 +919,37 @@ template <class T> struct ModelT : ModelBase {
         step_clock(dt);
         step_clock(dt);
         return 2;
+    }
+
+    // ---- member tables: the launches of two steps of Δt (the sequence a replay repeats) are
+    // recorded into R instead of run.  The model's stream is captured meanwhile, so a launch that
+    // has no member-table form (it bypasses the recorder) lands in the captured graph, not on the
+    // device, and is reported.
+    void record_two_steps(double dt, Recorder &R) {
+        if (!capturable()) throw ArgError{"the conjugate-gradient solver's steps read scalars on the host"};
+        if (iteration == 0) throw ArgError{"a model's first step is not recorded (it begins with update_state)"};
+        HIPCHECK(hipStreamSynchronize(st));
+        const bool pt = phase_timing;
+        phase_timing = false;
+        hipGraph_t gr = nullptr;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        thread_recorder() = &R;
+        bool threw = false;
+        try {
+            step_launches(dt);
+            step_launches(dt);
+        } catch (...) {
+            threw = true;
+        }
+        thread_recorder() = nullptr;
+        phase_timing = pt;
+        const hipError_t ec = hipStreamEndCapture(st, &gr);
+        size_t nodes = 0;
+        if (ec == hipSuccess && gr) (void)hipGraphGetNodes(gr, nullptr, &nodes);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (threw) throw ArgError{"recording the step failed"};
+        HIPCHECK(ec);
+        if (nodes != 0) throw ArgError{"a launch of this configuration's step has no member-table form"};
     }
 
     // ---- autotuner: the z-chunk length of the tiled kernels, timed on this model's own state
@@ -1219,7 +1263,10 @@ int om_time_step(om_model_t *m, double dt, int32_t nsteps) {
         M->evused = 0;
         M->cg_unconverged = 0;
         HIPCHECK(hipEventRecord(M->ev0, M->st));
-        for (int q = 0; q < nsteps; ++q) M->time_step(dt);
+        // pairs of steps replay the model's captured two-step graph (the same launches, so the
+        // same results) unless phases are being timed or the graph form is switched off
+        const bool graphs = M->tune_step_graphs && !M->phase_timing;
+        for (int left = nsteps; left > 0;) left -= M->issue_steps(dt, left, graphs);
         HIPCHECK(hipEventRecord(M->ev1, M->st));
         HIPCHECK(hipEventSynchronize(M->ev1));
         HIPCHECK(hipGetLastError());
@@ -1494,7 +1541,7 @@ int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *f
 
 static bool known_tuning_key(int32_t key) {
     return key == OM_TUNE_TENDENCY_KERNEL || key == OM_TUNE_KCHUNK || key == OM_TUNE_LEARNED_ITERATIONS ||
-           key == OM_TUNE_POISSON_REFINEMENT;
+           key == OM_TUNE_POISSON_REFINEMENT || key == OM_TUNE_STEP_GRAPHS || key == OM_TUNE_CONV_TILED;
 }
 
 int om_autotune(om_model_t *m, int32_t *kchunk) {
@@ -1506,10 +1553,147 @@ int om_autotune(om_model_t *m, int32_t *kchunk) {
     });
 }
 
+// The member-table form of an ensemble step: position p of every member's recorded two-step
+// launch sequence becomes one launch over a device table of the members' arguments, and the whole
+// sequence one hipGraph.  The members may be split into groups, each with its own tables, graph and
+// stream, so that the groups' graphs run concurrently and one group's bandwidth-bound phases
+// overlap another's compute-bound ones.
+struct BatchGroup {
+    std::vector<size_t> members;      // indices into the ensemble's member list
+    std::vector<void *> tables;       // device argument tables, one per position
+    hipGraphExec_t exec = nullptr;
+    hipStream_t st = nullptr;
+};
+
+struct BatchPlan {
+    double dt = 0;
+    int groups_wanted = 1;
+    std::vector<int> parity;          // each member's G⁻/Gⁿ exchange parity when recorded
+    std::vector<int> version;         // each member's launch_version when recorded
+    std::vector<BatchGroup> groups;
+    int positions = 0;
+    bool ready() const { return !groups.empty() && groups[0].exec; }
+    void clear() {
+        for (BatchGroup &g : groups) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            for (void *t : g.tables) (void)hipFree(t);
+            if (g.st) (void)hipStreamDestroy(g.st);
+        }
+        groups.clear();
+        parity.clear();
+        version.clear();
+        positions = 0;
+    }
+    ~BatchPlan() { clear(); }
+};
+
 struct om_ensemble {
     std::vector<om_model_t *> members;
     std::string err;
+    BatchPlan plan;
 };
+
+namespace {
+int member_parity(om_model_t *m) {
+    int p = 0;
+    (void)WITH_MODEL(m, { p = M->swap_parity; });
+    return p;
+}
+int member_version(om_model_t *m) {
+    int v = 0;
+    (void)WITH_MODEL(m, { v = M->launch_version; });
+    return v;
+}
+
+// records every member's two-step sequence, checks that the sequences match launch for launch,
+// splits the members into groups of about equal block counts, and per group uploads the member
+// tables and captures the batched launches into one graph on the group's stream
+void build_plan(om_ensemble *e, double dt) {
+    BatchPlan &P = e->plan;
+    P.clear();
+    const size_t n = e->members.size();
+    std::vector<Recorder> rec(n);
+    int device = -1;
+    for (size_t q = 0; q < n; ++q) {
+        om_model_t *m = e->members[q];
+        int dev = -1;
+        const int rc = WITH_MODEL(m, { M->record_two_steps(dt, rec[q]); dev = M->device; });
+        if (rc != OM_OK) throw ArgError{"member " + std::to_string(q) + ": " + m->err};
+        if (q == 0) device = dev;
+        else if (dev != device) throw ArgError{"member " + std::to_string(q) + " lives on another device"};
+    }
+    const size_t npos = rec[0].recs.size();
+    for (size_t q = 1; q < n; ++q) {
+        if (rec[q].recs.size() != npos) throw ArgError{"member " + std::to_string(q) + "'s step has another launch sequence than member 0's"};
+        for (size_t p = 0; p < npos; ++p) {
+            const LaunchRec &a = rec[0].recs[p], &b = rec[q].recs[p];
+            if (a.kind != b.kind || a.block.x != b.block.x || a.block.y != b.block.y || a.block.z != b.block.z)
+                throw ArgError{"member " + std::to_string(q) + "'s step has another launch sequence than member 0's"};
+        }
+    }
+    // groups: members in order, each group closed once it holds its share of the blocks
+    const int G = std::max(1, std::min<int>(P.groups_wanted, (int)n));
+    std::vector<uint64_t> work(n, 0);
+    uint64_t total = 0;
+    for (size_t q = 0; q < n; ++q) {
+        for (const LaunchRec &r : rec[q].recs) work[q] += (uint64_t)r.grid.x * r.grid.y * r.grid.z;
+        total += work[q];
+    }
+    P.groups.resize(G);
+    {
+        uint64_t acc = 0;
+        int gi = 0;
+        for (size_t q = 0; q < n; ++q) {
+            if (gi < G - 1 && acc >= (uint64_t)(gi + 1) * total / G && !P.groups[gi].members.empty()) ++gi;
+            P.groups[gi].members.push_back(q);
+            acc += work[q];
+        }
+        while (!P.groups.empty() && P.groups.back().members.empty()) P.groups.pop_back();
+    }
+    for (BatchGroup &grp : P.groups) {
+        HIPCHECK(hipStreamCreateWithFlags(&grp.st, hipStreamNonBlocking));
+        const int gn = (int)grp.members.size();
+        std::vector<uint32_t> blocks(npos);
+        std::vector<size_t> lds(npos);
+        for (size_t p = 0; p < npos; ++p) {
+            const size_t es = rec[0].recs[p].entry_size;
+            std::vector<unsigned char> host(es * gn);
+            uint64_t begin = 0;
+            size_t l = 0;
+            for (int q = 0; q < gn; ++q) {
+                const LaunchRec &r = rec[grp.members[q]].recs[p];
+                const uint64_t nb = (uint64_t)r.grid.x * r.grid.y * r.grid.z;
+                r.fill(host.data() + q * es, r, (uint32_t)begin);
+                begin += (nb + 7) & ~uint64_t(7);   // member ranges start on a multiple of 8 (XCD residue kept)
+                l = std::max(l, r.lds);
+            }
+            if (begin > 0x7fffffffull) throw ArgError{"the ensemble's launch has more than 2^31 blocks"};
+            void *t = nullptr;
+            HIPCHECK(hipMalloc(&t, host.size()));
+            grp.tables.push_back(t);
+            HIPCHECK(hipMemcpy(t, host.data(), host.size(), hipMemcpyHostToDevice));
+            blocks[p] = (uint32_t)begin;
+            lds[p] = l;
+        }
+        hipGraph_t gr = nullptr;
+        HIPCHECK(hipStreamBeginCapture(grp.st, hipStreamCaptureModeThreadLocal));
+        for (size_t p = 0; p < npos; ++p) {
+            const LaunchRec &r = rec[grp.members[0]].recs[p];
+            r.launch_many(grp.tables[p], gn, blocks[p], r.block, lds[p], grp.st);
+        }
+        HIPCHECK(hipStreamEndCapture(grp.st, &gr));
+        const hipError_t ei = hipGraphInstantiate(&grp.exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHECK(ei);
+    }
+    P.dt = dt;
+    P.positions = (int)npos;
+    for (size_t q = 0; q < n; ++q) {
+        P.parity.push_back(member_parity(e->members[q]));
+        P.version.push_back(member_version(e->members[q]));
+    }
+}
+}  // namespace
 
 int om_ensemble_create(om_model_t *const *members, int32_t n, om_ensemble_t **out) {
     if (!out || !members || n < 1) return OM_EINVAL;
@@ -1538,8 +1722,76 @@ int om_ensemble_destroy(om_ensemble_t *e) {
     return OM_OK;
 }
 
+// OM_ENSEMBLE_BATCHED: members at iteration 0 take their first step on their own; then every
+// pair of steps of all members is one replay of the member-table graph; a remainder (members with
+// an odd count) is stepped member by member
+static int ensemble_batched(om_ensemble_t *e, double dt, int32_t nsteps) {
+    const size_t n = e->members.size();
+    std::vector<int> left(n, nsteps);
+    auto each = [&](auto &&fn) -> int {
+        for (size_t q = 0; q < n; ++q) {
+            om_model_t *m = e->members[q];
+            const int rc = fn(q, m);
+            if (rc != OM_OK) { e->err = "member " + std::to_string(q) + ": " + m->err; return rc; }
+        }
+        return OM_OK;
+    };
+    int rc = each([&](size_t q, om_model_t *m) {
+        return WITH_MODEL(m, {
+            if (left[q] > 0 && M->iteration == 0) { M->time_step(dt); left[q] -= 1; }
+            HIPCHECK(hipStreamSynchronize(M->st));
+        });
+    });
+    if (rc != OM_OK) return rc;
+    int pairs = nsteps;
+    for (size_t q = 0; q < n; ++q) pairs = std::min(pairs, left[q] / 2);
+    if (pairs > 0) {
+        BatchPlan &P = e->plan;
+        bool stale = !P.ready() || P.dt != dt || P.parity.size() != n;
+        for (size_t q = 0; q < n && !stale; ++q)
+            stale = member_parity(e->members[q]) != P.parity[q] || member_version(e->members[q]) != P.version[q];
+        if (stale) {
+            try {
+                build_plan(e, dt);
+            } catch (const ArgError &a) {
+                e->plan.clear();
+                e->err = a.msg;
+                return OM_EINVAL;
+            } catch (const HipError &h) {
+                e->plan.clear();
+                e->err = std::string(h.what) + ": " + hipGetErrorString(h.e);
+                return OM_EHIP;
+            }
+        }
+        for (int r = 0; r < pairs; ++r) {
+            for (BatchGroup &grp : P.groups) {
+                const hipError_t he = hipGraphLaunch(grp.exec, grp.st);
+                if (he != hipSuccess) { e->err = std::string("hipGraphLaunch: ") + hipGetErrorString(he); return OM_EHIP; }
+            }
+        }
+        for (BatchGroup &grp : P.groups) {
+            const hipError_t hs = hipStreamSynchronize(grp.st);
+            if (hs != hipSuccess) { e->err = std::string("ensemble stream: ") + hipGetErrorString(hs); return OM_EHIP; }
+        }
+        rc = each([&](size_t q, om_model_t *m) {
+            return WITH_MODEL(m, { for (int r = 0; r < 2 * pairs; ++r) M->step_clock(dt); });
+        });
+        if (rc != OM_OK) return rc;
+        for (size_t q = 0; q < n; ++q) left[q] -= 2 * pairs;
+    }
+    return each([&](size_t q, om_model_t *m) {
+        return WITH_MODEL(m, {
+            for (int r = 0; r < left[q]; ++r) M->time_step(dt);
+            HIPCHECK(hipStreamSynchronize(M->st));
+            HIPCHECK(hipGetLastError());
+        });
+    });
+}
+
 int om_ensemble_time_step(om_ensemble_t *e, double dt, int32_t nsteps, int32_t use_graphs) {
     if (!e || nsteps < 0) return OM_EINVAL;
+    if (use_graphs < OM_ENSEMBLE_DIRECT || use_graphs > OM_ENSEMBLE_BATCHED) return OM_EINVAL;
+    if (use_graphs == OM_ENSEMBLE_BATCHED) return ensemble_batched(e, dt, nsteps);
     // members advance in rounds of up to two steps each, so every member's stream holds work
     // at once; then every stream is drained
     std::vector<int> left(e->members.size(), nsteps);
@@ -1569,17 +1821,30 @@ int om_ensemble_time_step(om_ensemble_t *e, double dt, int32_t nsteps, int32_t u
 
 const char *om_ensemble_last_error(const om_ensemble_t *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 
+int om_ensemble_set_batch_groups(om_ensemble_t *e, int32_t groups) {
+    if (!e || groups < 1 || groups > 64) return OM_EINVAL;
+    if (groups != e->plan.groups_wanted) {
+        e->plan.clear();
+        e->plan.groups_wanted = groups;
+    }
+    return OM_OK;
+}
+
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (!m) return OM_EINVAL;
     if (key == OM_TUNE_TENDENCY_KERNEL && (value < 0 || value > 1)) return OM_EINVAL;
     if (key == OM_TUNE_KCHUNK && (value < 1 || value > 1 << 20)) return OM_EINVAL;
     if (key == OM_TUNE_LEARNED_ITERATIONS && value < 0) return OM_EINVAL;
     if (key == OM_TUNE_POISSON_REFINEMENT && (value < 0 || value > 1)) return OM_EINVAL;
+    if (key == OM_TUNE_STEP_GRAPHS && (value < 0 || value > 1)) return OM_EINVAL;
+    if (key == OM_TUNE_CONV_TILED && (value < 0 || value > 1)) return OM_EINVAL;
     if (!known_tuning_key(key)) return OM_EINVAL;
     return WITH_MODEL(m, {
         if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
         else if (key == OM_TUNE_KCHUNK) M->tune_kchunk = M->tune_kchunk_closure = value;
         else if (key == OM_TUNE_LEARNED_ITERATIONS) M->tune_learned_iterations = value;
+        else if (key == OM_TUNE_STEP_GRAPHS) M->tune_step_graphs = value;
+        else if (key == OM_TUNE_CONV_TILED) M->net.tiled_conv = value;
         else M->set_refinement(value);
         M->drop_graph();   // a captured step holds the launch choices it was recorded with
     });
@@ -1593,6 +1858,8 @@ int om_get_tuning(const om_model_t *mc, int32_t key, int32_t *value) {
         *value = key == OM_TUNE_TENDENCY_KERNEL      ? M->tune_tendency_kernel
                : key == OM_TUNE_KCHUNK               ? M->tune_kchunk
                : key == OM_TUNE_LEARNED_ITERATIONS   ? M->tune_learned_iterations
+               : key == OM_TUNE_STEP_GRAPHS          ? M->tune_step_graphs
+               : key == OM_TUNE_CONV_TILED           ? M->net.tiled_conv
                                                      : M->tune_refine;
     });
 }

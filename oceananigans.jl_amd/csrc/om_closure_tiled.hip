@@ -10,6 +10,7 @@
 // (om_kernels.hip), so the two kernels give identically rounded results.
 #include <hip/hip_runtime.h>
 
+#include "om_batch.h"
 #include "om_kernels.h"
 
 namespace om {
@@ -33,15 +34,27 @@ template <class T> __device__ __forceinline__ void stage_ring(T *dst, const T *f
     }
 }
 
+template <class T> struct ClosureTiledArgs {
+    Ctx<T> C;
+    int kchunk, ntx;
+};
+
+template <class T, int NT> struct KClosureTiled {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const ClosureTiledArgs<T> &A, const Blk &B);
+};
+
 template <class T, int NT>
-__global__ void __launch_bounds__(256) k_closure_tiled(Ctx<T> C, int kchunk, int ntx) {
+__device__ __forceinline__ void KClosureTiled<T, NT>::run(const ClosureTiledArgs<T> &A, const Blk &B) {
     __shared__ ClosureSmem<T, NT> S;
+    const Ctx<T> &C = A.C;
+    const int kchunk = A.kchunk, ntx = A.ntx;
     const DevGrid<T> &g = C.g;
     const DevPhysics<T> &ph = C.ph;
     const int tid = threadIdx.x;
     const int a = tid & (CT - 1), b = tid >> 4;
     // work item → (z chunk, tile): the chunks of one tile are consecutive, tiles in row order
-    const int item = xcd_work_item((int)blockIdx.x, (int)gridDim.x);
+    const int item = xcd_work_item((int)B.x, (int)B.gx);
     const int nch = (g.Nz + kchunk - 1) / kchunk;
     const int tile = item / nch, chunk = item - tile * nch;
     const int tx = tile % ntx, ty = tile / ntx;
@@ -259,10 +272,11 @@ template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStr
     if (kchunk < 1) kchunk = 1;
     const int nch = (g.Nz + kchunk - 1) / kchunk;
     const dim3 grid(ntx * nty * nch), blk(CT * CT);
+    const ClosureTiledArgs<T> A{C, kchunk, ntx};
     switch (C.ph.n_tracers) {
-        case 0: hipLaunchKernelGGL((k_closure_tiled<T, 0>), grid, blk, 0, st, C, kchunk, ntx); break;
-        case 1: hipLaunchKernelGGL((k_closure_tiled<T, 1>), grid, blk, 0, st, C, kchunk, ntx); break;
-        default: hipLaunchKernelGGL((k_closure_tiled<T, 2>), grid, blk, 0, st, C, kchunk, ntx); break;
+        case 0: launch_k<KClosureTiled<T, 0>>(grid, blk, 0, st, A); break;
+        case 1: launch_k<KClosureTiled<T, 1>>(grid, blk, 0, st, A); break;
+        default: launch_k<KClosureTiled<T, 2>>(grid, blk, 0, st, A); break;
     }
 }
 

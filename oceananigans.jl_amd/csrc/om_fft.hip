@@ -17,6 +17,7 @@
 #include <stdexcept>
 
 #include "om_internal.h"
+#include "om_batch.h"
 #include "om_kernels.h"
 
 namespace om {
@@ -189,7 +190,11 @@ template <class T> __device__ __forceinline__ T source_at(const Ctx<T> &C, const
     return C.iVc(k) * ((sx + sy) + sz);
 }
 
-template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArgs<T> A) {
+template <class T> struct KXFwd {
+    static constexpr int kThreads = kFftThreads, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const XFwdArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KXFwd<T>::run(const XFwdArgs<T> &A, const Blk &B) {
     extern __shared__ unsigned char smem_raw[];
     const Ctx<T> &C = A.C;
     const int Nx = C.g.Nx, Ny = C.g.Ny;
@@ -200,11 +205,54 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArg
     C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *a = twl + M;
     C2<T> *b = a + R * ld;
-    const int row0 = blockIdx.x * R;
+    const int row0 = B.x * R;
     const int nrows = Ny * C.g.Nz;
     load_table<T>(twl, A.tw, M);
     T *ap = reinterpret_cast<T *>(a);
-    {
+    // without an immersed bottom the source is ∇·U of three non-Flat axes on a whole row: each
+    // thread forms its (at most kFastX) elements from loads all issued before any arithmetic,
+    // with no per-element branch (the general path below tests the topology per element)
+    constexpr int kFastX = 8;
+    const bool fast = !A.src && !C.g.kbot && !C.flx() && !C.fly() && !C.flz() && half && !A.scale_dzc &&
+                      Nx <= kFastX * (int)(blockDim.x >> A.L.lgR);
+    if (fast) {
+        const int lgt = __builtin_ctz((unsigned)blockDim.x) - A.L.lgR;   // log2 of threads per row
+        const int tpr = 1 << lgt;
+        const int r = threadIdx.x >> lgt, l = threadIdx.x & (tpr - 1);
+        const int row = row0 + r;
+        const bool live = row < nrows;
+        const int k = live ? (int)fdiv((uint32_t)row, A.fNy) : 0, j = live ? row - k * Ny : 0;
+        const int jp = (j + 1 == Ny) ? 0 : j + 1;
+        const int64_t o = C.at(0, j, k);
+        const T *u = C.s.U[0] + o, *v = C.s.U[1] + o, *w = C.s.U[2] + o;
+        const T *vp = C.s.U[1] + C.at(0, jp, k), *wp = w + C.g.sz;
+        T su[kFastX], sq[kFastX], sv[kFastX], svp[kFastX], sw[kFastX], swp[kFastX];
+#pragma unroll
+        for (int e = 0; e < kFastX; ++e) {
+            const int i = l + e * tpr;
+            const bool in = live && i < Nx;
+            const int ii = in ? i : 0;
+            const int ip = (ii + 1 == Nx) ? 0 : ii + 1;
+            su[e] = in ? u[ii] : T(0);
+            sq[e] = in ? u[ip] : T(0);
+            sv[e] = in ? v[ii] : T(0);
+            svp[e] = in ? vp[ii] : T(0);
+            sw[e] = in ? w[ii] : T(0);
+            swp[e] = in ? wp[ii] : T(0);
+        }
+        const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az, iV = C.iVc(k);
+#pragma unroll
+        for (int e = 0; e < kFastX; ++e) {
+            const int i = l + e * tpr;
+            if (i < Nx) {
+                // Ctx::divU's expression (source_at), in its order of operations
+                const T sx = Ax * sq[e] - Ax * su[e];
+                const T sy = Ay * svp[e] - Ay * sv[e];
+                const T sz = Az * swp[e] - Az * sw[e];
+                ap[2 * (r * ld + (i >> 1)) + (i & 1)] = iV * ((sx + sy) + sz);
+            }
+        }
+    } else {
         // blockDim.x / R threads per row: a thread's row (j, k), its metric factors and addresses are
         // fixed, and its elements i = l, l + tpr, … differ by constant offsets; the loads of four
         // elements are issued before their LDS stores.  Rows past the last stage zeros.
@@ -215,11 +263,11 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArg
         const bool live = row < nrows;
         const int k = live ? (int)fdiv((uint32_t)row, A.fNy) : 0, j = live ? row - k * Ny : 0;
         const T zs = A.scale_dzc && live ? C.g.dzc[k] : T(1);
-        constexpr int B = 4;
-        for (int i0 = l; i0 < Nx; i0 += B * tpr) {
-            T x[B];
+        constexpr int NB = 4;
+        for (int i0 = l; i0 < Nx; i0 += NB * tpr) {
+            T x[NB];
 #pragma unroll
-            for (int u = 0; u < B; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 const int i = i0 + u * tpr;
                 x[u] = T(0);
                 if (live && i < Nx) {
@@ -228,7 +276,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xfwd(XFwdArg
                 }
             }
 #pragma unroll
-            for (int u = 0; u < B; ++u) {
+            for (int u = 0; u < NB; ++u) {
                 const int i = i0 + u * tpr;
                 if (i < Nx) {
                     if (half) ap[2 * (r * ld + (i >> 1)) + (i & 1)] = x[u];
@@ -270,7 +318,11 @@ template <class T> struct XBwdArgs {
     int Ms;
 };
 
-template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArgs<T> A) {
+template <class T> struct KXBwd {
+    static constexpr int kThreads = kFftThreads, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const XBwdArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KXBwd<T>::run(const XBwdArgs<T> &A, const Blk &B) {
     extern __shared__ unsigned char smem_raw[];
     const int Nx = A.g.Nx, Ny = A.g.Ny;
     const bool half = (Nx & 1) == 0;
@@ -280,7 +332,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_xbwd(XBwdArg
     C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *a = twl + M;
     C2<T> *b = a + R * ld;
-    const int row0 = blockIdx.x * R;
+    const int row0 = B.x * R;
     const int nrows = Ny * A.g.Nz;
     load_table<T>(twl, A.tw, M);
     // blockDim.x / R threads per row, as in k_xfwd: a thread's row and its addresses are fixed
@@ -356,7 +408,11 @@ template <class T> struct YArgs {
     FastDiv fMs;        // division by the row stride Ms
 };
 
-template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<T> A) {
+template <class T> struct KYPass {
+    static constexpr int kThreads = kFftThreads, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const YArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KYPass<T>::run(const YArgs<T> &A, const Blk &B) {
     extern __shared__ unsigned char smem_raw[];
     const int Ny = A.g.Ny, Mx = A.Mx, R = A.L.R, lgR = A.L.lgR, ld = A.L.ld;
     C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
@@ -366,7 +422,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_ypass(YArgs<
     const int nlines = Ms * A.g.Nz;
     // blockDim.x is a multiple of R, so every element a thread stages belongs to the same line
     const int r = threadIdx.x & (R - 1);
-    const int line = blockIdx.x * R + r;
+    const int line = B.x * R + r;
     const int k = (int)fdiv((uint32_t)line, A.fMs), kx = line - k * Ms;
     const bool valid = line < nlines && kx < Mx;                         // padding lines: nothing
     C2<T> *col = A.S + (int64_t)k * Ny * Ms + kx;
@@ -415,7 +471,11 @@ template <class T> struct ZArgs {
     LineGeom G;
 };
 
-template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<T> A) {
+template <class T> struct KZPass {
+    static constexpr int kThreads = kFftThreads, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const ZArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KZPass<T>::run(const ZArgs<T> &A, const Blk &B) {
     extern __shared__ unsigned char smem_raw[];
     const int Ny = A.g.Ny, Nz = A.g.Nz, Mx = A.Mx, L = A.L, zmode = A.zmode;
     const int R = A.G.R, lgR = A.G.lgR, ld = A.G.ld;
@@ -426,7 +486,7 @@ template <class T> __global__ void __launch_bounds__(kFftThreads) k_zpass(ZArgs<
     const int Ms = A.Ms;
     const int nlines = Ms * Ny;
     const int r = threadIdx.x & (R - 1);   // blockDim.x is a multiple of R
-    const int line = blockIdx.x * R + r;
+    const int line = B.x * R + r;
     const int j = line < nlines ? line / Ms : 0, kx = line - j * Ms;
     const bool valid = line < nlines && kx < Mx;   // padding lines: nothing
     const int64_t plane = (int64_t)Ny * Ms;
@@ -684,7 +744,7 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     {
         const LineGeom L = line_geom(M, M, e, lds);
         XFwdArgs<T> A{C, src, P.S, P.tw_xh, P.post, P.tridiagonal, L, make_fastdiv((uint32_t)Ny), make_fastdiv((uint32_t)Mx), P.Ms};
-        hipLaunchKernelGGL(k_xfwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
+        launch_k<KXFwd<T>>(dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
     const bool z_active = g.tz != OM_FLAT;
     const LineGeom Ly = line_geom(Ny, Ny, e, lds);
@@ -692,7 +752,7 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     const int ygrid = (P.Ms * Nz + Ly.R - 1) / Ly.R;
     {
         YArgs<T> A{g, P.S, P.tw_y, Mx, P.Ms, 0, z_active ? 0 : 1, P.lx, P.ly, Ly, make_fastdiv((uint32_t)P.Ms)};
-        hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
+        launch_k<KYPass<T>>(dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     T scale = T(1) / (T(M) * T(Ny));
     if (z_active) {
@@ -710,16 +770,16 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
         } else {
             const LineGeom Lz = line_geom(P.Lz_line, P.Lz_line + (P.zmode == 1 ? Nz : 0), e, lds);
             ZArgs<T> A{g, P.S, P.tw_z, P.tw_dct, Mx, P.Ms, P.Lz_line, P.zmode, P.lx, P.ly, P.lz, Lz};
-            hipLaunchKernelGGL(k_zpass<T>, dim3((P.Ms * Ny + Lz.R - 1) / Lz.R), dim3(kFftThreads), lds, st, A);
+            launch_k<KZPass<T>>(dim3((P.Ms * Ny + Lz.R - 1) / Lz.R), dim3(kFftThreads), lds, st, A);
             scale = T(1) / ((T(M) * T(Ny)) * T(P.Lz_norm));
         }
         YArgs<T> A{g, P.S, P.tw_y, Mx, P.Ms, 1, 0, P.lx, P.ly, Ly, make_fastdiv((uint32_t)P.Ms)};
-        hipLaunchKernelGGL(k_ypass<T>, dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
+        launch_k<KYPass<T>>(dim3(ygrid), dim3(kFftThreads), lds_y, st, A);
     }
     {
         const LineGeom L = line_geom(M, M, e, lds);
         XBwdArgs<T> A{g, P.S, P.tw_xh, P.post, out, dense_out, scale, divisor, L, make_fastdiv((uint32_t)Ny), P.Ms};
-        hipLaunchKernelGGL(k_xbwd<T>, dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
+        launch_k<KXBwd<T>>(dim3((nrows + L.R - 1) / L.R), dim3(kFftThreads), lds, st, A);
     }
 }
 

@@ -617,6 +617,11 @@ template <class T> struct HaloJob {
     int nf;
 };
 
+// the argument pack of a kernel that takes only the model context
+template <class T> struct CtxArgs {
+    Ctx<T> C;
+};
+
 template <class T> struct RKArgs {
     T dt, gamma, zeta;
     int first;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "om_internal.h"
+#include "om_batch.h"
 #include "om_kernels.h"
 
 namespace om {
@@ -262,12 +263,17 @@ template <class T> __global__ void __launch_bounds__(256) k_closure(Ctx<T> C, in
 // ----------------------------------------------------------------------------------------
 // Hydrostatic pressure anomaly: downward column integral (update_hydrostatic_pressure.jl:11-21)
 // ----------------------------------------------------------------------------------------
-template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C) {
+template <class T> struct KHydrostatic {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const CtxArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KHydrostatic<T>::run(const CtxArgs<T> &A, const Blk &Bk) {
     // The column sum cancels between neighbouring columns when ∂x pHY′ is formed; it is
     // accumulated without FMA contraction, in the reference's order.
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const Ctx<T> &C = A.C;
+    const int i = Bk.x * blockDim.x + threadIdx.x;
+    const int j = Bk.y * blockDim.y + threadIdx.y;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     const int Nz = C.g.Nz;
     T bk = C.buoy(i, j, Nz);                      // halo value b[Nz+1] (0-based index Nz)
@@ -311,10 +317,15 @@ template <class T> __global__ void __launch_bounds__(256) k_hydrostatic(Ctx<T> C
 // associated in segments, so FP32 results differ from the sequential order by rounding only.
 constexpr int kHySegLevels = 64;   // levels per segment held in registers (Nz ≤ 4·64)
 
-template <class T> __global__ void __launch_bounds__(256) k_hydrostatic_seg(Ctx<T> C) {
+template <class T> struct KHydrostaticSeg {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const CtxArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KHydrostaticSeg<T>::run(const CtxArgs<T> &A, const Blk &B) {
     __shared__ T tot[4][64];
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y;
+    const Ctx<T> &C = A.C;
+    const int i = B.x * 64 + threadIdx.x;
+    const int j = B.y;
     const int seg = threadIdx.y;
     const int Nz = C.g.Nz;
     const int per = (Nz + 3) / 4;
@@ -468,11 +479,23 @@ __device__ __forceinline__ T halo_zvalue(const DevGrid<T> &g, const HaloJob<T> &
     return c[col + (int64_t)k * g.sz];
 }
 
-template <class T> __global__ void __launch_bounds__(256) k_halo_fused(DevGrid<T> g, HaloJob<T> J, int64_t n_ring, int64_t n_z) {
-    const int n = blockIdx.y;
+template <class T> struct HaloArgs {
+    DevGrid<T> g;
+    HaloJob<T> J;
+    int64_t n_ring, n_z;
+};
+template <class T> struct KHaloFused {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const HaloArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KHaloFused<T>::run(const HaloArgs<T> &A, const Blk &B) {
+    const DevGrid<T> &g = A.g;
+    const HaloJob<T> &J = A.J;
+    const int64_t n_ring = A.n_ring, n_z = A.n_z;
+    const int n = B.y;
     T *c = J.f[n];
     const int pnx = g.Nx + 2 * g.Hx, pnz = g.Nz + 2 * g.Hz + 1;
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t t = B.x * (int64_t)blockDim.x + threadIdx.x;
     auto wrap = [](int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); };
     if (t < n_ring) {
         // x-y halo columns over the whole parent z extent: rows j < 0 or j ≥ Ny (full parent width),
@@ -529,10 +552,20 @@ template <class T> __global__ void __launch_bounds__(256) k_halo_fused(DevGrid<T
 // The same update for NF ≤ 5 fields with every load issued before the first store (a load placed
 // after a store to U cannot be moved above it, as the arrays might alias, which would expose one
 // memory round trip per field).  Same operations, same order, per field.
-template <class T, int NF> __global__ void __launch_bounds__(256) k_rk3_batched(Ctx<T> C, RKArgs<T> a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+template <class T> struct RkPack {
+    Ctx<T> C;
+    RKArgs<T> a;
+};
+template <class T, int NF> struct KRk3Batched {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const RkPack<T> &A, const Blk &B);
+};
+template <class T, int NF> __device__ __forceinline__ void KRk3Batched<T, NF>::run(const RkPack<T> &A, const Blk &B) {
+    const Ctx<T> &C = A.C;
+    const RKArgs<T> &a = A.a;
+    const int i = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
+    const int k = B.z;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     const int64_t o = C.at(i, j, k);
     const bool bz = C.bz();
@@ -559,10 +592,16 @@ template <class T, int NF> __global__ void __launch_bounds__(256) k_rk3_batched(
     }
 }
 
-template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs<T> a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+template <class T> struct KRk3 {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const RkPack<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KRk3<T>::run(const RkPack<T> &A, const Blk &B) {
+    const Ctx<T> &C = A.C;
+    const RKArgs<T> &a = A.a;
+    const int i = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
+    const int k = B.z;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     const int64_t o = C.at(i, j, k);
     const bool bz = C.bz();
@@ -591,10 +630,22 @@ template <class T> __global__ void __launch_bounds__(256) k_rk3(Ctx<T> C, RKArgs
 // fetched again, and the loads of all KC levels are issued before the first store.
 constexpr int kCorrectLevels = 8;
 
-template <class T> __global__ void __launch_bounds__(256) k_correct(Ctx<T> C, T dtp, int phi) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k0 = blockIdx.z * kCorrectLevels;
+template <class T> struct CorrectArgs {
+    Ctx<T> C;
+    T dtp;
+    int phi;
+};
+template <class T> struct KCorrect {
+    static constexpr int kThreads = 256, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const CorrectArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KCorrect<T>::run(const CorrectArgs<T> &A, const Blk &B) {
+    const Ctx<T> &C = A.C;
+    const T dtp = A.dtp;
+    const int phi = A.phi;
+    const int i = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
+    const int k0 = B.z * kCorrectLevels;
     if (i >= C.g.Nx || j >= C.g.Ny) return;
     const int n = min(kCorrectLevels, C.g.Nz - k0);
     const T *p = C.s.pNHS;
@@ -689,27 +740,27 @@ template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
     if (sizeof(T) == 4 && C.g.Nz >= 8 && C.g.Nz <= 4 * kHySegLevels) {
         // FP32: four z segments per column scanned in parallel (the FP64 order is the reference's)
         dim3 blk(64, 4, 1);
-        hipLaunchKernelGGL(k_hydrostatic_seg<T>, dim3((C.g.Nx + 63) / 64, C.g.Ny), blk, 0, st, C);
+        launch_k<KHydrostaticSeg<T>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), blk, 0, st, CtxArgs<T>{C});
         return;
     }
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_hydrostatic<T>, cell_grid(C.g.Nx, C.g.Ny, 1, blk), blk, 0, st, C);
+    launch_k<KHydrostatic<T>>(cell_grid(C.g.Nx, C.g.Ny, 1, blk), blk, 0, st, CtxArgs<T>{C});
 }
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st) {
     dim3 blk(64, 4, 1);
-    RKArgs<T> a{dt, gamma, zeta, first, nfields};
+    const RkPack<T> A{C, RKArgs<T>{dt, gamma, zeta, first, nfields}};
     const dim3 grid = cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk);
     switch (nfields) {
-        case 3: hipLaunchKernelGGL((k_rk3_batched<T, 3>), grid, blk, 0, st, C, a); return;
-        case 4: hipLaunchKernelGGL((k_rk3_batched<T, 4>), grid, blk, 0, st, C, a); return;
-        case 5: hipLaunchKernelGGL((k_rk3_batched<T, 5>), grid, blk, 0, st, C, a); return;
-        default: hipLaunchKernelGGL(k_rk3<T>, grid, blk, 0, st, C, a);
+        case 3: launch_k<KRk3Batched<T, 3>>(grid, blk, 0, st, A); return;
+        case 4: launch_k<KRk3Batched<T, 4>>(grid, blk, 0, st, A); return;
+        case 5: launch_k<KRk3Batched<T, 5>>(grid, blk, 0, st, A); return;
+        default: launch_k<KRk3<T>>(grid, blk, 0, st, A);
     }
 }
 template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_correct<T>, cell_grid(C.g.Nx, C.g.Ny, (C.g.Nz + kCorrectLevels - 1) / kCorrectLevels, blk), blk, 0, st,
-                       C, dtp, phi);
+    launch_k<KCorrect<T>>(cell_grid(C.g.Nx, C.g.Ny, (C.g.Nz + kCorrectLevels - 1) / kCorrectLevels, blk), blk, 0, st,
+                          CorrectArgs<T>{C, dtp, phi});
 }
 template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st) {
     if (!g.kbot) return;
@@ -730,7 +781,7 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
         const int64_t n_z = layers * g.Nx * g.Ny;
         const int64_t total = n_ring + n_z;
         if (total == 0) return;
-        hipLaunchKernelGGL(k_halo_fused<T>, dim3((unsigned)((total + 255) / 256), J.nf), dim3(256), 0, st, g, J, n_ring, n_z);
+        launch_k<KHaloFused<T>>(dim3((unsigned)((total + 255) / 256), J.nf), dim3(256), 0, st, HaloArgs<T>{g, J, n_ring, n_z});
         return;
     }
     if (g.tz == OM_BOUNDED) {

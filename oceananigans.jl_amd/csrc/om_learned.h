@@ -27,6 +27,7 @@ struct NetState {
     double *lsum = nullptr;                                         // Σ over the active cells of (out − target)²
     int nchunk = 256;
     int adam_step = 0;
+    int tiled_conv = 1;   // convolutions with the input staged in LDS (1) or read per tap (0)
 };
 
 void net_pack(NetState &S, hipStream_t st);

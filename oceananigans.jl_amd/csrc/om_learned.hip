@@ -94,6 +94,92 @@ __global__ void __launch_bounds__(256) k_conv(NetDims d, const bf16 *__restrict_
     }
 }
 
+// -------------------------------------------------------------------------------------------
+// The same convolution with the input staged in LDS.  A block of 4 waves computes a tile of
+// 16 (x) × 4 (y) × 4 (z) voxels: its 18 × 6 × 6 input voxels (one-cell halo, 16 channels, 20.7 KB)
+// are loaded from the padded activation once, instead of once per tap and wave, and each wave
+// (one z level of the tile) reuses every weight fragment for its four 16-voxel rows.  Same
+// operands, same MFMA sequence per output: the results equal k_conv's bit for bit.
+// -------------------------------------------------------------------------------------------
+constexpr int CT_X = 16, CT_Y = 4, CT_Z = 4;
+constexpr int CT_SX = CT_X + 2, CT_SY = CT_Y + 2, CT_SZ = CT_Z + 2;
+constexpr int CT_V = CT_SX * CT_SY * CT_SZ;   // staged voxels per tile
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_conv_tiled(NetDims d, const bf16 *__restrict__ x, const bf16x8 *__restrict__ wp,
+                                                    bf16 *__restrict__ ybf, float *__restrict__ yf, const bf16 *__restrict__ act) {
+    __shared__ bf16x8 tile[CT_V * 2];
+    const int ntx = d.nx / CT_X, nty = (d.ny + CT_Y - 1) / CT_Y;
+    const int b = blockIdx.x;
+    const int tx = b % ntx, rest = b / ntx;
+    const int ty = rest % nty, tz = rest / nty;
+    const int i0 = tx * CT_X, j0 = ty * CT_Y, k0 = tz * CT_Z;
+    // stage the tile: voxel (xx, yy, zz) of the tile is padded voxel (i0 − 1 + xx, j0 − 1 + yy, k0 − 1 + zz);
+    // rows and levels past the padded box's last (ny, nz) are staged as zeros and never used
+    for (int q = threadIdx.x; q < CT_V * 2; q += 256) {
+        const int v = q >> 1, h = q & 1;
+        const int xx = v % CT_SX, r = v / CT_SX;
+        const int yy = r % CT_SY, zz = r / CT_SY;
+        const int i = i0 - 1 + xx, j = j0 - 1 + yy, k = k0 - 1 + zz;
+        bf16x8 val;
+        if (j <= d.ny && k <= d.nz) {
+            val = *reinterpret_cast<const bf16x8 *>(x + pidx(d, i, j, k) * 16 + h * 8);
+        } else {
+            for (int e = 0; e < 8; ++e) val[e] = (bf16)0.0f;
+        }
+        tile[q] = val;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = k0 + w;
+    if (k >= d.nz) return;   // whole wave
+    const int vi = lane & 15;
+    const int half = lane >> 5;                 // which of the step's two taps
+    const int c0h = (lane >> 4) & 1;            // which 8 input channels
+    f32x4 acc[CT_Y];
+#pragma unroll
+    for (int sgm = 0; sgm < CT_Y; ++sgm) acc[sgm] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 zero;
+    for (int e = 0; e < 8; ++e) zero[e] = (bf16)0.0f;
+#pragma unroll 2
+    for (int s = 0; s < 14; ++s) {
+        const int t = 2 * s + half;
+        int dx = 0, dy = 0, dz = 0;
+        if (t < 27) tap_offsets(t, dx, dy, dz);
+        const bf16x8 A = wp[s * 64 + lane];
+        const int base = ((w + 1 + dz) * CT_SY + (1 + dy)) * CT_SX + (vi + 1 + dx);
+#pragma unroll
+        for (int sgm = 0; sgm < CT_Y; ++sgm) {
+            const bf16x8 Bv = t < 27 ? tile[(base + sgm * CT_SX) * 2 + c0h] : zero;
+            acc[sgm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bv, acc[sgm], 0, 0, 0);
+        }
+    }
+    const int cout0 = (lane >> 4) << 2;
+#pragma unroll
+    for (int sgm = 0; sgm < CT_Y; ++sgm) {
+        const int j = j0 + sgm;
+        if (j >= d.ny) break;
+        const int iv = i0 + vi;
+        const int64_t o = pidx(d, iv, j, k);
+        if (EPI == EPI_RELU || EPI == EPI_MASK) {
+            bf16x4 out;
+            if (EPI == EPI_MASK) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(act + o * 16 + cout0);
+                for (int e = 0; e < 4; ++e) out[e] = (bf16)((float)a[e] > 0.f ? acc[sgm][e] : 0.f);
+            } else {
+                for (int e = 0; e < 4; ++e) out[e] = (bf16)(acc[sgm][e] > 0.f ? acc[sgm][e] : 0.f);
+            }
+            *reinterpret_cast<bf16x4 *>(ybf + o * 16 + cout0) = out;
+        } else {
+            const int64_t dn = (int64_t)iv + (int64_t)d.nx * ((int64_t)j + (int64_t)d.ny * k);
+            if (cout0 == 0) {
+                if (EPI == EPI_OUT) yf[dn] = acc[sgm][0];
+                else { yf[dn * 3 + 0] = acc[sgm][0]; yf[dn * 3 + 1] = acc[sgm][1]; yf[dn * 3 + 2] = acc[sgm][2]; }
+            }
+        }
+    }
+}
+
 // halo of a padded activation: x, y periodic (copied from the opposite interior edge), z zero
 __global__ void __launch_bounds__(256) k_net_halo(NetDims d, bf16 *a) {
     const int64_t nxy = (int64_t)(d.nx + 2) * (d.ny + 2);
@@ -290,10 +376,21 @@ void net_pack(NetState &S, hipStream_t st) {
 
 static void conv_launch(const NetState &S, int epi, const void *x, const void *wp, void *ybf, float *yf, const void *act, hipStream_t st) {
     const NetDims &d = S.d;
-    const int64_t segs = (int64_t)(d.nx / 16) * d.ny * d.nz;
-    const dim3 g(blocks_for(segs, 4)), b(256);
     const bf16 *X = (const bf16 *)x;
     const bf16x8 *W = (const bf16x8 *)wp;
+    if (S.tiled_conv) {
+        const int64_t tiles = (int64_t)(d.nx / CT_X) * ((d.ny + CT_Y - 1) / CT_Y) * ((d.nz + CT_Z - 1) / CT_Z);
+        const dim3 g((unsigned)tiles), b(256);
+        switch (epi) {
+            case EPI_RELU: hipLaunchKernelGGL(k_conv_tiled<EPI_RELU>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;
+            case EPI_OUT: hipLaunchKernelGGL(k_conv_tiled<EPI_OUT>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;
+            case EPI_MASK: hipLaunchKernelGGL(k_conv_tiled<EPI_MASK>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;
+            default: hipLaunchKernelGGL(k_conv_tiled<EPI_IN>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;
+        }
+        return;
+    }
+    const int64_t segs = (int64_t)(d.nx / 16) * d.ny * d.nz;
+    const dim3 g(blocks_for(segs, 4)), b(256);
     switch (epi) {
         case EPI_RELU: hipLaunchKernelGGL(k_conv<EPI_RELU>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;
         case EPI_OUT: hipLaunchKernelGGL(k_conv<EPI_OUT>, g, b, 0, st, d, X, W, (bf16 *)ybf, yf, (const bf16 *)act); break;

@@ -17,6 +17,7 @@
 // kernels give identically rounded results.
 #include <hip/hip_runtime.h>
 
+#include "om_batch.h"
 #include "om_kernels.h"
 
 namespace om {
@@ -70,15 +71,27 @@ template <class T> __device__ __forceinline__ void stage_small(T *dst, const T *
 // SIMD (5 blocks per CU, 149 KiB of LDS); the hint asks the compiler for that budget.
 template <class T, int NT> constexpr int tend_min_blocks() { return (sizeof(T) == 4 && NT <= 1) ? 5 : 1; }
 
+template <class T> struct TendTiledArgs {
+    Ctx<T> C;
+    int kchunk, ntx;
+};
+
+template <class T, int NT> struct KTendTiled {
+    static constexpr int kThreads = 256, kMinBlocks = tend_min_blocks<T, NT>();
+    static __device__ __forceinline__ void run(const TendTiledArgs<T> &A, const Blk &B);
+};
+
 template <class T, int NT>
-__global__ void __launch_bounds__(256, (tend_min_blocks<T, NT>())) k_tend_tiled(Ctx<T> C, int kchunk, int ntx) {
+__device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A, const Blk &B) {
     __shared__ TileSmem<T, NT> S;
+    const Ctx<T> &C = A.C;
+    const int kchunk = A.kchunk, ntx = A.ntx;
     const DevGrid<T> &g = C.g;
     const DevPhysics<T> &ph = C.ph;
     const int tid = threadIdx.x;
     const int a = tid & (TT - 1), b = tid >> 4;
     // work item → (z chunk, tile): the chunks of one tile are consecutive, tiles in row order
-    const int item = xcd_work_item((int)blockIdx.x, (int)gridDim.x);
+    const int item = xcd_work_item((int)B.x, (int)B.gx);
     const int nch = (g.Nz + kchunk - 1) / kchunk;
     const int tile = item / nch, chunk = item - tile * nch;
     const int tx = tile % ntx, ty = tile / ntx;
@@ -443,10 +456,11 @@ template <class T> void launch_tend_tiled(const Ctx<T> &C, int kchunk, hipStream
     if (kchunk < 1) kchunk = 1;
     const int nch = (g.Nz + kchunk - 1) / kchunk;
     const dim3 grid(ntx * nty * nch), blk(256);
+    const TendTiledArgs<T> A{C, kchunk, ntx};
     switch (C.ph.n_tracers) {
-        case 0: hipLaunchKernelGGL((k_tend_tiled<T, 0>), grid, blk, 0, st, C, kchunk, ntx); break;
-        case 1: hipLaunchKernelGGL((k_tend_tiled<T, 1>), grid, blk, 0, st, C, kchunk, ntx); break;
-        default: hipLaunchKernelGGL((k_tend_tiled<T, 2>), grid, blk, 0, st, C, kchunk, ntx); break;
+        case 0: launch_k<KTendTiled<T, 0>>(grid, blk, 0, st, A); break;
+        case 1: launch_k<KTendTiled<T, 1>>(grid, blk, 0, st, A); break;
+        default: launch_k<KTendTiled<T, 2>>(grid, blk, 0, st, A); break;
     }
 }
 

@@ -28,6 +28,9 @@ OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 OM_TUNE_LEARNED_ITERATIONS = 2
 OM_TUNE_POISSON_REFINEMENT = 3
+OM_TUNE_STEP_GRAPHS = 4
+OM_TUNE_CONV_TILED = 5
+OM_ENSEMBLE_DIRECT, OM_ENSEMBLE_MEMBER_GRAPHS, OM_ENSEMBLE_BATCHED = 0, 1, 2
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
 
@@ -119,6 +122,7 @@ SIGNATURES = {
     "om_ensemble_time_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]),
     "om_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "om_ensemble_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "om_ensemble_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "om_cell_advection_timescale": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "om_learned_set_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "om_learned_get_weights": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),

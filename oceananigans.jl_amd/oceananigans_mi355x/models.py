@@ -349,11 +349,18 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_last_solve_info(self._h, ctypes.byref(it), ctypes.byref(r), ctypes.byref(t)), self._h)
         return it.value, r.value, t.value
 
-    def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None, poisson_refinement=None):
+    def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None, poisson_refinement=None,
+                   step_graphs=None, conv_tiled=None):
         """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';
-        kchunk = z levels per block of the tiled kernel.  Results do not depend on them.
+        kchunk = z levels per block of the tiled kernel; step_graphs: time_step replays a captured
+        two-step hipGraph (default) or issues every launch; conv_tiled: the learned preconditioner's
+        convolutions with LDS-staged input tiles (default) or not.  Results do not depend on them.
         poisson_refinement: one FP64 iterative-refinement step after each FFT / Fourier–tridiagonal
         Poisson solve (off by default)."""
+        if conv_tiled is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_CONV_TILED, 1 if conv_tiled else 0), self._h)
+        if step_graphs is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_STEP_GRAPHS, 1 if step_graphs else 0), self._h)
         if poisson_refinement is not None:
             _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_POISSON_REFINEMENT, 1 if poisson_refinement else 0), self._h)
         if tendency_kernel is not None:
@@ -369,7 +376,9 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_TENDENCY_KERNEL, ctypes.byref(v)), self._h)
         kernel = ("tiled", "generic")[v.value]
         _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_KCHUNK, ctypes.byref(v)), self._h)
-        return {"tendency_kernel": kernel, "kchunk": v.value}
+        kchunk = v.value
+        _abi.check(self._lib.om_get_tuning(self._h, _abi.OM_TUNE_STEP_GRAPHS, ctypes.byref(v)), self._h)
+        return {"tendency_kernel": kernel, "kchunk": kchunk, "step_graphs": bool(v.value)}
 
     def poisson_refinement(self):
         v = ctypes.c_int32()
@@ -532,9 +541,16 @@ def time_step_(model, dt):
 
 class Ensemble:
     """Independent models advanced together (no NonhydrostaticModel counterpart in the reference,
-    where members are stepped one after another).  Each member's two-step launch sequence is
-    recorded once into a hipGraph on its own stream and replayed, members issued in rounds so that
-    their streams run concurrently.  Members keep their own sizes and configurations."""
+    where members are stepped one after another).  Members keep their own sizes.
+
+    Forms (``mode``): "batched" merges every member's launch of each kernel of the step into one
+    launch over a table of the members (one grid, each member a contiguous block range) and replays
+    the whole two-step sequence as one hipGraph; members must share one launch sequence (same
+    precision, closure, buoyancy, tracers, FFT solver).  "member_graphs" replays each member's own
+    two-step hipGraph on its own stream, members issued in rounds so that their streams run
+    concurrently; "direct" issues the same launches member by member."""
+
+    MODES = {"direct": _abi.OM_ENSEMBLE_DIRECT, "member_graphs": _abi.OM_ENSEMBLE_MEMBER_GRAPHS, "batched": _abi.OM_ENSEMBLE_BATCHED}
 
     def __init__(self, models):
         self.models = list(models)
@@ -546,8 +562,16 @@ class Ensemble:
             raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: {self._lib.om_ensemble_last_error(None).decode()}")
         self._h = h
 
-    def time_step(self, dt, nsteps=1, graphs=True):
-        rc = self._lib.om_ensemble_time_step(self._h, float(dt), int(nsteps), 1 if graphs else 0)
+    def set_batch_groups(self, groups):
+        """The batched form's member groups, each a member-table graph on its own stream."""
+        rc = self._lib.om_ensemble_set_batch_groups(self._h, int(groups))
+        if rc != 0:
+            raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: invalid group count {groups}")
+
+    def time_step(self, dt, nsteps=1, graphs=True, mode=None):
+        if mode is None:
+            mode = "member_graphs" if graphs else "direct"
+        rc = self._lib.om_ensemble_time_step(self._h, float(dt), int(nsteps), self.MODES[mode])
         if rc != 0:
             raise _abi.OmError(f"{_abi.ERRORS.get(rc, rc)}: {self._lib.om_ensemble_last_error(self._h).decode()}")
 

@@ -1,9 +1,12 @@
-"""Ensembles of independent models advanced together (hipGraph replay of each member's two-step
-launch sequence, members on their own streams) and the z-chunk autotuner, through the C ABI.
+"""Ensembles of independent models advanced together and the z-chunk autotuner, through the C ABI:
+member graphs (each member's two-step launch sequence replayed on its own stream) and the batched
+form (one launch per kernel of the step over a table of all members, the whole sequence one
+hipGraph).
 
-A replayed graph runs the same kernels with the same arguments as the direct launches, so every
+Both forms run the same kernel bodies with the same arguments as the direct launches, so every
 member must match a twin advanced by om_time_step bit for bit, after each of several replays with
-the program's own eager work and synchronisations in between."""
+the program's own eager work and synchronisations in between; the batched members also match the
+FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -79,3 +82,63 @@ def test_conjugate_gradient_members_are_rejected():
     cg = product_for(grid, kw, init)
     with pytest.raises(om.OmError, match="conjugate-gradient"):
         om.Ensemble([cg])
+
+
+def _same_precision_members(float_type, seeds=(11, 12, 13)):
+    cfgs = [convection(N=(16, 16, 16), float_type=float_type, noise_velocity=1e-2, seed=seeds[0]),
+            convection(N=(24, 20, 12), float_type=float_type, noise_velocity=1e-2, seed=seeds[1]),
+            convection(N=(32, 16, 8), float_type=float_type, noise_velocity=1e-2, seed=seeds[2])]
+    return cfgs, [product_for(*c) for c in cfgs], [product_for(*c) for c in cfgs]
+
+
+@pytest.mark.parametrize("float_type", [np.float32, np.float64])
+def test_batched_ensemble_matches_members_stepped_one_after_another(float_type):
+    _, members, twins = _same_precision_members(float_type)
+    ens = om.Ensemble(members)
+    for nsteps in (1, 2, 3, 2, 4, 1):      # the first step runs member by member; odd counts leave a remainder
+        ens.time_step(0.5, nsteps, mode="batched")
+        for t in twins:
+            t.time_step(0.5, nsteps)
+        for m, t in zip(members, twins):
+            _same(m, t)
+        print(float_type.__name__, "batched steps", nsteps, "iterations", [m.clock.iteration for m in members], flush=True)
+    members[1].set_tuning(kchunk=3)           # a member's launch shape changes: the tables are rebuilt
+    ens.time_step(0.25, 4, mode="batched")
+    twins[1].set_tuning(kchunk=3)
+    for t in twins:
+        t.time_step(0.25, 4)
+    for m, t in zip(members, twins):
+        _same(m, t)
+    ens.set_batch_groups(2)                   # two member-table graphs on concurrent streams
+    ens.time_step(0.25, 6, mode="batched")
+    for t in twins:
+        t.time_step(0.25, 6)
+    for m, t in zip(members, twins):
+        _same(m, t)
+    with pytest.raises(om.OmError):
+        ens.set_batch_groups(0)
+    ens.close()
+
+
+def test_batched_members_match_the_oracle():
+    from cases import oracle_for, rel_err
+    cfgs, members, _ = _same_precision_members(np.float64)
+    ens = om.Ensemble(members)
+    ens.time_step(1.0, 5, mode="batched")
+    for (grid, kw, init), m in zip(cfgs, members):
+        o = oracle_for(grid, kw, init)
+        o.time_step(1.0, 5)
+        for q, n in enumerate(NAMES):
+            f = m.velocities[n] if q < 3 else m.tracers[n]
+            e = rel_err(f.interior(), o.get(q))
+            print("member", grid.N, n, "rel err vs oracle", e, flush=True)
+            assert e <= 1e-11
+    ens.close()
+
+
+def test_batched_ensemble_rejects_members_with_other_launch_sequences():
+    members, _ = _members()                   # FP32 and FP64 members
+    ens = om.Ensemble(members)
+    with pytest.raises(om.OmError, match="launch sequence"):
+        ens.time_step(0.5, 4, mode="batched")
+    ens.close()
