@@ -255,6 +255,16 @@ int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *f
 int om_learned_energy_loss(om_model_t *m, const float *output, const float *target, double *energy, float *doutput);
 int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
                         int64_t *nan_count);
+/* om_field_statistics_noalloc: the same statistics as om_field_statistics, computed without any
+ * allocation.  No counterpart in the reference: it is for callers that cannot allocate device or
+ * pinned memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its reduction works in a file-scope device array of the library, 3·1024 + 3 doubles
+ * (24 KiB) held by each GPU the library runs on until the process ends, instead of the handle's
+ * buffers; the result is copied to the caller's host variables before the call returns.  Fields of
+ * any model size are accepted up to 2^40 elements (OM_EINVAL beyond).  Every call on one GPU uses
+ * that GPU's one array: calls that overlap on one GPU (from several host threads) race. */
+int om_field_statistics_noalloc(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
+                                int64_t *nan_count);
 
 /* timing of the last om_time_step call (HIP events on the handle's stream).
  * ms_total: the whole call.  With phase timing enabled, ms_per_phase[p] and launches_per_phase[p]

@@ -1383,6 +1383,38 @@ int om_field_statistics(om_model_t *m, int32_t field, int32_t with_halos, double
     });
 }
 
+int om_field_statistics_noalloc(om_model_t *m, int32_t field, int32_t with_halos, double *maximum, double *minimum,
+                                int64_t *nan_count) {
+    if (!m) return OM_EINVAL;
+    return WITH_MODEL(m, {
+        bool is_w;
+        auto *o = M->field_ptr(field, is_w);
+        if (!o) throw ArgError{"unknown field id"};
+        // the reduction's partials and results take a fixed 3 · 1024 + 3 doubles whatever
+        // the field's size; fields beyond kNoallocMaxElements are refused
+        const int64_t n = (int64_t)(with_halos ? M->pnx : M->g.Nx) * (with_halos ? M->pny : M->g.Ny) *
+                          (with_halos ? M->nz_parent(is_w) : M->nz_interior(is_w));
+        if (n > kNoallocMaxElements)
+            throw ArgError{"om_field_statistics_noalloc: the field has more than 2^40 elements"};
+        int err = 0;
+        double *scr = diag_scratch(M->device, &err);
+        if (!scr) {
+            if (err > 0) throw (HipError{(hipError_t)err, "hipGetSymbolAddress(scratch)"});
+            throw ArgError{"om_field_statistics_noalloc: device index beyond the scratch table"};
+        }
+        if (diag_partial_doubles() + 3 > diag_scratch_doubles()) throw ArgError{"scratch smaller than the reduction"};
+        double *out = scr + diag_partial_doubles();
+        launch_field_stats(M->g, o, with_halos ? 1 : 0, (is_w && M->g.tz == OM_BOUNDED) ? 1 : 0, scr, out, M->st);
+        HIPCHECK(hipGetLastError());
+        double h[3];
+        HIPCHECK(hipMemcpyAsync(h, out, 3 * sizeof(double), hipMemcpyDeviceToHost, M->st));
+        HIPCHECK(hipStreamSynchronize(M->st));
+        if (maximum) *maximum = h[0];
+        if (minimum) *minimum = h[1];
+        if (nan_count) *nan_count = (int64_t)h[2];
+    });
+}
+
 int om_learned_set_weights(om_model_t *m, const float *weights) {
     if (!m || !weights) return OM_EINVAL;
     return WITH_MODEL(m, {

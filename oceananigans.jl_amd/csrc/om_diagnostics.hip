@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "om_kernels.h"
 
@@ -115,6 +116,43 @@ template <class T> void launch_field_stats(const DevGrid<T> &g, const T *origin,
     hipLaunchKernelGGL(k_field_stats<T>, dim3(kDiagBlocks), dim3(kDiagThreads), 0, st, g, origin, with_halos, face_z, partial);
     hipLaunchKernelGGL(k_stats_finish, dim3(1), dim3(kDiagThreads), 0, st, (const double *)partial, 1, out);
 }
+
+#ifndef OM_ONLY_DOUBLE
+// ----------------------------------------------------------------------------------------
+// Allocation-free reductions.  `scratch` holds the per-block partials (3 × kDiagBlocks doubles)
+// and the three results of one field-statistics reduction; it lives in the library's code object,
+// so each GPU holds its own copy from load to process end and no call allocates.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device address is taken once
+// per device with hipGetSymbolAddress (with that device current) and only that address is handed
+// to kernels and copies.  Every entry point that uses it on one GPU shares the one array, so calls
+// that overlap on one GPU race.
+// ----------------------------------------------------------------------------------------
+constexpr int kScratchDoubles = 3 * kDiagBlocks + 3;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[kMaxScratchDevices];
+
+double *diag_scratch(int device, int *err) {
+    *err = 0;
+    if (device < 0 || device >= kMaxScratchDevices) { *err = -1; return nullptr; }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) { *err = (int)e; return nullptr; }
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+int diag_scratch_doubles() { return kScratchDoubles; }
+int diag_partial_doubles() { return 3 * kDiagBlocks; }
+#endif
 
 #define OM_DIAG_INSTANTIATE(T)                                                                              \
     template void launch_adv_timescale<T>(const Ctx<T> &, double *, double *, hipStream_t);                 \

@@ -774,6 +774,12 @@ template <class T> void launch_add_correction(const DevGrid<T> &g, T *p, int64_t
 // diagnostics reductions (om_diagnostics.hip); partial holds 3·1024 doubles, out 3 doubles
 // (max, min, NaN count); the advection timescale reduction leaves max = max inverse timescale
 template <class T> void launch_adv_timescale(const Ctx<T> &C, double *partial, double *out, hipStream_t st);
+// allocation-free reductions: the device address of the library's static scratch array on `device`
+// (taken once per device; nullptr with *err set on failure), its size and the part the partials take
+double *diag_scratch(int device, int *err);
+int diag_scratch_doubles();
+int diag_partial_doubles();
+constexpr int64_t kNoallocMaxElements = int64_t(1) << 40;
 template <class T> void launch_field_stats(const DevGrid<T> &g, const T *origin, int with_halos, int face_z, double *partial,
                                            double *out, hipStream_t st);
 

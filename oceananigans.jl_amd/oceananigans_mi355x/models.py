@@ -409,6 +409,13 @@ class NonhydrostaticModel:
         _abi.check(self._lib.om_cell_advection_timescale(self._h, ctypes.byref(t)), self._h)
         return t.value
 
+    def field_statistics_noalloc(self, field_id, with_halos=True):
+        """field_statistics through om_field_statistics_noalloc (no allocation in the call)."""
+        mx, mn, nn = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _abi.check(self._lib.om_field_statistics_noalloc(self._h, int(field_id), 1 if with_halos else 0, ctypes.byref(mx),
+                                                         ctypes.byref(mn), ctypes.byref(nn)), self._h)
+        return mx.value, mn.value, nn.value
+
     def field_statistics(self, field_id, with_halos=True):
         """(maximum, minimum, number of NaNs) of a field's parent data (or interior), on the device."""
         mx, mn, nn = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
