@@ -162,17 +162,23 @@ def run_cg_seamount(args):
     learned = om.NonhydrostaticModel(g_ln, **kw_ln)
     om.set_(learned, **init)
     rng = np.random.default_rng(7)
-    w = (rng.normal(0, np.sqrt(2.0 / 432), size=(4, 27, 16, 16))).astype(np.float32)
+    w = (rng.normal(0, np.sqrt(2.0 / 432), size=(7, 27, 16, 16))).astype(np.float32)
     w[0, :, 3:, :] = 0
-    w[3] = 0                                     # zero output layer: starts as the FFT preconditioner
+    w[6] = 0                                     # zero output layer: starts as the FFT preconditioner
     learned.learned_set_weights(w)
-    learned.learned_log(iterations_per_solve=8, capacity=args.cg_pairs)
+    # online training over the first cg_rounds time steps: each step's solves log (input, target)
+    # pairs from their first cg_log_iterations iterations into a ring of cg_pairs, then Adam steps
+    # run over the ring
+    learned.learned_log(iterations_per_solve=args.cg_log_iterations, capacity=args.cg_pairs)
     t0 = time.perf_counter()
     for rnd in range(args.cg_rounds):
         learned.time_step(20.0, 1)
         l0, l1 = learned.learned_train(args.cg_train_steps, lr=5e-4)
-        print(f"bench.py: seamount training round {rnd}: energy loss {l0:.3f} -> {l1:.3f}", file=sys.stderr, flush=True)
+        if rnd % 20 == 0 or rnd == args.cg_rounds - 1:
+            print(f"bench.py: seamount training step {rnd}: energy loss {l0:.3f} -> {l1:.3f}, "
+                  f"{learned.learned_pairs_logged()} pairs logged", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t0
+    pairs_logged = learned.learned_pairs_logged()
     learned.learned_log(iterations_per_solve=8, capacity=24)
     if args.cg_learned_iterations:
         learned.set_tuning(learned_iterations=args.cg_learned_iterations)
@@ -192,8 +198,9 @@ def run_cg_seamount(args):
     learned.close()
     return {"grid": "x".join(map(str, N)), "reltol": 1e-8, "fft": res["fft"], "learned": res["learned"],
             "learned_over_fft_time": res["learned"]["ms_per_solve"] / res["fft"]["ms_per_solve"],
-            "training": {"rounds": args.cg_rounds, "adam_steps_per_round": args.cg_train_steps,
-                         "pair_capacity": args.cg_pairs, "wall_s": t_train},
+            "training": {"time_steps": args.cg_rounds, "adam_steps_per_time_step": args.cg_train_steps,
+                         "pairs_logged": pairs_logged, "pair_ring_capacity": args.cg_pairs,
+                         "iterations_logged_per_solve": args.cg_log_iterations, "wall_s": t_train},
             "network_iterations_per_solve": args.cg_learned_iterations or "all"}
 
 
@@ -318,9 +325,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the seamount CG measurement (learned vs FFT preconditioner)")
     ap.add_argument("--cg-size", default="256,256,128")
-    ap.add_argument("--cg-rounds", type=int, default=12)
-    ap.add_argument("--cg-train-steps", type=int, default=100)
-    ap.add_argument("--cg-pairs", type=int, default=96)
+    ap.add_argument("--cg-rounds", type=int, default=200, help="time steps of online training (one logged step each)")
+    ap.add_argument("--cg-train-steps", type=int, default=10, help="Adam steps after each logged time step")
+    ap.add_argument("--cg-pairs", type=int, default=96, help="capacity of the ring of logged pairs")
+    ap.add_argument("--cg-log-iterations", type=int, default=3, help="leading iterations of each solve that log a pair")
     ap.add_argument("--cg-learned-iterations", type=int, default=2,
                     help="leading CG iterations that apply the network (0: all)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")

@@ -216,10 +216,12 @@ int om_divergence(om_model_t *m, double *div_interior);
  *   maximum and minimum. */
 int om_cell_advection_timescale(om_model_t *m, double *tau);
 
-/* learned preconditioner (no reference counterpart; see OM_PRECOND_LEARNED).  The network: four
- * 3×3×3 convolutions of 16 channels (ReLU after the first three), BF16 weights and activations,
- * FP32 accumulation on the matrix cores.  Input channels per cell: r V⁻¹ / rms, z_FFT / rms,
- * active-cell mask; output: the correction to z_FFT in units of rms(z_FFT).
+/* learned preconditioner (no reference counterpart; see OM_PRECOND_LEARNED).  The network: a
+ * four-level hierarchy of seven 3×3×3 convolutions of 16 channels on the grid and on the grid
+ * coarsened 2×, 4× and 8× (2×2×2 means down, copies up, skip sums; ReLU after all but the last),
+ * BF16 weights and activations, FP32 accumulation on the matrix cores.  Needs Nx a multiple of 128,
+ * Ny and Nz multiples of 8.  Input channels per cell: r V⁻¹ / rms, z_FFT / rms, active-cell mask;
+ * output: the correction to z_FFT in units of rms(z_FFT).
  * Weights: OM_LEARNED_NWEIGHTS FP32 values, layout [layer][tap][cin][cout], tap = (dz+1)·9 +
  * (dy+1)·3 + (dx+1).  Host arrays of the test hooks are dense, x fastest, channels innermost.
  *   om_learned_set_weights / get_weights   the FP32 master weights (setting them resets Adam)
@@ -241,11 +243,12 @@ int om_cell_advection_timescale(om_model_t *m, double *tau);
  *   om_learned_energy_loss  the training loss's energy and its gradient for a given network output
  *                        y and target t (test hook): d = active·(y − t), E = dᵀ(−V∇²)d with the
  *                        CG solver's conditional operator, dE/dy = −2·active·V∇²d (dense, x fastest) */
-#define OM_LEARNED_NWEIGHTS (4 * 27 * 16 * 16)
+#define OM_LEARNED_NWEIGHTS (7 * 27 * 16 * 16)
 int om_learned_set_weights(om_model_t *m, const float *weights);
 int om_learned_get_weights(om_model_t *m, float *weights);
 int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity);
 int om_learned_pairs(om_model_t *m, int32_t *count);
+int om_learned_pairs_logged(om_model_t *m, int64_t *total);   /* pairs logged since om_learned_log (the ring keeps the last `capacity`) */
 int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first, double *loss_last);
 int om_learned_evaluate(om_model_t *m, double *mean_loss);
 int om_learned_forward(om_model_t *m, const float *input3, float *output);
@@ -306,6 +309,8 @@ int om_last_step_timing(const om_model_t *m, double *ms_total, double *ms_per_ph
                                        and the conjugate-gradient solver always launch directly. */
 #define OM_TUNE_CONV_TILED 5       /* convolutions of the learned preconditioner: 1 (default) input tiles
                                        staged in LDS; 0 the input read per tap.  Identical results. */
+#define OM_TUNE_WGRAD_MFMA 6       /* weight gradients of the learned preconditioner: 1 (default) on the
+                                       matrix cores; 0 FP32 sums.  Equal to FP32 rounding. */
 int om_set_tuning(om_model_t *m, int32_t key, int32_t value);
 int om_get_tuning(const om_model_t *m, int32_t key, int32_t *value);
 

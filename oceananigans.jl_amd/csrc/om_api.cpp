@@ -201,6 +201,7 @@ template <class T> struct ModelT : ModelBase {
     double *cg_rprev = nullptr;                    // r of the previous iteration (flexible CG)
     int ln_learned_its = 0, ln_fell_back = 0;
     int log_iters = 0, log_cap = 0, log_count = 0, log_next = 0;   // pair logging (om_learned_log)
+    int64_t log_total = 0;                                         // pairs logged since om_learned_log
     std::vector<float *> pair_in3, pair_tgt;       // logged pairs (device, dense)
     std::vector<double> pair_energy;               // tᵀ(−V∇²)t of each pair's target
     float *ln_gout = nullptr;                      // dense output gradient of the energy loss
@@ -407,7 +408,8 @@ template <class T> struct ModelT : ModelBase {
         // fields: one block; order U[nf], Gn[nf], Gm[nf], pNHS, pHY, nu, kap[nt]
         fsize = lead + (size_t)g.sz * pnz;
         fsize = (fsize + 63) & ~(size_t)63;   // 256-byte aligned field blocks
-        const int nblocks = 3 * nf + 3 + nt;
+        const bool diff_pHY = sizeof(T) == 4 && has_pHY;   // FP32: pHY′ differences as fields of their own
+        const int nblocks = 3 * nf + 3 + nt + (diff_pHY ? 2 : 0);
         fields_block = dalloc<T>(fsize * nblocks);
         memset(&s, 0, sizeof(s));
         int q = 0;
@@ -418,6 +420,10 @@ template <class T> struct ModelT : ModelBase {
         s.pHY = field_origin(fields_block + fsize * (q++));
         s.nu = field_origin(fields_block + fsize * (q++));
         for (int n = 0; n < nt; ++n) s.kap[n] = field_origin(fields_block + fsize * (q++));
+        if (diff_pHY) {
+            s.dxpHY = field_origin(fields_block + fsize * (q++));
+            s.dypHY = field_origin(fields_block + fsize * (q++));
+        }
         // Poisson plan
         const size_t ncell = (size_t)g.Nx * g.Ny * g.Nz;
         rhs_dense = dalloc<T>(ncell);
@@ -645,13 +651,11 @@ template <class T> struct ModelT : ModelBase {
     // zero correction, i.e. the FFT preconditioner)
     void net_alloc() {
         if (net.w) return;
-        if (g.Nx % 16 != 0) throw ArgError{"the learned preconditioner needs Nx to be a multiple of 16"};
+        if (!net_dims_ok(g.Nx, g.Ny, g.Nz))
+            throw ArgError{"the learned preconditioner's four levels need Nx a multiple of 128 and Ny, Nz multiples of 8"};
         if (g.tz != OM_BOUNDED) throw ArgError{"the learned preconditioner needs a Bounded z"};
         net.d = NetDims{g.Nx, g.Ny, g.Nz};
-        const size_t pv = (size_t)(g.Nx + 2) * (g.Ny + 2) * (g.Nz + 2) * 16;
-        for (int l = 0; l < NET_LAYERS; ++l) net.act[l] = dalloc<uint16_t>(pv);
-        net.g[0] = dalloc<uint16_t>(pv);
-        net.g[1] = dalloc<uint16_t>(pv);
+        net_place_buffers(net, dalloc<char>(net_arena_bytes(net.d)));
         net.w = dalloc<float>(NET_LAYERS * NET_WPL);
         net.m = dalloc<float>(NET_LAYERS * NET_WPL);
         net.v = dalloc<float>(NET_LAYERS * NET_WPL);
@@ -660,7 +664,7 @@ template <class T> struct ModelT : ModelBase {
         net.wb = dalloc<uint16_t>(NET_LAYERS * 14 * 64 * 8);
         const size_t nc = (size_t)g.Nx * g.Ny * g.Nz;
         net.out = dalloc<float>(nc);
-        net.wpartial = dalloc<float>((size_t)net.nchunk * 27 * 256);
+        net.wpartial = dalloc<float>((size_t)std::max(net.nchunk, 512) * 27 * 256);
         net.lpartial = dalloc<double>(NET_RED_BLOCKS);
         net.lsum = dalloc<double>(1);
         ln_in3 = dalloc<float>(3 * nc);
@@ -707,6 +711,7 @@ template <class T> struct ModelT : ModelBase {
             const int slot = log_next;
             log_next = (log_next + 1) % log_cap;
             log_count = std::min(log_count + 1, log_cap);
+            ++log_total;
             HIPCHECK(hipMemcpyAsync(pair_in3[slot], snap_in3[k], 3 * nc * sizeof(float), hipMemcpyDeviceToDevice, st));
             ln_target(g, cg_x, snap_x[k], snap_in3[k], snap_isz[k], pair_tgt[slot], cg_partial, cg_out, st);
             cg_read(2);
@@ -1447,6 +1452,7 @@ int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity
         M->log_iters = capacity > 0 ? iterations_per_solve : 0;
         M->log_cap = capacity;
         M->log_count = 0;
+        M->log_total = 0;
         M->log_next = 0;
         while ((int)M->pair_in3.size() > capacity) {
             M->dfree(M->pair_in3.back());
@@ -1461,6 +1467,11 @@ int om_learned_log(om_model_t *m, int32_t iterations_per_solve, int32_t capacity
 int om_learned_pairs(om_model_t *m, int32_t *count) {
     if (!m || !count) return OM_EINVAL;
     return WITH_MODEL(m, { *count = M->log_count; });
+}
+
+int om_learned_pairs_logged(om_model_t *m, int64_t *total) {
+    if (!m || !total) return OM_EINVAL;
+    return WITH_MODEL(m, { *total = M->log_total; });
 }
 
 int om_learned_train(om_model_t *m, int32_t steps, double lr, double *loss_first, double *loss_last) {
@@ -1573,7 +1584,8 @@ int om_learned_last_solve(om_model_t *m, int32_t *learned_iterations, int32_t *f
 
 static bool known_tuning_key(int32_t key) {
     return key == OM_TUNE_TENDENCY_KERNEL || key == OM_TUNE_KCHUNK || key == OM_TUNE_LEARNED_ITERATIONS ||
-           key == OM_TUNE_POISSON_REFINEMENT || key == OM_TUNE_STEP_GRAPHS || key == OM_TUNE_CONV_TILED;
+           key == OM_TUNE_POISSON_REFINEMENT || key == OM_TUNE_STEP_GRAPHS || key == OM_TUNE_CONV_TILED ||
+           key == OM_TUNE_WGRAD_MFMA;
 }
 
 int om_autotune(om_model_t *m, int32_t *kchunk) {
@@ -1870,6 +1882,7 @@ int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
     if (key == OM_TUNE_POISSON_REFINEMENT && (value < 0 || value > 1)) return OM_EINVAL;
     if (key == OM_TUNE_STEP_GRAPHS && (value < 0 || value > 1)) return OM_EINVAL;
     if (key == OM_TUNE_CONV_TILED && (value < 0 || value > 1)) return OM_EINVAL;
+    if (key == OM_TUNE_WGRAD_MFMA && (value < 0 || value > 1)) return OM_EINVAL;
     if (!known_tuning_key(key)) return OM_EINVAL;
     return WITH_MODEL(m, {
         if (key == OM_TUNE_TENDENCY_KERNEL) M->tune_tendency_kernel = value;
@@ -1877,6 +1890,7 @@ int om_set_tuning(om_model_t *m, int32_t key, int32_t value) {
         else if (key == OM_TUNE_LEARNED_ITERATIONS) M->tune_learned_iterations = value;
         else if (key == OM_TUNE_STEP_GRAPHS) M->tune_step_graphs = value;
         else if (key == OM_TUNE_CONV_TILED) M->net.tiled_conv = value;
+        else if (key == OM_TUNE_WGRAD_MFMA) M->net.mfma_wgrad = value;
         else M->set_refinement(value);
         M->drop_graph();   // a captured step holds the launch choices it was recorded with
     });
@@ -1892,6 +1906,7 @@ int om_get_tuning(const om_model_t *mc, int32_t key, int32_t *value) {
                : key == OM_TUNE_LEARNED_ITERATIONS   ? M->tune_learned_iterations
                : key == OM_TUNE_STEP_GRAPHS          ? M->tune_step_graphs
                : key == OM_TUNE_CONV_TILED           ? M->net.tiled_conv
+               : key == OM_TUNE_WGRAD_MFMA           ? M->net.mfma_wgrad
                                                      : M->tune_refine;
     });
 }

@@ -392,8 +392,9 @@ template <class T> __device__ __forceinline__ C2<T> eig_divide(C2<T> v, T den, b
 }
 
 // ---------------------------------------------------------------------------------------------
-// Y pass: complex FFT along y.  Lines ℓ = k·Mx + kx (ℓ < Mx·Nz), R consecutive lines per block;
-// element j of line ℓ sits at S[(k·Ny + j)·Mx + kx].
+// Y pass: complex FFT along y.  Lines ℓ = k·Ms + kx (ℓ < Ms·Nz, Ms the padded row stride), R
+// consecutive lines per block; element j of line ℓ sits at S[(k·Ny + j)·Ms + kx].  Lines with
+// kx ≥ Mx are row padding and are skipped.
 // fused_divide: forward → divide by eigenvalues → backward (y is the last non-Flat axis).
 // ---------------------------------------------------------------------------------------------
 template <class T> struct YArgs {
@@ -450,8 +451,9 @@ template <class T> __device__ __forceinline__ void KYPass<T>::run(const YArgs<T>
 
 // ---------------------------------------------------------------------------------------------
 // Z pass (last axis): forward transform, eigenvalue division, backward transform.  Lines
-// ℓ = j·Mx + kx (ℓ < Mx·Ny), R consecutive lines per block; element k of line ℓ sits at
-// S[k·Ny·Mx + ℓ], so a block reads R contiguous values per level.
+// ℓ = j·Ms + kx (ℓ < Ms·Ny), R consecutive lines per block; element k of line ℓ sits at
+// S[k·Ny·Ms + ℓ], so a block reads R contiguous values per level; lines with kx ≥ Mx are row
+// padding and are skipped.
 //   zmode 0 (Periodic z): FFT of length Nz.
 //   zmode 1 (Bounded z, Nz even): the DCT-II of length N = Nz through one complex FFT of length N
 //     (Makhoul 1980): v = (x0, x2, …, x_{N−2}, x_{N−1}, …, x3, x1), V = FFT(v),
@@ -541,9 +543,10 @@ template <class T> __device__ __forceinline__ void KZPass<T>::run(const ZArgs<T>
 
 // ---------------------------------------------------------------------------------------------
 // Fourier–tridiagonal z solve (Thomas per (kx, ky) column; diagonals from
-// fourier_tridiagonal_poisson_solver.jl:199-229).  One thread per column, the columns ℓ = ky·Mx + kx
+// fourier_tridiagonal_poisson_solver.jl:199-229).  One thread per column, the columns ℓ = ky·Ms + kx
 // flattened so that a level of a block's columns is one contiguous run (level k of column ℓ at
-// k·(Mx·Ny) + ℓ); t[k] kept in LDS.  Each sweep reads its levels in batches of tri_batch (16 FP32, 8 FP64), all
+// k·(Ms·Ny) + ℓ; columns with kx ≥ Mx are row padding and are skipped); t[k] kept in LDS.  Each sweep
+// reads its levels in batches of tri_batch (16 FP32, 8 FP64), all
 // loads of a batch issued before its arithmetic, so a column costs 2·Nz/batch memory round
 // trips instead of 2·Nz.  The (0,0) column is singular: its last pivot is a rounding residue and
 // is elided as in the reference; the constant this leaves is removed with the column mean

@@ -70,6 +70,10 @@ template <class T> struct StateView {
     T *nu;                   // νₑ
     T *kap[kMaxTracers];     // κₑ per tracer (AMD)
     T *pHY;                  // hydrostatic pressure anomaly
+    // FP32 only (nullptr in FP64): pHY′(i) − pHY′(i−1) and pHY′(j) − pHY′(j−1) at each cell, formed as
+    // column sums of the buoyancy's horizontal differences, which FP32 holds to its relative
+    // precision where the difference of two rounded column sums would not
+    T *dxpHY, *dypHY;
     T *pNHS;
     const T *btr, *Ttr, *Str;  // buoyancy tracer / temperature / salinity (resolved on the host)
 };

@@ -61,8 +61,10 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C,
             T I = C.fly() ? a : T(0.5) * (a + b);
             G += (T(0.5) * (C.ph.f + C.ph.f)) * I * recip<T>(A);
         }
-        if (C.ph.hydrostatic)
-            G -= (ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k)) * C.g.idx;
+        if (C.ph.hydrostatic) {
+            const T d = C.s.dxpHY ? C.ld(C.s.dxpHY, i, j, k) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i - 1, j, k);
+            G -= (ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : d) * C.g.idx;
+        }
         if (clo) {
             const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
             T sx = C.flx() || ib.template zd<0, 0, 0>(i, j, k, i - 1, j, k) ? T(0) : Ax * C.f_ux(i, j, k) - Ax * C.f_ux(i - 1, j, k);
@@ -90,8 +92,10 @@ template <class T> __global__ void __launch_bounds__(256) k_tendencies(Ctx<T> C,
             T I = C.fly() ? b : T(0.5) * (a + b);
             G -= (T(0.5) * (C.ph.f + C.ph.f)) * I * recip<T>(A);
         }
-        if (C.ph.hydrostatic)
-            G -= (ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k)) * C.g.idy;
+        if (C.ph.hydrostatic) {
+            const T d = C.s.dypHY ? C.ld(C.s.dypHY, i, j, k) : C.ld(C.s.pHY, i, j, k) - C.ld(C.s.pHY, i, j - 1, k);
+            G -= (ib.template zd<0, 0, 0>(i, j, k, i, j - 1, k) ? T(0) : d) * C.g.idy;
+        }
         if (clo) {
             const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az;
             T sx = C.flx() || ib.template zd<1, 1, 0>(i + 1, j, k, i, j, k) ? T(0) : Ax * C.f_uy(i + 1, j, k) - Ax * C.f_uy(i, j, k);
@@ -310,19 +314,91 @@ template <class T> __device__ __forceinline__ void KHydrostatic<T>::run(const Ct
     }
 }
 
-// FP32 pHY′ with the column split into four z segments, one wave each, for 64 columns of one
-// row j: each thread forms its segment's terms ½(b(m) + b(m+1))·Δzᶠ(m+1) and their suffix sums
-// (at most kHySegLevels in registers), the segment totals are exchanged through LDS, and
-// p(k) = −(terms of the segments above + the local suffix sum).  The same sum as k_hydrostatic,
-// associated in segments, so FP32 results differ from the sequential order by rounding only.
-constexpr int kHySegLevels = 64;   // levels per segment held in registers (Nz ≤ 4·64)
+// FP32 pHY′ with the column split into z segments, one wave each, for 64 columns of one row j,
+// and the horizontal differences pHY′(i) − pHY′(i−1), pHY′(j) − pHY′(j−1) formed as column sums of
+// the buoyancy's x and y differences (when the model holds fields for them): the horizontal
+// gradient of pHY′ is a difference of two column sums of size ~Σ b Δz that agree to a few FP32 ulps
+// where b varies horizontally by ~1e-5 of itself, and differencing the rounded sums would lose
+// most of it; b's neighbour differences are exact in FP32 (Sterbenz) and their sums keep FP32's
+// relative precision.
+// KHydrostaticReg (eight segments of at most kHyLevels levels, Nz ≤ 8·kHyLevels): each thread forms its
+// segment's terms ½(b(m) + b(m+1))·Δzᶠ(m+1) from the top down with their running sums in registers,
+// the segment totals are exchanged through LDS, and p(m) = −(terms of the segments above + the
+// running sum at m).  KHydrostaticSeg (any Nz, four segments): the same in two passes, the first
+// for the totals, the second recomputing the terms.  Both associate the reference's sequential sum
+// in segments, so FP32 results differ from the sequential order by rounding only.
+constexpr int kHyLevels = 16;
+
+template <class T> struct KHydrostaticReg {
+    static constexpr int kThreads = 512, kMinBlocks = 1;
+    static __device__ __forceinline__ void run(const CtxArgs<T> &A, const Blk &B);
+};
+template <class T> __device__ __forceinline__ void KHydrostaticReg<T>::run(const CtxArgs<T> &A, const Blk &B) {
+    __shared__ T tot[3][8][64];
+    const Ctx<T> &C = A.C;
+    const int i = B.x * 64 + threadIdx.x;
+    const int j = B.y;
+    const int seg = threadIdx.y, nseg = blockDim.y;
+    const int Nz = C.g.Nz;
+    const int per = (Nz + nseg - 1) / nseg;
+    const int lo = seg * per, hi = min(Nz, lo + per);   // levels lo .. hi-1 of this segment
+    const bool col = i < C.g.Nx;
+    const bool diff = C.s.dxpHY != nullptr;
+    T tp[kHyLevels], tx[kHyLevels], ty[kHyLevels];
+    T rp = T(0), rx = T(0), ry = T(0);
+    if (col) {
+        // terms m = hi-1 down to lo; b(m+1) at m = Nz-1 is the halo value b(Nz)
+        T bu = C.buoy(i, j, hi);
+        T xu = diff ? C.buoy_diff(i, j, hi, 1, 0) : T(0), yu = diff ? C.buoy_diff(i, j, hi, 0, 1) : T(0);
+#pragma unroll
+        for (int q = 0; q < kHyLevels; ++q) {
+            const int m = hi - 1 - q;
+            if (m < lo) break;
+            const T bm = C.buoy(i, j, m);
+            rp += (T(0.5) * (bm + bu)) * C.g.dzf[m + 1];
+            tp[q] = rp;
+            bu = bm;
+            if (diff) {
+                const T xm = C.buoy_diff(i, j, m, 1, 0), ym = C.buoy_diff(i, j, m, 0, 1);
+                rx += (T(0.5) * (xm + xu)) * C.g.dzf[m + 1];
+                ry += (T(0.5) * (ym + yu)) * C.g.dzf[m + 1];
+                tx[q] = rx;
+                ty[q] = ry;
+                xu = xm;
+                yu = ym;
+            }
+        }
+    }
+    tot[0][seg][threadIdx.x] = rp;
+    tot[1][seg][threadIdx.x] = rx;
+    tot[2][seg][threadIdx.x] = ry;
+    __syncthreads();
+    if (!col) return;
+    T ap = T(0), ax = T(0), ay = T(0);
+    for (int s2 = seg + 1; s2 < nseg; ++s2) {
+        ap += tot[0][s2][threadIdx.x];
+        ax += tot[1][s2][threadIdx.x];
+        ay += tot[2][s2][threadIdx.x];
+    }
+#pragma unroll
+    for (int q = 0; q < kHyLevels; ++q) {
+        const int m = hi - 1 - q;
+        if (m < lo) break;
+        const int64_t o = C.at(i, j, m);
+        C.s.pHY[o] = -(ap + tp[q]);
+        if (diff) {
+            C.s.dxpHY[o] = -(ax + tx[q]);
+            C.s.dypHY[o] = -(ay + ty[q]);
+        }
+    }
+}
 
 template <class T> struct KHydrostaticSeg {
     static constexpr int kThreads = 256, kMinBlocks = 1;
     static __device__ __forceinline__ void run(const CtxArgs<T> &A, const Blk &B);
 };
 template <class T> __device__ __forceinline__ void KHydrostaticSeg<T>::run(const CtxArgs<T> &A, const Blk &B) {
-    __shared__ T tot[4][64];
+    __shared__ T tot[3][4][64];
     const Ctx<T> &C = A.C;
     const int i = B.x * 64 + threadIdx.x;
     const int j = B.y;
@@ -331,31 +407,53 @@ template <class T> __device__ __forceinline__ void KHydrostaticSeg<T>::run(const
     const int per = (Nz + 3) / 4;
     const int lo = seg * per, hi = min(Nz, lo + per);   // levels lo .. hi-1 of this segment
     const bool col = i < C.g.Nx;
-    T t[kHySegLevels];
-    T run = T(0);
+    const bool diff = C.s.dxpHY != nullptr;
+    // terms m = hi-1 down to lo; b(m+1) at m = Nz-1 is the halo value b(Nz)
+    T sp = T(0), sx = T(0), sy = T(0);
     if (col) {
-        // terms m = hi-1 down to lo; b(m+1) at m = Nz-1 is the halo value b(Nz)
-        T bu = C.buoy(i, j, hi);
-#pragma unroll
-        for (int q = 0; q < kHySegLevels; ++q) {
-            const int m = hi - 1 - q;
-            if (m < lo) break;
+        T bu = C.buoy(i, j, hi), xu = diff ? C.buoy_diff(i, j, hi, 1, 0) : T(0), yu = diff ? C.buoy_diff(i, j, hi, 0, 1) : T(0);
+#pragma unroll 4
+        for (int m = hi - 1; m >= lo; --m) {
             const T bm = C.buoy(i, j, m);
-            run += (T(0.5) * (bm + bu)) * C.g.dzf[m + 1];
-            t[q] = run;                                  // suffix sum of the segment from m up
+            sp += (T(0.5) * (bm + bu)) * C.g.dzf[m + 1];
             bu = bm;
+            if (diff) {
+                const T xm = C.buoy_diff(i, j, m, 1, 0), ym = C.buoy_diff(i, j, m, 0, 1);
+                sx += (T(0.5) * (xm + xu)) * C.g.dzf[m + 1];
+                sy += (T(0.5) * (ym + yu)) * C.g.dzf[m + 1];
+                xu = xm;
+                yu = ym;
+            }
         }
     }
-    tot[seg][threadIdx.x] = run;
+    tot[0][seg][threadIdx.x] = sp;
+    tot[1][seg][threadIdx.x] = sx;
+    tot[2][seg][threadIdx.x] = sy;
     __syncthreads();
     if (!col) return;
-    T above = T(0);
-    for (int s2 = seg + 1; s2 < 4; ++s2) above += tot[s2][threadIdx.x];
-#pragma unroll
-    for (int q = 0; q < kHySegLevels; ++q) {
-        const int m = hi - 1 - q;
-        if (m < lo) break;
-        C.s.pHY[C.at(i, j, m)] = -(above + t[q]);
+    T ap = T(0), ax = T(0), ay = T(0);
+    for (int s2 = seg + 1; s2 < 4; ++s2) {
+        ap += tot[0][s2][threadIdx.x];
+        ax += tot[1][s2][threadIdx.x];
+        ay += tot[2][s2][threadIdx.x];
+    }
+    T rp = T(0), rx = T(0), ry = T(0);
+    T bu = C.buoy(i, j, hi), xu = diff ? C.buoy_diff(i, j, hi, 1, 0) : T(0), yu = diff ? C.buoy_diff(i, j, hi, 0, 1) : T(0);
+#pragma unroll 4
+    for (int m = hi - 1; m >= lo; --m) {
+        const T bm = C.buoy(i, j, m);
+        rp += (T(0.5) * (bm + bu)) * C.g.dzf[m + 1];
+        bu = bm;
+        C.s.pHY[C.at(i, j, m)] = -(ap + rp);
+        if (diff) {
+            const T xm = C.buoy_diff(i, j, m, 1, 0), ym = C.buoy_diff(i, j, m, 0, 1);
+            rx += (T(0.5) * (xm + xu)) * C.g.dzf[m + 1];
+            ry += (T(0.5) * (ym + yu)) * C.g.dzf[m + 1];
+            xu = xm;
+            yu = ym;
+            C.s.dxpHY[C.at(i, j, m)] = -(ax + rx);
+            C.s.dypHY[C.at(i, j, m)] = -(ay + ry);
+        }
     }
 }
 
@@ -737,8 +835,15 @@ template <class T> void launch_closure(const Ctx<T> &C, hipStream_t st, bool fix
     hipLaunchKernelGGL(k_closure<T>, cell_grid(C.g.Nx, C.g.Ny, C.g.Nz, blk), blk, 0, st, C, 0);
 }
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
-    if (sizeof(T) == 4 && C.g.Nz >= 8 && C.g.Nz <= 4 * kHySegLevels) {
-        // FP32: four z segments per column scanned in parallel (the FP64 order is the reference's)
+    if (sizeof(T) == 4 && C.g.Nz <= 8 * kHyLevels) {
+        // FP32: eight z segments per column scanned in parallel, running sums in registers (the
+        // FP64 order is the reference's), with the horizontal differences of pHY′ where held; the
+        // block shape does not depend on the grid, so ensemble members of any size share it
+        launch_k<KHydrostaticReg<T>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 8, 1), 0, st, CtxArgs<T>{C});
+        return;
+    }
+    if (sizeof(T) == 4) {
+        // FP32, Nz > 8·kHyLevels: four segments, two passes
         dim3 blk(64, 4, 1);
         launch_k<KHydrostaticSeg<T>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), blk, 0, st, CtxArgs<T>{C});
         return;

@@ -386,7 +386,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                     const T I = half * (ca + cb);
                     G += (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
-                if (ph.hydrostatic) G -= (at(C.s.pHY, 0) - at(C.s.pHY, -1)) * g.idx;
+                if (ph.hydrostatic) G -= (C.s.dxpHY ? at(C.s.dxpHY, 0) : at(C.s.pHY, 0) - at(C.s.pHY, -1)) * g.idx;
                 if (has_clo) { diffs(0, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
                 at(C.s.Gn[0], 0) = G;
             }
@@ -401,7 +401,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                     const T I = half * (ca + cb);
                     G -= (half * (ph.f + ph.f)) * I * recip<T>(A);
                 }
-                if (ph.hydrostatic) G -= (at(C.s.pHY, 0) - at(C.s.pHY, -g.sy)) * g.idy;
+                if (ph.hydrostatic) G -= (C.s.dypHY ? at(C.s.dypHY, 0) : at(C.s.pHY, 0) - at(C.s.pHY, -g.sy)) * g.idy;
                 if (has_clo) { diffs(1, sx, sy, sz, 1); G -= iVc * ((sx + sy) + sz); }
                 at(C.s.Gn[1], 0) = G;
             }

@@ -23,13 +23,15 @@ OM_SOLVER_FFT, OM_SOLVER_FOURIER_TRIDIAGONAL, OM_SOLVER_CONJUGATE_GRADIENT = 0, 
 OM_PRECOND_DEFAULT, OM_PRECOND_NONE, OM_PRECOND_DIAGONALLY_DOMINANT, OM_PRECOND_FFT = 0, 1, 2, 3
 OM_PRECOND_COLUMNWISE_TRIDIAGONAL = 4
 OM_PRECOND_LEARNED = 5
-OM_LEARNED_NWEIGHTS = 4 * 27 * 16 * 16
+OM_LEARNED_LAYERS = 7
+OM_LEARNED_NWEIGHTS = OM_LEARNED_LAYERS * 27 * 16 * 16
 OM_TUNE_TENDENCY_KERNEL = 0
 OM_TUNE_KCHUNK = 1
 OM_TUNE_LEARNED_ITERATIONS = 2
 OM_TUNE_POISSON_REFINEMENT = 3
 OM_TUNE_STEP_GRAPHS = 4
 OM_TUNE_CONV_TILED = 5
+OM_TUNE_WGRAD_MFMA = 6
 OM_ENSEMBLE_DIRECT, OM_ENSEMBLE_MEMBER_GRAPHS, OM_ENSEMBLE_BATCHED = 0, 1, 2
 PHASES = ("rk_update", "poisson", "correction", "halos", "closure", "hydrostatic", "tendencies")
 OM_NPHASES = len(PHASES)
@@ -138,6 +140,7 @@ SIGNATURES = {
     "om_learned_last_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "om_learned_energy_loss": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)]),
+    "om_learned_pairs_logged": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "om_field_statistics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "om_field_statistics_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,

@@ -350,13 +350,15 @@ class NonhydrostaticModel:
         return it.value, r.value, t.value
 
     def set_tuning(self, tendency_kernel=None, kchunk=None, learned_iterations=None, poisson_refinement=None,
-                   step_graphs=None, conv_tiled=None):
+                   step_graphs=None, conv_tiled=None, wgrad_mfma=None):
         """Launch-shape choices: tendency_kernel 'tiled' (default where supported) or 'generic';
         kchunk = z levels per block of the tiled kernel; step_graphs: time_step replays a captured
         two-step hipGraph (default) or issues every launch; conv_tiled: the learned preconditioner's
         convolutions with LDS-staged input tiles (default) or not.  Results do not depend on them.
         poisson_refinement: one FP64 iterative-refinement step after each FFT / Fourier–tridiagonal
         Poisson solve (off by default)."""
+        if wgrad_mfma is not None:
+            _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_WGRAD_MFMA, 1 if wgrad_mfma else 0), self._h)
         if conv_tiled is not None:
             _abi.check(self._lib.om_set_tuning(self._h, _abi.OM_TUNE_CONV_TILED, 1 if conv_tiled else 0), self._h)
         if step_graphs is not None:
@@ -433,7 +435,7 @@ class NonhydrostaticModel:
     def learned_get_weights(self):
         w = np.empty(_abi.OM_LEARNED_NWEIGHTS, dtype=np.float32)
         _abi.check(self._lib.om_learned_get_weights(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), self._h)
-        return w.reshape(4, 27, 16, 16)
+        return w.reshape(_abi.OM_LEARNED_LAYERS, 27, 16, 16)
 
     def learned_log(self, iterations_per_solve=4, capacity=64):
         _abi.check(self._lib.om_learned_log(self._h, int(iterations_per_solve), int(capacity)), self._h)
@@ -441,6 +443,12 @@ class NonhydrostaticModel:
     def learned_pairs(self):
         n = ctypes.c_int32()
         _abi.check(self._lib.om_learned_pairs(self._h, ctypes.byref(n)), self._h)
+        return n.value
+
+    def learned_pairs_logged(self):
+        """Pairs logged since learned_log (the ring holds the last `capacity` of them)."""
+        n = ctypes.c_int64()
+        _abi.check(self._lib.om_learned_pairs_logged(self._h, ctypes.byref(n)), self._h)
         return n.value
 
     def learned_train(self, steps, lr=1e-3):
@@ -468,7 +476,7 @@ class NonhydrostaticModel:
         return np.transpose(out, (2, 1, 0))
 
     def learned_backward(self, input3, target, mask):
-        """(loss, dL/dW (4, 27, 16, 16), dL/d(input) (Nx, Ny, Nz, 3)) of the mean-square loss."""
+        """(loss, dL/dW (7, 27, 16, 16), dL/d(input) (Nx, Ny, Nz, 3)) of the mean-square loss."""
         g = self.grid
         tr = lambda a: np.ascontiguousarray(np.transpose(np.asarray(a, dtype=np.float32), (2, 1, 0) + tuple(range(3, np.ndim(a)))))
         x, t, m = tr(input3), tr(target), tr(mask)
@@ -477,7 +485,7 @@ class NonhydrostaticModel:
         loss = ctypes.c_double()
         _abi.check(self._lib.om_learned_backward(self._h, self._fp(x), self._fp(t), self._fp(m), ctypes.byref(loss),
                                                  self._fp(dw), self._fp(din)), self._h)
-        return loss.value, dw.reshape(4, 27, 16, 16), np.transpose(din, (2, 1, 0, 3))
+        return loss.value, dw.reshape(_abi.OM_LEARNED_LAYERS, 27, 16, 16), np.transpose(din, (2, 1, 0, 3))
 
     def learned_energy_loss(self, output, target):
         """(E, dE/dy) of the training loss's energy for a network output y and target t, both

@@ -144,10 +144,11 @@ class ColumnwiseTridiagonalPreconditioner:
 
 class LearnedPreconditioner:
     """The FFT preconditioner of the underlying grid plus a learned convolutional correction
-    (four 3×3×3 convolutions of 16 channels, BF16 on the matrix cores), applied inside flexible CG
+    (a four-level hierarchy of seven 3×3×3 convolutions of 16 channels on the grid and on the grid
+    coarsened 2×, 4× and 8×, BF16 on the matrix cores), applied inside flexible CG
     with a fallback to the FFT preconditioner when convergence stalls.  No reference counterpart:
     it takes the place of the preconditioner call of conjugate_gradient_poisson_solver.jl:169-211.
-    `weights`: optional (4, 27, 16, 16) FP32 array [layer][tap][cin][cout]; zero weights give the
+    `weights`: optional (7, 27, 16, 16) FP32 array [layer][tap][cin][cout]; zero weights give the
     FFT preconditioner."""
 
     def __init__(self, weights=None):

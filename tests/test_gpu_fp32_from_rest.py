@@ -4,11 +4,14 @@ the FP64 CPU oracle on the same FP32-rounded inputs.
 
 Conditioning.  From rest, the velocities after 10 steps (~1e-8 m/s) are driven by buoyancy
 anomalies of ~1e-8 on a background of up to 1.3e-3, i.e. at a few FP32 ulps of the field they
-sit in.  The reference's own algorithm evaluated in FP32 (the oracle's FP32 mode, same inputs)
-lands about 6e-3 from FP64 at 16³, and FP64 itself moves by ~8e-4 when its inputs are perturbed
-at FP32 epsilon.  No FP32 evaluation can therefore hold the velocities to 1e-5 here; the bar
-this test holds is the reference algorithm's own FP32 error (the HIP FP32 path may be no farther
-from FP64 than that), and 1e-5 for the buoyancy, which is well conditioned.
+sit in, and FP64 itself moves by ~8e-4 when its inputs are perturbed at FP32 epsilon.  The
+reference's own algorithm evaluated in FP32 (the oracle's FP32 mode, same inputs) lands 6e-3 from
+FP64 at 16³ and 0.15 at 256×256×128: it forms the horizontal gradient of the hydrostatic pressure
+anomaly as the difference of two column sums of b that agree to a few ulps.  The HIP FP32 path
+forms that gradient as column sums of b's horizontal differences (exact in FP32), which brings
+its velocities to ~1e-4 of FP64 at both sizes, below the problem's own sensitivity to FP32 input
+rounding; the 1e-5 bar holds for the buoyancy, which is well conditioned, and the velocity bars
+are the measured ones.
 """
 import numpy as np
 import pytest
@@ -40,13 +43,16 @@ def test_fp32_from_rest_16cubed_against_oracle():
           "| oracle FP32 vs FP64 oracle", {k: f"{v:.2e}" for k, v in ref32.items()}, flush=True)
     assert gpu["b"] <= 1e-5
     for n in ("u", "v", "w"):
-        assert gpu[n] <= ref32[n], (n, gpu[n], ref32[n])
+        # measured 8.1e-5, 7.9e-5, 1.4e-4 against the reference algorithm's 6.5e-3, 6.2e-3, 4.6e-3
+        assert gpu[n] <= 3e-4 and gpu[n] <= ref32[n] / 10, (n, gpu[n], ref32[n])
     model.close()
 
 
 # measured at 256×256×128 (FP32 HIP path, 10 steps from rest, max|a − b| / max|b| against the FP64
-# oracle): velocities 0.62–0.63 with the FP32 FFT solve, which the bound below holds with margin
-PLAIN_FP32_VELOCITY_BOUND = 0.8
+# oracle): velocities 8.4e-5 (u), 8.8e-5 (v), 1.5e-4 (w); the reference algorithm in FP32 (the
+# oracle's FP32 mode) is 0.15–0.16 off, from the horizontal gradient of pHY′ taken as a difference of
+# two rounded column sums, which the HIP path forms as column sums of buoyancy differences instead
+PLAIN_FP32_VELOCITY_BOUND = 3e-4
 
 
 def test_fp32_from_rest_full_size_against_oracle():
@@ -54,8 +60,9 @@ def test_fp32_from_rest_full_size_against_oracle():
     FP32 mode on the same FP32-rounded inputs (each about a minute of the host's cores).  The
     oracle's FP32 mode evaluates the tendencies in FP32 but its Poisson solve in FP64; the HIP path
     with one FP64 iterative-refinement step per Poisson solve is the matching comparison and must
-    stay within a factor 2 of it (two FP32 evaluations that round differently).  The plain FP32 HIP
-    path (FP32 FFTs, as the benchmark runs) is held to its measured bound; the buoyancy to 1e-5."""
+    stay within a factor 2 of it.  The plain FP32 HIP path (FP32 FFTs, as the benchmark runs) is held
+    to its measured bound, which is more than a hundred times closer to FP64 than the reference
+    algorithm's FP32 evaluation; the buoyancy to 1e-5."""
     N = (256, 256, 128)
     grid32, kw, init = convection(N=N, float_type=np.float32)
     grid64, kw64, _ = convection(N=N, float_type=np.float64)
@@ -84,3 +91,4 @@ def test_fp32_from_rest_full_size_against_oracle():
     for n in ("u", "v", "w"):
         assert errs["refined"][n] <= 2 * ref32[n], (n, errs["refined"][n], ref32[n])
         assert errs["plain"][n] <= PLAIN_FP32_VELOCITY_BOUND, (n, errs["plain"][n])
+        assert errs["plain"][n] <= ref32[n] / 100, (n, errs["plain"][n], ref32[n])

@@ -16,7 +16,7 @@ from oracle import learned as L
 pytestmark = pytest.mark.gpu
 
 
-def _net_model(N=(32, 8, 6)):
+def _net_model(N=(128, 8, 8)):
     grid = om.RectilinearGrid(size=N, extent=(1.0, 0.5, 0.4))
     return om.NonhydrostaticModel(grid, advection=om.WENO(),
                                   pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=om.LearnedPreconditioner()))
@@ -59,14 +59,56 @@ def test_backward_matches_oracle():
     out, acts = L.forward(x3, w)
     loss_o, dout = L.loss_and_grad_output(out, t, m)
     dwo, gino = L.backward(acts, w, dout.astype(np.float32))
-    errs = [_rel(dw[l], dwo[l]) for l in range(4)]
+    errs = [_rel(dw[l], dwo[l]) for l in range(L.NL)]
     ein = _rel(din, gino[..., :3])
     print("loss", loss, loss_o, "dW rel err per layer", errs, "input grad", ein, flush=True)
     assert loss == pytest.approx(loss_o, rel=2e-3)
     assert max(errs) <= 2e-3 and ein <= 2e-3
 
 
-def _seamount_models(pc_a, pc_b, N=(32, 32, 16)):
+@pytest.mark.parametrize("N", [(128, 8, 8), (256, 24, 16)])
+def test_tiled_and_per_tap_convolutions_agree_bitwise(N):
+    """The convolutions with LDS-staged input tiles and the ones that read every tap from memory use
+    the same operands in the same MFMA order: forward outputs and every gradient agree bit for bit
+    (Ny = 24, Nz = 16 leave partly filled tiles on the coarse levels)."""
+    model = _net_model(N)
+    model.learned_set_weights(L.init_weights(9, scale=1.0))
+    x3 = _inputs(N, 4)
+    t = np.random.default_rng(5).normal(size=N).astype(np.float32)
+    res = {}
+    for tiled in (True, False):
+        model.set_tuning(conv_tiled=tiled)
+        res[tiled] = (model.learned_forward(x3),) + tuple(model.learned_backward(x3, t, x3[..., 2]))
+    for a, b in zip(res[True], res[False]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    model.close()
+
+
+def test_matrix_core_weight_gradients_match_the_fp32_sums():
+    """The weight gradients on the matrix cores (the default) and as FP32 sums: the same BF16
+    products summed in different orders, equal to FP32 rounding; both against the oracle."""
+    N = (256, 16, 16)
+    model = _net_model(N)
+    w = L.init_weights(12, scale=1.0)
+    model.learned_set_weights(w)
+    x3 = _inputs(N, 6)
+    t = np.random.default_rng(7).normal(size=N).astype(np.float32)
+    grads = {}
+    for mfma in (True, False):
+        model.set_tuning(wgrad_mfma=mfma)
+        grads[mfma] = model.learned_backward(x3, t, x3[..., 2])[1]
+    out, st = L.forward(x3, w)
+    _, dout = L.loss_and_grad_output(out, t, x3[..., 2])
+    dwo, _ = L.backward(st, w, dout.astype(np.float32))
+    between = [_rel(grads[True][l], grads[False][l]) for l in range(L.NL)]
+    vs_oracle = [_rel(grads[True][l], dwo[l]) for l in range(L.NL)]
+    print("MFMA vs FP32-sum weight gradients per layer", between, "MFMA vs oracle", vs_oracle, flush=True)
+    assert max(between) <= 1e-5
+    assert max(vs_oracle) <= 2e-3
+    model.close()
+
+
+def _seamount_models(pc_a, pc_b, N=(128, 32, 16)):
     grid, kw, init = seamount(N=N)
     ka = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=pc_a, reltol=1e-10, abstol=0))
     kb = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=pc_b, reltol=1e-10, abstol=0))
@@ -83,7 +125,7 @@ def test_training_gain_on_held_out_solves_and_flexible_cg_matches_fft_cg():
     FP64 residual stays within a hundred times the recursive one; the iteration counts are reported."""
     learned, fft = _seamount_models(om.LearnedPreconditioner(), om.FourierTridiagonalPoissonSolver())
     w0 = L.init_weights(7)
-    w0[3] = 0.0
+    w0[L.NL - 1] = 0.0
     learned.learned_set_weights(w0)
     # rounds of logging and training, so that the pairs include residuals of the network's own solves
     learned.learned_log(iterations_per_solve=8, capacity=96)
@@ -119,7 +161,7 @@ def test_untrained_large_weights_hand_over_to_fft():
     hands over to the FFT preconditioner (three iterations without progress, or the recomputed FP64
     residual above a hundred times the tolerance), and the state still matches FFT-preconditioned CG."""
     learned, fft = _seamount_models(om.LearnedPreconditioner(weights=L.init_weights(11, scale=50.0)),
-                                    om.FourierTridiagonalPoissonSolver(), N=(16, 16, 12))
+                                    om.FourierTridiagonalPoissonSolver(), N=(128, 16, 16))
     for _ in range(2):
         om.time_step_(learned, 20.0)
         om.time_step_(fft, 20.0)
@@ -135,7 +177,7 @@ def test_energy_loss_and_gradient_match_oracle():
     solver's conditional operator on the immersed seamount, and dE/dy, against the NumPy
     restatement (oracle/learned.py energy, itself pinned to the oracle CG operator's quadratic
     form and to finite differences)."""
-    grid, kw, init = seamount(N=(32, 32, 16))
+    grid, kw, init = seamount(N=(128, 16, 16))
     kw = dict(kw, pressure_solver=om.ConjugateGradientPoissonSolver(preconditioner=om.LearnedPreconditioner()))
     model = om.NonhydrostaticModel(grid, **kw)
     from oracle import oracle as O

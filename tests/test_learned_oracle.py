@@ -39,33 +39,55 @@ def test_convolution_against_brute_force():
 
 
 def test_gradients_against_finite_differences():
+    """Every layer's weight gradient (all seven, on all four levels) and the input gradient of the
+    four-level network against central differences of the loss in FP64."""
     rng = np.random.default_rng(2)
-    N = (6, 5, 4)
+    N = (128, 8, 8)
     x3 = rng.normal(size=N + (3,))
     x3[..., 2] = (rng.random(N) > 0.2)
     w = L.init_weights(3, scale=1.0).astype(np.float64)
     t = rng.normal(size=N)
     m = x3[..., 2].copy()
-    out, acts = L.forward(x3, w, precise=True)
+    out, st = L.forward(x3, w, precise=True)
     loss, dout = L.loss_and_grad_output(out, t, m)
-    dw, gin = L.backward(acts, w, dout, precise=True)
+    dw, gin = L.backward(st, w, dout, precise=True)
 
     def lossof(ww, xx):
         return L.loss_and_grad_output(L.forward(xx, ww, True)[0], t, m)[0]
 
     e = 1e-6
-    for idx in [(0, 13, 0, 5), (0, 2, 2, 9), (1, 4, 7, 2), (2, 26, 15, 15), (3, 0, 3, 0), (3, 13, 9, 0)]:
+    for idx in [(0, 13, 0, 5), (0, 2, 2, 9), (1, 4, 7, 2), (2, 26, 15, 15), (3, 13, 9, 4), (4, 0, 3, 0),
+                (5, 22, 11, 6), (6, 0, 3, 0), (6, 13, 9, 0)]:
         wp, wm = w.copy(), w.copy()
         wp[idx] += e
         wm[idx] -= e
         fd = (lossof(wp, x3) - lossof(wm, x3)) / (2 * e)
-        assert dw[idx] == pytest.approx(fd, rel=1e-6, abs=1e-10)
-    for v in [(1, 2, 3, 0), (0, 0, 0, 1), (5, 4, 0, 2)]:
+        assert dw[idx] == pytest.approx(fd, rel=1e-6, abs=1e-10), idx
+    for v in [(1, 2, 3, 0), (0, 0, 0, 1), (127, 7, 7, 2), (64, 4, 5, 0)]:
         xp, xm = x3.copy(), x3.copy()
         xp[v] += e
         xm[v] -= e
         fd = (lossof(w, xp) - lossof(w, xm)) / (2 * e)
-        assert gin[v] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+        assert gin[v] == pytest.approx(fd, rel=1e-6, abs=1e-10), v
+
+
+def test_level_transfers_against_brute_force():
+    """pool is the mean of the 2×2×2 children, up copies the parent, and child_sum is up's adjoint:
+    <up(c), f> = <c, child_sum(f)>."""
+    rng = np.random.default_rng(5)
+    f = rng.normal(size=(8, 6, 4, 3))
+    p = L.pool(f, np.float64)
+    for i, j, k in [(0, 0, 0), (3, 2, 1), (2, 1, 0)]:
+        assert p[i, j, k] == pytest.approx(f[2 * i:2 * i + 2, 2 * j:2 * j + 2, 2 * k:2 * k + 2].mean(axis=(0, 1, 2)), rel=1e-14)
+    c = rng.normal(size=(4, 3, 2, 3))
+    u = L.up(c)
+    assert u.shape == f.shape and u[5, 3, 2, 1] == c[2, 1, 1, 1]
+    assert np.sum(u * f) == pytest.approx(np.sum(c * L.child_sum(f, np.float64)), rel=1e-12)
+
+
+def test_network_sizes_are_checked():
+    with pytest.raises(ValueError, match="multiple of 128"):
+        L.forward(np.zeros((64, 8, 8, 3)), L.init_weights(0))
 
 
 def test_first_adam_step_closed_form():
