@@ -172,8 +172,8 @@ def run_cg_seamount(args):
     learned.learned_log(iterations_per_solve=args.cg_log_iterations, capacity=args.cg_pairs)
     t0 = time.perf_counter()
     for rnd in range(args.cg_rounds):
-        learned.time_step(20.0, 1)
-        l0, l1 = learned.learned_train(args.cg_train_steps, lr=5e-4)
+        learned.time_step(args.cg_dt, 1)
+        l0, l1 = learned.learned_train(args.cg_train_steps, lr=args.cg_lr)
         if rnd % 20 == 0 or rnd == args.cg_rounds - 1:
             print(f"bench.py: seamount training step {rnd}: energy loss {l0:.3f} -> {l1:.3f}, "
                   f"{learned.learned_pairs_logged()} pairs logged", file=sys.stderr, flush=True)
@@ -187,7 +187,7 @@ def run_cg_seamount(args):
     res = {}
     for name, m in (("fft", fft), ("learned", learned)):
         m.set_phase_timing(True)
-        m.time_step(20.0, 1)
+        m.time_step(args.cg_dt, 1)
         ph = m.phase_times()
         it, r, true_r = m.last_solve_info()
         res[name] = {"ms_per_solve": ph["poisson"][0] / 3, "iterations_last_solve": it, "recomputed_residual": true_r,
@@ -198,7 +198,7 @@ def run_cg_seamount(args):
     learned.close()
     return {"grid": "x".join(map(str, N)), "reltol": 1e-8, "fft": res["fft"], "learned": res["learned"],
             "learned_over_fft_time": res["learned"]["ms_per_solve"] / res["fft"]["ms_per_solve"],
-            "training": {"time_steps": args.cg_rounds, "adam_steps_per_time_step": args.cg_train_steps,
+            "dt": args.cg_dt, "training": {"time_steps": args.cg_rounds, "adam_steps_per_time_step": args.cg_train_steps, "lr": args.cg_lr,
                          "pairs_logged": pairs_logged, "pair_ring_capacity": args.cg_pairs,
                          "iterations_logged_per_solve": args.cg_log_iterations, "wall_s": t_train},
             "network_iterations_per_solve": args.cg_learned_iterations or "all"}
@@ -329,6 +329,9 @@ def main():
     ap.add_argument("--cg-train-steps", type=int, default=10, help="Adam steps after each logged time step")
     ap.add_argument("--cg-pairs", type=int, default=96, help="capacity of the ring of logged pairs")
     ap.add_argument("--cg-log-iterations", type=int, default=3, help="leading iterations of each solve that log a pair")
+    ap.add_argument("--cg-lr", type=float, default=1e-4, help="Adam learning rate of the online training")
+    ap.add_argument("--cg-dt", type=float, default=10.0,
+                    help="time step of the seamount run (s); its flow speeds up near the seamount, to ~1 m/s by t = 2000 s")
     ap.add_argument("--cg-learned-iterations", type=int, default=2,
                     help="leading CG iterations that apply the network (0: all)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-chunk length of the tiled kernels (0: autotune)")

@@ -104,7 +104,7 @@ extern "C" {
                                               in flexible CG with an FFT fallback (no reference counterpart;
                                               it replaces the preconditioner call of
                                               conjugate_gradient_poisson_solver.jl:169-211); Nx a multiple
-                                              of 16, Bounded z */
+                                              of 128, Ny and Nz multiples of 8, Bounded z */
 
 #define OM_MAX_TRACERS 8
 #define OM_MAX_FIELDS  (3 + OM_MAX_TRACERS)

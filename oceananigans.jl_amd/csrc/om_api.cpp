@@ -199,6 +199,8 @@ template <class T> struct ModelT : ModelBase {
     float *ln_in3 = nullptr, *ln_maskd = nullptr;  // current network input (3 channels, dense), active mask
     double ln_nactive = 0;
     double *cg_rprev = nullptr;                    // r of the previous iteration (flexible CG)
+    double *cg_xsave = nullptr;                    // the warm start of a learned solve, for a restart
+    double cg_res0_last = 0;                       // ‖V⁻¹(b − A x₀)‖ of the last solve's warm start
     int ln_learned_its = 0, ln_fell_back = 0;
     int log_iters = 0, log_cap = 0, log_count = 0, log_next = 0;   // pair logging (om_learned_log)
     int64_t log_total = 0;                                         // pairs logged since om_learned_log
@@ -500,6 +502,7 @@ template <class T> struct ModelT : ModelBase {
             if (cg_precond() == OM_PRECOND_COLUMNWISE_TRIDIAGONAL) cg_t = dalloc<T>((size_t)g.Nx * g.Ny * g.Nz);
             if (cg_precond() == OM_PRECOND_LEARNED) {
                 cg_rprev = field_origin(dalloc<double>(fsize));
+                cg_xsave = field_origin(dalloc<double>(fsize));
                 net_alloc();
             }
             HIPCHECK(hipHostMalloc((void **)&cg_host, 4 * sizeof(double), hipHostMallocDefault));
@@ -725,8 +728,12 @@ template <class T> struct ModelT : ModelBase {
     void note_unconverged(int64_t it, double res, double tol) {
         ++cg_unconverged;
         char buf[256];
-        snprintf(buf, sizeof buf, "conjugate-gradient solve stopped at maxiter = %lld with |r| = %.3e above the tolerance %.3e "
-                 "(%d solve(s) of this call; the step completed as the reference's does)", (long long)it, res, tol, cg_unconverged);
+        if (std::isfinite(res))
+            snprintf(buf, sizeof buf, "conjugate-gradient solve stopped at maxiter = %lld with |r| = %.3e above the tolerance %.3e "
+                     "(%d solve(s) of this call; the step completed as the reference's does)", (long long)it, res, tol, cg_unconverged);
+        else
+            snprintf(buf, sizeof buf, "conjugate-gradient solve stopped after %lld iterations with a non-finite residual "
+                     "(%d solve(s) of this call)", (long long)it, cg_unconverged);
         cg_unconverged_msg = buf;
     }
     // throws NotConverged (after the call's work is done) when a solve of the call stopped at maxiter
@@ -735,15 +742,20 @@ template <class T> struct ModelT : ModelBase {
     }
     void cg_solve() {
         const int pk = cg_precond();
+        if (pk == OM_PRECOND_LEARNED) cg_copy_d(g, cg_x, cg_xsave, st);   // the warm start, kept for a restart
         cg_solve_with(pk);
         // the recomputed FP64 residual judges the learned preconditioner: a solve it leaves above
-        // a hundred times the tolerance is finished by FFT-preconditioned CG from where it stopped
-        // (FFT-preconditioned CG itself ends with the recomputed residual up to ~15× the recursive
-        // one: the usual drift of the recursively updated residual)
+        // a hundred times the tolerance is finished by FFT-preconditioned CG, from where it stopped
+        // or from the warm start, whichever has the smaller residual (FFT-preconditioned CG itself
+        // ends with the recomputed residual up to ~15× the recursive one: the usual drift of the
+        // recursively updated residual)
         if (pk == OM_PRECOND_LEARNED && !(cg_true_res_last <= 100.0 * std::max(cg_tol_last, 1e-300))) {
             const int it0 = cg_iterations;
             const double tol0 = cg_tol_last;
             ln_fell_back = 1;
+            // from the learned iterate when it improved on the warm start, else from the warm start
+            // (a diverged iterate is too large for FP64 residuals to recover from)
+            if (!(cg_true_res_last < cg_res0_last)) cg_copy_d(g, cg_xsave, cg_x, st);
             cg_solve_with(OM_PRECOND_FFT, tol0);   // the original absolute tolerance, not one relative to the small residual left
             cg_iterations += it0;
         }
@@ -760,6 +772,7 @@ template <class T> struct ModelT : ModelBase {
         cg_halo_r(g, x, st);
         cg_residual(g, x, cg_b, cg_r, cg_partial, cg_out, st);
         const double res0 = std::sqrt(cg_read(1));
+        cg_res0_last = res0;
         const double tol = tol_override >= 0 ? tol_override : std::max(reltol * res0, abstol);
         cg_tol_last = tol;
         // with the FFT or learned preconditioner the gauge sweeps also write its source r V⁻¹
@@ -809,6 +822,10 @@ template <class T> struct ModelT : ModelBase {
             cg_gauge_norm(g, x, cg_p, cg_r, cg_q, cg_out, itp, 1, ncell, fsrc, cg_partial, cg_out + kCgSlotGauge, st);
             res = std::sqrt(cg_read_slot(kCgSlotGauge));
             it += 1;
+            if (!std::isfinite(res)) break;   // reported as not converged; iterating on cannot recover
+            // a learned correction that sends the flexible iteration off (‖V⁻¹r‖ a thousand times its
+            // best) ends the solve: cg_solve restarts it from the warm start with the FFT preconditioner
+            if (flex && res > 1e3 * res_best) break;
             if (flex && !fallback) {
                 stall = res < 0.999 * res_best ? 0 : stall + 1;
                 if (stall >= 3) { fallback = true; ln_fell_back = 1; }

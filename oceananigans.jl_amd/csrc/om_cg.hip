@@ -455,9 +455,13 @@ template <class T> __global__ void __launch_bounds__(256) k_ln_input(DevGrid<T> 
     });
 }
 // z += s_z · active · (network output)
+// z += s_z · m · N: a non-finite network output (an overflowed activation) adds nothing, so the
+// preconditioned residual stays finite and the solve's stall test can hand over to the FFT
+// preconditioner
 template <class T> __global__ void __launch_bounds__(256) k_ln_correct(DevGrid<T> g, T *__restrict__ z, const float *__restrict__ out, double sz) {
     for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
-        z[o] = (T)((double)z[o] + sz * active(g, i, j, k) * (double)out[t]);
+        const float y = out[t];
+        if (isfinite(y)) z[o] = (T)((double)z[o] + sz * active(g, i, j, k) * (double)y);
     });
 }
 // interior copy of an FP64 cell field

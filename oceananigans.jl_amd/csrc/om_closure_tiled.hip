@@ -89,22 +89,29 @@ __device__ __forceinline__ void KClosureTiled<T, NT>::run(const ClosureTiledArgs
     stage_ring(S.w[k0 & 1], C.s.U[2], g, i0, j0, k0, tid);
 
     const T half = T(0.5);
+    // The planes a level adds (u, v, w at k + 1, the tracers at k) are loaded into registers while
+    // the previous level is evaluated, so their memory latency is hidden behind that work; the
+    // plane shares are rebuilt for each load from an opaque copy of tid.
+    T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2];
+    auto load_level = [&](int k) {
+        int otid = tid;
+        asm volatile("" : "+v"(otid));
+        const PlaneShare<T, CW> ps(g, i0, j0, 1, otid);
+        ps.load(ru, ps.origin(C.s.U[0], g, i0, j0, 1, k + 1));
+        ps.load(rv, ps.origin(C.s.U[1], g, i0, j0, 1, k + 1));
+        ps.load(rw, ps.origin(C.s.U[2], g, i0, j0, 1, k + 1));
+#pragma unroll
+        for (int n = 0; n < NT; ++n) ps.load(rc[n], ps.origin(C.s.U[3 + n], g, i0, j0, 1, k));
+    };
+    load_level(k0);
     for (int k = k0; k < k1; ++k) {
         int otid = tid;
         asm volatile("" : "+v"(otid));
         colb = colb0;
         asm volatile("" : "+v"(colb));
-        // every load of the level is issued before the barrier and the LDS stores
         const PlaneShare<T, CW> ps(g, i0, j0, 1, otid);
-        T ru[2], rv[2], rw[2], rc[NT > 0 ? NT : 1][2];
-        ps.load(ru, ps.origin(C.s.U[0], g, i0, j0, 1, k + 1));
-        ps.load(rv, ps.origin(C.s.U[1], g, i0, j0, 1, k + 1));
-        ps.load(rw, ps.origin(C.s.U[2], g, i0, j0, 1, k + 1));
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            ps.load(rc[n], ps.origin(C.s.U[3 + n], g, i0, j0, 1, k));
-            cw[n][0] = cw[n][1]; cw[n][1] = cw[n][2]; cw[n][2] = ldcol(C.s.U[3 + n], k + 1);
-        }
+        for (int n = 0; n < NT; ++n) { cw[n][0] = cw[n][1]; cw[n][1] = cw[n][2]; cw[n][2] = ldcol(C.s.U[3 + n], k + 1); }
         __syncthreads();   // previous level's readers are done
         ps.store(S.u[slot3(k + 1)], ru, tid);
         ps.store(S.v[slot3(k + 1)], rv, tid);
@@ -112,6 +119,7 @@ __device__ __forceinline__ void KClosureTiled<T, NT>::run(const ClosureTiledArgs
 #pragma unroll
         for (int n = 0; n < NT; ++n) ps.store(S.c[n], rc[n], tid);
         __syncthreads();
+        if (k + 1 < k1) load_level(k + 1);
         if (!cell) continue;
 
         const T *Um = S.u[slot3(k - 1)], *U0 = S.u[slot3(k)], *Up = S.u[slot3(k + 1)];

@@ -113,13 +113,18 @@ template <> __device__ __forceinline__ float weno5<float>(float s0, float s1, fl
     // differences from each stencil's middle value (weno_interpolants.jl:276-281); neighbouring
     // stencils share them: b's right difference is −xa, c's is −xb
     const float xa = t2 - t3, za = t4 - t3, xb = t1 - t2, xc = t0 - t1;
-    const float ba = fmaf(xa, fmaf(10.0f, xa, 11.0f * za), 4.0f * za * za);
-    const float bb = fmaf(xb, fmaf(4.0f, xb, -5.0f * xa), 4.0f * xa * xa);
-    const float bc = fmaf(xc, fmaf(4.0f, xc, -11.0f * xb), 10.0f * xb * xb);
+    // The smoothness indicators and ε scaled by 1/4 (exact), which leaves every τ / (β + ε) as it
+    // is: β_a/4 = 2.5 xa² + za (2.75 xa + za), β_b/4 = xb (xb − 1.25 xa) + xa²,
+    // β_c/4 = 2.5 xb² + xc (xc − 2.75 xb), with xa² and xb² each formed once
+    const float sa = xa * xa, sb = xb * xb;
+    const float ba = fmaf(2.5f, sa, za * fmaf(2.75f, xa, za));
+    const float bb = fmaf(xb, fmaf(-1.25f, xa, xb), sa);
+    const float bc = fmaf(2.5f, sb, xc * fmaf(-2.75f, xb, xc));
     const float tau = fabsf(ba - bc);
-    const float qa = tau * __builtin_amdgcn_rcpf(ba + K::eps);
-    const float qb = tau * __builtin_amdgcn_rcpf(bb + K::eps);
-    const float qc = tau * __builtin_amdgcn_rcpf(bc + K::eps);
+    constexpr float eps4 = K::eps * 0.25f;
+    const float qa = tau * __builtin_amdgcn_rcpf(ba + eps4);
+    const float qb = tau * __builtin_amdgcn_rcpf(bb + eps4);
+    const float qc = tau * __builtin_amdgcn_rcpf(bc + eps4);
     // the weights are normalised at the end, so the three αₖ may share any factor: here 1 / C₅₁,
     // which saves the middle stencil's multiplication (C₅₀ / C₅₁ = 1/2, C₅₂ / C₅₁ = 1/6)
     const float al0 = 0.5f * fmaf(qa, qa, 1.0f), al1 = fmaf(qb, qb, 1.0f), al2 = (1.0f / 6.0f) * fmaf(qc, qc, 1.0f);

@@ -504,7 +504,8 @@ __global__ void __launch_bounds__(256) k_loss_sum(const double *partial, double 
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master weights; the padded input channels of
-// layer 1 and the unused output channels of layer 4 stay zero
+// the first layer and the unused output channels of the last stay zero.  A non-finite gradient
+// (an overflowed activation) leaves its weight and moments as they are.
 __global__ void __launch_bounds__(256) k_adam(float *w, float *m, float *v, const float *g, int step, float lr, float b1, float b2,
                                               float eps) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -512,6 +513,7 @@ __global__ void __launch_bounds__(256) k_adam(float *w, float *m, float *v, cons
     const int l = q / NET_WPL, r = q % NET_WPL, c = (r / 16) % 16, o = r % 16;
     if ((l == 0 && c >= 3) || (l == NET_LAYERS - 1 && o >= 1)) return;
     const float gq = g[q];
+    if (!isfinite(gq)) return;
     const float mq = b1 * m[q] + (1.f - b1) * gq;
     const float vq = b2 * v[q] + (1.f - b2) * gq * gq;
     m[q] = mq;
