@@ -200,7 +200,6 @@ template <class T> struct ModelT : ModelBase {
     double ln_nactive = 0;
     double *cg_rprev = nullptr;                    // r of the previous iteration (flexible CG)
     double *cg_xsave = nullptr;                    // the warm start of a learned solve, for a restart
-    double cg_res0_last = 0;                       // ‖V⁻¹(b − A x₀)‖ of the last solve's warm start
     int ln_learned_its = 0, ln_fell_back = 0;
     int log_iters = 0, log_cap = 0, log_count = 0, log_next = 0;   // pair logging (om_learned_log)
     int64_t log_total = 0;                                         // pairs logged since om_learned_log
@@ -745,17 +744,15 @@ template <class T> struct ModelT : ModelBase {
         if (pk == OM_PRECOND_LEARNED) cg_copy_d(g, cg_x, cg_xsave, st);   // the warm start, kept for a restart
         cg_solve_with(pk);
         // the recomputed FP64 residual judges the learned preconditioner: a solve it leaves above
-        // a hundred times the tolerance is finished by FFT-preconditioned CG, from where it stopped
-        // or from the warm start, whichever has the smaller residual (FFT-preconditioned CG itself
-        // ends with the recomputed residual up to ~15× the recursive one: the usual drift of the
-        // recursively updated residual)
+        // a hundred times the tolerance is redone by FFT-preconditioned CG from the warm start
+        // (FFT-preconditioned CG itself ends with the recomputed residual up to ~15× the recursive
+        // one: the usual drift of the recursively updated residual).  Continuing from the learned
+        // iterate instead was seen to diverge on the 256×256×128 seamount.
         if (pk == OM_PRECOND_LEARNED && !(cg_true_res_last <= 100.0 * std::max(cg_tol_last, 1e-300))) {
             const int it0 = cg_iterations;
             const double tol0 = cg_tol_last;
             ln_fell_back = 1;
-            // from the learned iterate when it improved on the warm start, else from the warm start
-            // (a diverged iterate is too large for FP64 residuals to recover from)
-            if (!(cg_true_res_last < cg_res0_last)) cg_copy_d(g, cg_xsave, cg_x, st);
+            cg_copy_d(g, cg_xsave, cg_x, st);
             cg_solve_with(OM_PRECOND_FFT, tol0);   // the original absolute tolerance, not one relative to the small residual left
             cg_iterations += it0;
         }
@@ -772,7 +769,6 @@ template <class T> struct ModelT : ModelBase {
         cg_halo_r(g, x, st);
         cg_residual(g, x, cg_b, cg_r, cg_partial, cg_out, st);
         const double res0 = std::sqrt(cg_read(1));
-        cg_res0_last = res0;
         const double tol = tol_override >= 0 ? tol_override : std::max(reltol * res0, abstol);
         cg_tol_last = tol;
         // with the FFT or learned preconditioner the gauge sweeps also write its source r V⁻¹

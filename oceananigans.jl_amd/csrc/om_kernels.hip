@@ -321,7 +321,8 @@ template <class T> __device__ __forceinline__ void KHydrostatic<T>::run(const Ct
 // where b varies horizontally by ~1e-5 of itself, and differencing the rounded sums would lose
 // most of it; b's neighbour differences are exact in FP32 (Sterbenz) and their sums keep FP32's
 // relative precision.
-// KHydrostaticReg (eight segments of at most kHyLevels levels, Nz ≤ 8·kHyLevels): each thread forms its
+// KHydrostaticReg<T, S> (S segments of at most kHyLevels levels, one wave of 64 columns each,
+// Nz ≤ S·kHyLevels; S = 8 up to 128 levels, 16 up to 256): each thread forms its
 // segment's terms ½(b(m) + b(m+1))·Δzᶠ(m+1) from the top down with their running sums in registers,
 // the segment totals are exchanged through LDS, and p(m) = −(terms of the segments above + the
 // running sum at m).  KHydrostaticSeg (any Nz, four segments): the same in two passes, the first
@@ -329,12 +330,12 @@ template <class T> __device__ __forceinline__ void KHydrostatic<T>::run(const Ct
 // in segments, so FP32 results differ from the sequential order by rounding only.
 constexpr int kHyLevels = 16;
 
-template <class T> struct KHydrostaticReg {
-    static constexpr int kThreads = 512, kMinBlocks = 1;
+template <class T, int S> struct KHydrostaticReg {
+    static constexpr int kThreads = 64 * S, kMinBlocks = 1;
     static __device__ __forceinline__ void run(const CtxArgs<T> &A, const Blk &B);
 };
-template <class T> __device__ __forceinline__ void KHydrostaticReg<T>::run(const CtxArgs<T> &A, const Blk &B) {
-    __shared__ T tot[3][8][64];
+template <class T, int S> __device__ __forceinline__ void KHydrostaticReg<T, S>::run(const CtxArgs<T> &A, const Blk &B) {
+    __shared__ T tot[3][S][64];
     const Ctx<T> &C = A.C;
     const int i = B.x * 64 + threadIdx.x;
     const int j = B.y;
@@ -838,8 +839,13 @@ template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
     if (sizeof(T) == 4 && C.g.Nz <= 8 * kHyLevels) {
         // FP32: eight z segments per column scanned in parallel, running sums in registers (the
         // FP64 order is the reference's), with the horizontal differences of pHY′ where held; the
-        // block shape does not depend on the grid, so ensemble members of any size share it
-        launch_k<KHydrostaticReg<T>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 8, 1), 0, st, CtxArgs<T>{C});
+        // block shape does not depend on the grid, so ensemble members of any size up to 128
+        // levels share it
+        launch_k<KHydrostaticReg<T, 8>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 8, 1), 0, st, CtxArgs<T>{C});
+        return;
+    }
+    if (sizeof(T) == 4 && C.g.Nz <= 16 * kHyLevels) {   // sixteen segments, 1024-thread blocks
+        launch_k<KHydrostaticReg<T, 16>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 16, 1), 0, st, CtxArgs<T>{C});
         return;
     }
     if (sizeof(T) == 4) {

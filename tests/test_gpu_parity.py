@@ -240,6 +240,25 @@ def test_fp32_ten_steps_against_fp64_oracle():
     compare_state(model, orc, kw, fields=("state",), tol=1e-5, label="fp32 wind 10 steps")
 
 
+@pytest.mark.parametrize("nz", [100, 200, 300])
+def test_fp32_column_scan_forms_match_the_oracle(nz):
+    """The FP32 pHY′ column scan in its three forms (eight register segments up to 128 levels,
+    sixteen up to 256, two passes beyond) against the FP64 oracle on the same FP32-rounded state:
+    pHY′ itself, and the u and v tendencies, which take ∂x pHY′ and ∂y pHY′ from the scan's
+    buoyancy-difference sums."""
+    grid, kw, init = convection(N=(16, 16, nz), noise_velocity=1e-2, float_type=np.float32)
+    model = product_for(grid, kw, init)
+    orc = oracle_for(grid, kw, init, oracle_float_type=np.float64, round_inputs_to=np.float32)
+    names = prognostic_names(kw)
+    e_p = rel_err(model.pressures.pHY.interior(), orc.get(om._abi.OM_FIELD_PHY))
+    e_g = {n: tend_err(model.Gn[n].interior(), orc.get(om._abi.OM_FIELD_GN + q), flux_scale(orc, grid, names, q))
+           for q, n in enumerate(names[:2])}
+    print(f"Nz {nz}: pHY rel err {e_p:.2e}, Gu, Gv {e_g}", flush=True)
+    # measured: pHY′ 1.2e-7 … 2.5e-7, Gu and Gv 1.0e-6 … 2.2e-6
+    assert e_p <= 1e-6 and all(e <= 1e-5 for e in e_g.values())
+    model.close()
+
+
 def test_fp32_free_convection_ten_steps():
     """The free-convection LES in FP32 (the benchmark's physics at 16³, with velocity noise) within
     1e-5 of the FP64 oracle after 10 steps, on FP32-rounded identical inputs."""
@@ -474,5 +493,5 @@ def test_mixed_precision_wind_convection_full_size():
     print("512x512x256, 3 steps, FP32 vs FP64 model: plain", {n: f"{e:.2e}" for n, e in errs[False].items()},
           "refined", {n: f"{e:.2e}" for n, e in errs[True].items()}, flush=True)
     assert divs[True] <= 1e-5 and divs[True] <= 2 * divs[False]
-    for refine in (False, True):
-        assert all(e <= 1e-4 for e in errs[refine].values()), (refine, errs[refine])
+    for refine in (False, True):   # measured: u 3.6e-6, v 3.3e-6, w 1.0e-5, T 3.8e-7 in both
+        assert all(e <= 3e-5 for e in errs[refine].values()), (refine, errs[refine])
