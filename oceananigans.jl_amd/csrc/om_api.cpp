@@ -679,15 +679,15 @@ template <class T> struct ModelT : ModelBase {
         ln_nactive = 0;
         for (float v : hm) ln_nactive += v;
     }
-    // scales s_r = rms(r V⁻¹), s_z = rms(z) over the active cells and the network input for (r, z)
-    bool ln_prepare(double &isz) {
+    // the network input's scales 1/s_r, 1/s_z (s_r = rms(r V⁻¹), s_z = rms(z_FFT) over the active cells)
+    bool ln_prepare(double &isr, double &isz) {
         ln_scales(g, cg_r, cg_z, cg_partial, cg_out, st);
         cg_read(3);
         const double na = std::max(cg_host[2], 1.0);
         const double sr = std::sqrt(cg_host[0] / na), sz = std::sqrt(cg_host[1] / na);
         if (!(sr > 0) || !(sz > 0)) return false;
+        isr = 1.0 / sr;
         isz = 1.0 / sz;
-        ln_input(g, cg_r, cg_z, 1.0 / sr, isz, ln_in3, st);
         return true;
     }
     void ln_log_snapshot(int k, double isz) {
@@ -793,11 +793,19 @@ template <class T> struct ModelT : ModelBase {
                 const bool log_this = logging && it < log_iters;
                 const bool learn_this = pk == OM_PRECOND_LEARNED && !fallback &&
                                         (tune_learned_iterations == 0 || it < tune_learned_iterations);
-                double isz = 0;
-                if ((log_this || learn_this) && ln_prepare(isz)) {
-                    if (log_this) { ln_log_snapshot((int)it, isz); nlogged = (int)it + 1; }
+                double isr = 0, isz = 0;
+                if ((log_this || learn_this) && ln_prepare(isr, isz)) {
+                    if (log_this) {   // the dense input is kept with the pair
+                        ln_input(g, cg_r, cg_z, isr, isz, ln_in3, st);
+                        ln_log_snapshot((int)it, isz);
+                        nlogged = (int)it + 1;
+                    }
                     if (learn_this) {
-                        net_input_dense(net, ln_in3, st);
+                        if (log_this) net_input_dense(net, ln_in3, st);
+                        else {
+                            ln_x0(g, cg_r, cg_z, isr, isz, net.b.x0, st);
+                            net_input_halo(net, st);
+                        }
                         net_forward(net, net.out, st);
                         ln_correct(g, cg_z, net.out, 1.0 / isz, st);
                         ++ln_learned_its;

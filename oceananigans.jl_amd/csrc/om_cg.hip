@@ -455,6 +455,24 @@ template <class T> __global__ void __launch_bounds__(256) k_ln_input(DevGrid<T> 
     });
 }
 // z += s_z · active · (network output)
+// the same three channels written straight into the network's padded BF16 input box (16 channels
+// per voxel, 3..15 zero; halos filled afterwards by the network's halo kernel): each channel is the
+// FP32 value k_ln_input stores, rounded to BF16 as the dense input's conversion rounds it
+template <class T> __global__ void __launch_bounds__(256) k_ln_x0(DevGrid<T> g, const double *__restrict__ r, const T *__restrict__ z,
+                                                                  double isr, double isz, __bf16 *__restrict__ x0) {
+    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+    for_cells(g, [&](int i, int j, int k, int64_t o, int64_t t) {
+        bf16x8 a, b;
+        for (int e = 0; e < 8; ++e) { a[e] = (__bf16)0.0f; b[e] = (__bf16)0.0f; }
+        a[0] = (__bf16)(float)(r[o] * (double)(T(1) / vol(g, k)) * isr);
+        a[1] = (__bf16)(float)((double)z[o] * isz);
+        a[2] = (__bf16)(float)active(g, i, j, k);
+        const int64_t v = ((int64_t)(k + 1) * (g.Ny + 2) + (j + 1)) * (int64_t)(g.Nx + 2) + (i + 1);
+        __bf16 *p = x0 + v * 16;
+        *reinterpret_cast<bf16x8 *>(p) = a;
+        *reinterpret_cast<bf16x8 *>(p + 8) = b;
+    });
+}
 // z += s_z · m · N: a non-finite network output (an overflowed activation) adds nothing, so the
 // preconditioned residual stays finite and the solve's stall test can hand over to the FFT
 // preconditioner
@@ -610,6 +628,9 @@ template <class T> void ln_scales(const DevGrid<T> &g, const double *r, const T 
 template <class T> void ln_input(const DevGrid<T> &g, const double *r, const T *z, double isr, double isz, float *in3, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_input<T>, red_grid(), red_blk(), 0, st, g, r, z, isr, isz, in3);
 }
+template <class T> void ln_x0(const DevGrid<T> &g, const double *r, const T *z, double isr, double isz, void *x0, hipStream_t st) {
+    hipLaunchKernelGGL(k_ln_x0<T>, red_grid(), red_blk(), 0, st, g, r, z, isr, isz, (__bf16 *)x0);
+}
 template <class T> void ln_correct(const DevGrid<T> &g, T *z, const float *out, double sz, hipStream_t st) {
     hipLaunchKernelGGL(k_ln_correct<T>, red_grid(), red_blk(), 0, st, g, z, out, sz);
 }
@@ -680,6 +701,7 @@ template <class T> void cg_gauged_norm(const DevGrid<T> &g, const double *e, dou
     template void cg_ct<T>(const DevGrid<T> &, const double *, T *, T *, hipStream_t);                          \
     template void ln_scales<T>(const DevGrid<T> &, const double *, const T *, double *, double *, hipStream_t);    \
     template void ln_input<T>(const DevGrid<T> &, const double *, const T *, double, double, float *, hipStream_t); \
+    template void ln_x0<T>(const DevGrid<T> &, const double *, const T *, double, double, void *, hipStream_t);     \
     template void ln_correct<T>(const DevGrid<T> &, T *, const float *, double, hipStream_t);                    \
     template void cg_copy_d<T>(const DevGrid<T> &, const double *, double *, hipStream_t);                       \
     template void cg_dot2<T>(const DevGrid<T> &, const T *, const double *, const double *, double *, double *, hipStream_t); \

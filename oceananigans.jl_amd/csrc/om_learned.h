@@ -53,6 +53,7 @@ void net_place_buffers(NetState &S, void *arena);
 
 void net_pack(NetState &S, hipStream_t st);
 void net_input_dense(NetState &S, const float *in3, hipStream_t st);
+void net_input_halo(NetState &S, hipStream_t st);   // halos of the input box, its interior written by the caller
 void net_forward(NetState &S, float *out, hipStream_t st);
 void net_backward(NetState &S, float *din, hipStream_t st);
 void net_loss(NetState &S, const float *out, const float *target, const float *mask, float inv_n, hipStream_t st);

@@ -106,31 +106,35 @@ __global__ void __launch_bounds__(256) k_conv(NetDims d, const bf16 *__restrict_
 }
 
 // -------------------------------------------------------------------------------------------
-// The same convolution with the input staged in LDS.  A block of 4 waves computes a tile of
-// 16 (x) × 4 (y) × 4 (z) voxels: its 18 × 6 × 6 input voxels (one-cell halo, 16 channels, 20.7 KB)
-// are loaded from the padded activation once, instead of once per tap and wave, and each wave
-// (one z level of the tile) reuses every weight fragment for its four 16-voxel rows.  Same
-// operands, same MFMA sequence per output: the results equal k_conv's bit for bit.
+// The same convolution with the input staged in LDS.  A block of 8 waves computes a tile of
+// 16 (x) × 8 (y) × 8 (z) voxels: its 18 × 10 × 10 input voxels (one-cell halo, 16 channels,
+// 57.6 KB) are loaded from the padded activation once, instead of once per tap and wave (1.76×
+// the voxels the tile produces; a 16 × 4 × 4 tile stages 2.5×), and each wave (one z level of the
+// tile) reuses every weight fragment for its eight 16-voxel rows.  Same operands, same MFMA
+// sequence per output: the results equal k_conv's bit for bit.
 // -------------------------------------------------------------------------------------------
-constexpr int CT_X = 16, CT_Y = 4, CT_Z = 4;
+constexpr int CT_X = 16, CT_Y = 4, CT_Z = 4;   // the weight-gradient kernel's tile (below)
 constexpr int CT_SX = CT_X + 2, CT_SY = CT_Y + 2, CT_SZ = CT_Z + 2;
 constexpr int CT_V = CT_SX * CT_SY * CT_SZ;   // staged voxels per tile
+constexpr int CV_X = 16, CV_Y = 8, CV_Z = 8, CV_THREADS = 64 * CV_Z;   // the convolution's tile
+constexpr int CV_SX = CV_X + 2, CV_SY = CV_Y + 2, CV_SZ = CV_Z + 2;
+constexpr int CV_V = CV_SX * CV_SY * CV_SZ;
 
 template <int EPI>
-__global__ void __launch_bounds__(256) k_conv_tiled(NetDims d, const bf16 *__restrict__ x, const bf16x8 *__restrict__ wp,
-                                                    bf16 *__restrict__ ybf, float *__restrict__ yf, const bf16 *__restrict__ act) {
-    __shared__ bf16x8 tile[CT_V * 2];
-    const int ntx = d.nx / CT_X, nty = (d.ny + CT_Y - 1) / CT_Y;
+__global__ void __launch_bounds__(CV_THREADS) k_conv_tiled(NetDims d, const bf16 *__restrict__ x, const bf16x8 *__restrict__ wp,
+                                                           bf16 *__restrict__ ybf, float *__restrict__ yf, const bf16 *__restrict__ act) {
+    __shared__ bf16x8 tile[CV_V * 2];
+    const int ntx = d.nx / CV_X, nty = (d.ny + CV_Y - 1) / CV_Y;
     const int b = blockIdx.x;
     const int tx = b % ntx, rest = b / ntx;
     const int ty = rest % nty, tz = rest / nty;
-    const int i0 = tx * CT_X, j0 = ty * CT_Y, k0 = tz * CT_Z;
+    const int i0 = tx * CV_X, j0 = ty * CV_Y, k0 = tz * CV_Z;
     // stage the tile: voxel (xx, yy, zz) of the tile is padded voxel (i0 − 1 + xx, j0 − 1 + yy, k0 − 1 + zz);
     // rows and levels past the padded box's last (ny, nz) are staged as zeros and never used
-    for (int q = threadIdx.x; q < CT_V * 2; q += 256) {
+    for (int q = threadIdx.x; q < CV_V * 2; q += CV_THREADS) {
         const int v = q >> 1, h = q & 1;
-        const int xx = v % CT_SX, r = v / CT_SX;
-        const int yy = r % CT_SY, zz = r / CT_SY;
+        const int xx = v % CV_SX, r = v / CV_SX;
+        const int yy = r % CV_SY, zz = r / CV_SY;
         const int i = i0 - 1 + xx, j = j0 - 1 + yy, k = k0 - 1 + zz;
         bf16x8 val;
         if (j <= d.ny && k <= d.nz) {
@@ -147,9 +151,9 @@ __global__ void __launch_bounds__(256) k_conv_tiled(NetDims d, const bf16 *__res
     const int vi = lane & 15;
     const int half = lane >> 5;                 // which of the step's two taps
     const int c0h = (lane >> 4) & 1;            // which 8 input channels
-    f32x4 acc[CT_Y];
+    f32x4 acc[CV_Y];
 #pragma unroll
-    for (int sgm = 0; sgm < CT_Y; ++sgm) acc[sgm] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int sgm = 0; sgm < CV_Y; ++sgm) acc[sgm] = f32x4{0.f, 0.f, 0.f, 0.f};
     bf16x8 zero;
     for (int e = 0; e < 8; ++e) zero[e] = (bf16)0.0f;
 #pragma unroll 2
@@ -158,16 +162,16 @@ __global__ void __launch_bounds__(256) k_conv_tiled(NetDims d, const bf16 *__res
         int dx = 0, dy = 0, dz = 0;
         if (t < 27) tap_offsets(t, dx, dy, dz);
         const bf16x8 A = wp[s * 64 + lane];
-        const int base = ((w + 1 + dz) * CT_SY + (1 + dy)) * CT_SX + (vi + 1 + dx);
+        const int base = ((w + 1 + dz) * CV_SY + (1 + dy)) * CV_SX + (vi + 1 + dx);
 #pragma unroll
-        for (int sgm = 0; sgm < CT_Y; ++sgm) {
-            const bf16x8 Bv = t < 27 ? tile[(base + sgm * CT_SX) * 2 + c0h] : zero;
+        for (int sgm = 0; sgm < CV_Y; ++sgm) {
+            const bf16x8 Bv = t < 27 ? tile[(base + sgm * CV_SX) * 2 + c0h] : zero;
             acc[sgm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bv, acc[sgm], 0, 0, 0);
         }
     }
     const int cout0 = (lane >> 4) << 2;
 #pragma unroll
-    for (int sgm = 0; sgm < CT_Y; ++sgm) {
+    for (int sgm = 0; sgm < CV_Y; ++sgm) {
         const int j = j0 + sgm;
         if (j >= d.ny) break;
         const int iv = i0 + vi;
@@ -267,27 +271,53 @@ __global__ void __launch_bounds__(256) k_up_combine(NetDims df, const bf16 *gski
     *reinterpret_cast<bf16x8 *>(out + o) = r;
 }
 
-// halo of a padded activation: x, y periodic (copied from the opposite interior edge), z zero
+// halo of a padded activation: x, y periodic (copied from the opposite interior edge), z zero.
+// Threads enumerate the halo voxels only, two per voxel (8 channels each): the two z faces (all
+// of their (nx + 2)(ny + 2) voxels, zeros), then the two y faces of the interior levels, then the
+// two x faces of the interior levels and rows.
 __global__ void __launch_bounds__(256) k_net_halo(NetDims d, bf16 *a) {
-    const int64_t nxy = (int64_t)(d.nx + 2) * (d.ny + 2);
-    const int64_t n = nxy * (d.nz + 2);
+    const int64_t px = d.nx + 2, py = d.ny + 2;
+    const int64_t nzf = 2 * px * py, nyf = 2 * (int64_t)d.nz * px, nxf = 2 * (int64_t)d.nz * d.ny;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= n * 2) return;
-    const int64_t v = t >> 1;                  // two threads per voxel: 8 channels each
+    const int64_t v = t >> 1;
     const int hh = (int)(t & 1);
-    const int k = (int)(v / nxy) - 1;
-    const int64_t r = v % nxy;
-    const int j = (int)(r / (d.nx + 2)) - 1, i = (int)(r % (d.nx + 2)) - 1;
-    const bool in_x = i >= 0 && i < d.nx, in_y = j >= 0 && j < d.ny, in_z = k >= 0 && k < d.nz;
-    if (in_x && in_y && in_z) return;
+    int i, j, k;
+    bool zero = false;
+    if (v < nzf) {
+        const int64_t per = px * py;
+        const bool top = v >= per;
+        const int64_t r = top ? v - per : v;
+        k = top ? d.nz : -1;
+        j = (int)(r / px) - 1;
+        i = (int)(r - (r / px) * px) - 1;
+        zero = true;
+    } else if (v < nzf + nyf) {
+        const int64_t per = (int64_t)d.nz * px;
+        int64_t u = v - nzf;
+        const bool hi = u >= per;
+        if (hi) u -= per;
+        j = hi ? d.ny : -1;
+        k = (int)(u / px);
+        i = (int)(u - (int64_t)k * px) - 1;
+    } else if (v < nzf + nyf + nxf) {
+        const int64_t per = (int64_t)d.nz * d.ny;
+        int64_t u = v - nzf - nyf;
+        const bool hi = u >= per;
+        if (hi) u -= per;
+        i = hi ? d.nx : -1;
+        k = (int)(u / d.ny);
+        j = (int)(u - (int64_t)k * d.ny);
+    } else {
+        return;
+    }
     bf16x8 val;
-    if (!in_z) {
+    if (zero) {
         for (int e = 0; e < 8; ++e) val[e] = (bf16)0.0f;
     } else {
         const int ii = (i + d.nx) % d.nx, jj = (j + d.ny) % d.ny;
         val = *reinterpret_cast<const bf16x8 *>(a + pidx(d, ii, jj, k) * 16 + hh * 8);
     }
-    *reinterpret_cast<bf16x8 *>(a + v * 16 + hh * 8) = val;
+    *reinterpret_cast<bf16x8 *>(a + pidx(d, i, j, k) * 16 + hh * 8) = val;
 }
 
 // weight fragments for the MFMA A operand: wp[s][lane][e] = W[t][c][o] with t = 2s + lane/32,
@@ -557,8 +587,8 @@ static void conv_launch(const NetState &S, const NetDims &d, int epi, const void
     bf16 *Y = (bf16 *)ybf;
     const bf16 *Ac = (const bf16 *)act;
     if (S.tiled_conv) {
-        const int64_t tiles = (int64_t)(d.nx / CT_X) * ((d.ny + CT_Y - 1) / CT_Y) * ((d.nz + CT_Z - 1) / CT_Z);
-        const dim3 g((unsigned)tiles), b(256);
+        const int64_t tiles = (int64_t)(d.nx / CV_X) * ((d.ny + CV_Y - 1) / CV_Y) * ((d.nz + CV_Z - 1) / CV_Z);
+        const dim3 g((unsigned)tiles), b(CV_THREADS);
         switch (epi) {
             case EPI_RELU: hipLaunchKernelGGL(k_conv_tiled<EPI_RELU>, g, b, 0, st, d, X, W, Y, yf, Ac); break;
             case EPI_OUT: hipLaunchKernelGGL(k_conv_tiled<EPI_OUT>, g, b, 0, st, d, X, W, Y, yf, Ac); break;
@@ -582,7 +612,8 @@ static void conv_launch(const NetState &S, const NetDims &d, int epi, const void
 static NetDims level(const NetDims &d, int l) { return NetDims{d.nx >> l, d.ny >> l, d.nz >> l}; }
 
 static void halo(const NetDims &d, void *a, hipStream_t st) {
-    const int64_t n = (int64_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 2) * 2;
+    const int64_t px = d.nx + 2, py = d.ny + 2;
+    const int64_t n = 2 * (2 * px * py + 2 * (int64_t)d.nz * px + 2 * (int64_t)d.nz * d.ny);   // 2 threads per halo voxel
     hipLaunchKernelGGL(k_net_halo, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, (bf16 *)a);
 }
 
@@ -616,6 +647,7 @@ void net_input_dense(NetState &S, const float *in3, hipStream_t st) {
     hipLaunchKernelGGL(k_net_input, dim3(blocks_for(nv, 256)), dim3(256), 0, st, S.d, in3, (bf16 *)S.b.x0);
     halo(S.d, S.b.x0, st);
 }
+void net_input_halo(NetState &S, hipStream_t st) { halo(S.d, S.b.x0, st); }
 
 static const bf16 *wf_layer(const NetState &S, int l) { return (const bf16 *)S.wf + (size_t)l * 14 * 64 * 8; }
 static const bf16 *wb_layer(const NetState &S, int l) { return (const bf16 *)S.wb + (size_t)l * 14 * 64 * 8; }

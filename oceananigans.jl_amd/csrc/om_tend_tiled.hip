@@ -173,8 +173,8 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
             T nfcf, ncff, nccc;
             if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nfcf = ncff = nccc = ph.c0; }
             else {
-                nfcf = half * (half * (Q(N0, -1, 0) + Q(N0, 0, 0)) + half * (Q(N1, -1, 0) + Q(N1, 0, 0)));
-                ncff = half * (half * (Q(N0, 0, -1) + Q(N0, 0, 0)) + half * (Q(N1, 0, -1) + Q(N1, 0, 0)));
+                nfcf = avg4<T>(Q(N0, -1, 0), Q(N0, 0, 0), Q(N1, -1, 0), Q(N1, 0, 0));
+                ncff = avg4<T>(Q(N0, 0, -1), Q(N0, 0, 0), Q(N1, 0, -1), Q(N1, 0, 0));
                 nccc = Q(N0, 0, 0);
             }
             const T s13 = half * ((wu[3] - wu[2]) * idzf1 + (P(W1, 0, 0) - P(W1, -1, 0)) * g.idx);
@@ -268,7 +268,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
         T nffc_e = T(0), s12_e = T(0);
         if (has_clo) {
             nffc_e = clo == OM_CLOSURE_SCALAR_DIFFUSIVITY ? ph.c0
-                   : half * (half * (Q(N0, -1, -1) + Q(N0, 0, -1)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
+                   : avg4<T>(Q(N0, -1, -1), Q(N0, 0, -1), Q(N0, -1, 0), Q(N0, 0, 0));
             s12_e = half * ((P(U, 0, 0) - P(U, 0, -1)) * g.idy + (P(V, 0, 0) - P(V, -1, 0)) * g.idx);
         }
         // ---------------- x faces (thread a: u flux at centre i0+a-1, others at face i0+a)
@@ -298,7 +298,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                 if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = nfcf = ph.c0; }
                 else {
                     nccc = Q(N0, -1, 0);
-                    nfcf = half * (half * (Q(Nm, -1, 0) + Q(Nm, 0, 0)) + half * (Q(N0, -1, 0) + Q(N0, 0, 0)));
+                    nfcf = avg4<T>(Q(Nm, -1, 0), Q(Nm, 0, 0), Q(N0, -1, 0), Q(N0, 0, 0));
                 }
                 S.fx[1][tid] = A * (T(-2) * (nccc * ((P(U, 0, 0) - P(U, -1, 0)) * g.idx)));
                 S.fx[3][tid] = A * (T(-2) * (nffc_e * s12_e));
@@ -342,7 +342,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                 if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = ncff = ph.c0; }
                 else {
                     nccc = Q(N0, 0, -1);
-                    ncff = half * (half * (Q(Nm, 0, -1) + Q(Nm, 0, 0)) + half * (Q(N0, 0, -1) + Q(N0, 0, 0)));
+                    ncff = avg4<T>(Q(Nm, 0, -1), Q(Nm, 0, 0), Q(N0, 0, -1), Q(N0, 0, 0));
                 }
                 S.fy[1][tid] = A * (T(-2) * (nffc_e * s12_e));
                 S.fy[3][tid] = A * (T(-2) * (nccc * ((P(V, 0, 0) - P(V, 0, -1)) * g.idy)));

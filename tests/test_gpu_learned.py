@@ -156,6 +156,26 @@ def test_training_gain_on_held_out_solves_and_flexible_cg_matches_fft_cg():
     assert true_res <= 100 * res + 1e-30 or true_res <= 1e-8
 
 
+def test_network_input_written_directly_equals_the_dense_input_path():
+    """In flexible CG the network input is written straight into its padded BF16 box, except in the
+    iterations that log a pair, which keep the dense FP32 input and convert it.  Two models step from
+    the same state with the same weights, one logging its first 8 iterations per solve and one not:
+    the same channels reach the network either way, so the states are bit-identical."""
+    w = L.init_weights(5)
+    a, b = _seamount_models(om.LearnedPreconditioner(weights=w), om.LearnedPreconditioner(weights=w), N=(128, 16, 16))
+    a.learned_log(iterations_per_solve=8, capacity=32)
+    for _ in range(2):
+        om.time_step_(a, 20.0)
+        om.time_step_(b, 20.0)
+    print("iterations logged / direct", a.last_solve_info(), b.last_solve_info(), a.learned_last_solve(),
+          b.learned_last_solve(), flush=True)
+    for n in ("u", "v", "w"):
+        assert np.array_equal(a.velocities[n].interior(), b.velocities[n].interior()), n
+    assert np.array_equal(a.pressures.pNHS.interior(), b.pressures.pNHS.interior())
+    a.close()
+    b.close()
+
+
 def test_untrained_large_weights_hand_over_to_fft():
     """A network far from useful (large random weights, output layer scaled up): flexible CG stalls,
     hands over to the FFT preconditioner (three iterations without progress, or the recomputed FP64
