@@ -337,8 +337,12 @@ template <class T, int S> struct KHydrostaticReg {
 template <class T, int S> __device__ __forceinline__ void KHydrostaticReg<T, S>::run(const CtxArgs<T> &A, const Blk &B) {
     __shared__ T tot[3][S][64];
     const Ctx<T> &C = A.C;
-    const int i = B.x * 64 + threadIdx.x;
-    const int j = B.y;
+    // a 1-D grid of (64-column strip, row) items in row-major order, each XCD taking one contiguous
+    // run of them: a row's y-neighbour row j − 1 was read by the same XCD just before (its L2)
+    const int nbx = (C.g.Nx + 63) / 64;
+    const int item = xcd_work_item((int)B.x, (int)B.gx);
+    const int i = (item % nbx) * 64 + threadIdx.x;
+    const int j = item / nbx;
     const int seg = threadIdx.y, nseg = blockDim.y;
     const int Nz = C.g.Nz;
     const int per = (Nz + nseg - 1) / nseg;
@@ -841,11 +845,11 @@ template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st) {
         // FP64 order is the reference's), with the horizontal differences of pHY′ where held; the
         // block shape does not depend on the grid, so ensemble members of any size up to 128
         // levels share it
-        launch_k<KHydrostaticReg<T, 8>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 8, 1), 0, st, CtxArgs<T>{C});
+        launch_k<KHydrostaticReg<T, 8>>(dim3((C.g.Nx + 63) / 64 * C.g.Ny), dim3(64, 8, 1), 0, st, CtxArgs<T>{C});
         return;
     }
     if (sizeof(T) == 4 && C.g.Nz <= 16 * kHyLevels) {   // sixteen segments, 1024-thread blocks
-        launch_k<KHydrostaticReg<T, 16>>(dim3((C.g.Nx + 63) / 64, C.g.Ny), dim3(64, 16, 1), 0, st, CtxArgs<T>{C});
+        launch_k<KHydrostaticReg<T, 16>>(dim3((C.g.Nx + 63) / 64 * C.g.Ny), dim3(64, 16, 1), 0, st, CtxArgs<T>{C});
         return;
     }
     if (sizeof(T) == 4) {
