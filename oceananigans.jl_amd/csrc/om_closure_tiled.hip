@@ -104,6 +104,10 @@ __device__ __forceinline__ void KClosureTiled<T, NT>::run(const ClosureTiledArgs
         for (int n = 0; n < NT; ++n) ps.load(rc[n], ps.origin(C.s.U[3 + n], g, i0, j0, 1, k));
     };
     load_level(k0);
+    // AMD: the x-z and y-z gradients at level k + 1 of one level are those at level k of the next
+    // (same expressions on the same planes and ratios), so they are carried instead of recomputed
+    T c_xw[2], c_zu[2], c_ywx[2], c_yw[2], c_zv[2];
+    bool carried = false;
     for (int k = k0; k < k1; ++k) {
         int otid = tid;
         asm volatile("" : "+v"(otid));
@@ -195,25 +199,32 @@ __device__ __forceinline__ void KClosureTiled<T, NT>::run(const ClosureTiledArgs
             }
         }
         T ndxw[4], ndzu[4], ndyw_x[4];
-        {
-            // points (i,k), (i+1,k), (i,k+1), (i+1,k+1)
+        T ndyw[4], ndzv[4];
+        // x-z points (i,k), (i+1,k), (i,k+1), (i+1,k+1); y-z points (j,k), (j+1,k), (j,k+1), (j+1,k+1)
+        if (carried) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ndxw[q] = c_xw[q]; ndzu[q] = c_zu[q]; ndyw_x[q] = c_ywx[q]; ndyw[q] = c_yw[q]; ndzv[q] = c_zv[q];
+            }
+        } else {
             ndxw[0] = rxz0 * dx_w(W0, 0);            ndzu[0] = rzx0 * dz_u(U0, Um, 0, k);
             ndxw[1] = rxz0 * dx_w(W0, 1);            ndzu[1] = rzx0 * dz_u(U0, Um, 1, k);
-            ndxw[2] = rxz1 * dx_w(W1, 0);            ndzu[2] = rzx1 * dz_u(Up, U0, 0, k + 1);
-            ndxw[3] = rxz1 * dx_w(W1, 1);            ndzu[3] = rzx1 * dz_u(Up, U0, 1, k + 1);
             ndyw_x[0] = ryz0 * dy_w(W0, 0, 0);
             ndyw_x[1] = ryz0 * dy_w(W0, 1, 0);
-            ndyw_x[2] = ryz1 * dy_w(W1, 0, 0);
-            ndyw_x[3] = ryz1 * dy_w(W1, 1, 0);
-        }
-        T ndyw[4], ndzv[4];
-        {
-            // points (j,k), (j+1,k), (j,k+1), (j+1,k+1)
             ndyw[0] = ryz0 * dy_w(W0, 0, 0);         ndzv[0] = rzy0 * dz_v(V0, Vm, 0, k);
             ndyw[1] = ryz0 * dy_w(W0, 0, 1);         ndzv[1] = rzy0 * dz_v(V0, Vm, 1, k);
-            ndyw[2] = ryz1 * dy_w(W1, 0, 0);         ndzv[2] = rzy1 * dz_v(Vp, V0, 0, k + 1);
-            ndyw[3] = ryz1 * dy_w(W1, 0, 1);         ndzv[3] = rzy1 * dz_v(Vp, V0, 1, k + 1);
         }
+        ndxw[2] = rxz1 * dx_w(W1, 0);            ndzu[2] = rzx1 * dz_u(Up, U0, 0, k + 1);
+        ndxw[3] = rxz1 * dx_w(W1, 1);            ndzu[3] = rzx1 * dz_u(Up, U0, 1, k + 1);
+        ndyw_x[2] = ryz1 * dy_w(W1, 0, 0);
+        ndyw_x[3] = ryz1 * dy_w(W1, 1, 0);
+        ndyw[2] = ryz1 * dy_w(W1, 0, 0);         ndzv[2] = rzy1 * dz_v(Vp, V0, 0, k + 1);
+        ndyw[3] = ryz1 * dy_w(W1, 0, 1);         ndzv[3] = rzy1 * dz_v(Vp, V0, 1, k + 1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            c_xw[q] = ndxw[2 + q]; c_zu[q] = ndzu[2 + q]; c_ywx[q] = ndyw_x[2 + q]; c_yw[q] = ndyw[2 + q]; c_zv[q] = ndzv[2 + q];
+        }
+        carried = true;
         T nS12[4], nS13[4], nS23[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
