@@ -145,8 +145,11 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
     const T Az = g.Az;
     const T half = T(0.5);
 
-    // top z-face fluxes of level `lev` for this column (advective [0], viscous [1])
+    // top z-face fluxes of level `lev` for this column (advective [0], viscous [1]); p13, p23: the
+    // products ν·S₁₃ and ν·S₂₃ at this column's fcf and cff points of that z face, which the next
+    // level's x- and y-face viscous fluxes of w take over (the same expressions on the same data)
     T zt[3 + NT][2], zb[3 + NT][2];
+    T p13 = T(0), p23 = T(0);
     auto ztop = [&](int lev) {
         const T *W1 = S.w[(lev + 1) & 1];
         const T *N0 = S.nu[nslot(lev)], *N1 = S.nu[nslot(lev + 1)];
@@ -179,8 +182,10 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
             }
             const T s13 = half * ((wu[3] - wu[2]) * idzf1 + (P(W1, 0, 0) - P(W1, -1, 0)) * g.idx);
             const T s23 = half * ((wv[3] - wv[2]) * idzf1 + (P(W1, 0, 0) - P(W1, 0, -1)) * g.idy);
-            zt[0][1] = Az * (T(-2) * (nfcf * s13));
-            zt[1][1] = Az * (T(-2) * (ncff * s23));
+            p13 = nfcf * s13;
+            p23 = ncff * s23;
+            zt[0][1] = Az * (T(-2) * p13);
+            zt[1][1] = Az * (T(-2) * p23);
             zt[2][1] = Az * (T(-2) * (nccc * ((ww[3] - ww[2]) * g.idzc[lev])));
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
@@ -258,7 +263,7 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
         if (kk + 1 < k1) load_level(kk + 1);
 
         const T *U = S.u, *V = S.v, *W0 = S.w[kk & 1];
-        const T *Nm = S.nu[nslot(kk - 1)], *N0 = S.nu[nslot(kk)];
+        const T *N0 = S.nu[nslot(kk)];
         const T Axc = g.dy * g.dzc[kk], Ayc = g.dx * g.dzc[kk];
         const T Axf = g.dy * g.dzf[kk], Ayf = g.dx * g.dzf[kk];
         const bool s4 = C.zs4_face(kk);
@@ -294,16 +299,10 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                 S.fx[2 * (3 + n)][tid] = A * ui * weno5<T>(P(Cn, -3, 0), P(Cn, -2, 0), P(Cn, -1, 0), P(Cn, 0, 0), P(Cn, 1, 0), P(Cn, 2, 0), ui > T(0));
             }
             if (has_clo) {
-                T nccc, nfcf;
-                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = nfcf = ph.c0; }
-                else {
-                    nccc = Q(N0, -1, 0);
-                    nfcf = avg4<T>(Q(Nm, -1, 0), Q(Nm, 0, 0), Q(N0, -1, 0), Q(N0, 0, 0));
-                }
+                const T nccc = clo == OM_CLOSURE_SCALAR_DIFFUSIVITY ? ph.c0 : Q(N0, -1, 0);
                 S.fx[1][tid] = A * (T(-2) * (nccc * ((P(U, 0, 0) - P(U, -1, 0)) * g.idx)));
                 S.fx[3][tid] = A * (T(-2) * (nffc_e * s12_e));
-                const T s13 = half * ((wu[2] - wu[1]) * g.idzf[kk] + (P(W0, 0, 0) - P(W0, -1, 0)) * g.idx);
-                S.fx[5][tid] = Axf * (T(-2) * (nfcf * s13));
+                S.fx[5][tid] = Axf * (T(-2) * p13);   // ν·S₁₃ at (face i, z face kk): the previous level's top face
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
                     const T *Cn = S.c[n];
@@ -338,16 +337,10 @@ __device__ __forceinline__ void KTendTiled<T, NT>::run(const TendTiledArgs<T> &A
                 S.fy[2 * (3 + n)][tid] = A * vj * weno5<T>(P(Cn, 0, -3), P(Cn, 0, -2), P(Cn, 0, -1), P(Cn, 0, 0), P(Cn, 0, 1), P(Cn, 0, 2), vj > T(0));
             }
             if (has_clo) {
-                T nccc, ncff;
-                if (clo == OM_CLOSURE_SCALAR_DIFFUSIVITY) { nccc = ncff = ph.c0; }
-                else {
-                    nccc = Q(N0, 0, -1);
-                    ncff = avg4<T>(Q(Nm, 0, -1), Q(Nm, 0, 0), Q(N0, 0, -1), Q(N0, 0, 0));
-                }
+                const T nccc = clo == OM_CLOSURE_SCALAR_DIFFUSIVITY ? ph.c0 : Q(N0, 0, -1);
                 S.fy[1][tid] = A * (T(-2) * (nffc_e * s12_e));
                 S.fy[3][tid] = A * (T(-2) * (nccc * ((P(V, 0, 0) - P(V, 0, -1)) * g.idy)));
-                const T s23 = half * ((wv[2] - wv[1]) * g.idzf[kk] + (P(W0, 0, 0) - P(W0, 0, -1)) * g.idy);
-                S.fy[5][tid] = Ayf * (T(-2) * (ncff * s23));
+                S.fy[5][tid] = Ayf * (T(-2) * p23);   // ν·S₂₃ at (face j, z face kk): the previous level's top face
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
                     const T *Cn = S.c[n];
