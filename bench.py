@@ -390,7 +390,8 @@ def main():
     valu_issue = None
     if os.path.exists(args.traffic):
         t = json.load(open(args.traffic))
-        if t.get("kernel") == r["dom"] and t.get("grid") == "x".join(map(str, r["N"])):
+        if (t.get("kernel") == r["dom"] and t.get("grid") == "x".join(map(str, r["N"]))
+                and t.get("precision") == args.precision):
             traffic = t["hbm_bytes_per_launch"]
             if t.get("sq_insts_valu_per_launch") and r["dom_avg_ms"] > 0:
                 # the dominant kernel is issue-bound: its measured VALU wave-instructions per launch
