@@ -520,6 +520,18 @@ template <class T> struct ModelT : ModelBase {
     }
 
     // ---- halo fills
+    // p's halo fill deferred by a pressure correction that read no halo of p: it joins the next
+    // halo launch (p is not written in between), or flush_p_halo fills it at the end of a call
+    bool p_halo_pending = false;
+    void add_pending_p(HaloJob<T> &J) {
+        if (!p_halo_pending) return;
+        for (int n = 0; n < J.nf; ++n) if (J.f[n] == s.pNHS) { p_halo_pending = false; return; }
+        const int n = J.nf++;
+        J.f[n] = s.pNHS;
+        J.kind_bottom[n] = OM_BC_DEFAULT; J.kind_top[n] = OM_BC_DEFAULT;
+        p_halo_pending = false;
+    }
+    void flush_p_halo() { if (p_halo_pending) halos_aux({}); }
     void halos_prognostic(uint32_t mask, bool fill_normal) {
         HaloJob<T> J;
         memset(&J, 0, sizeof(J));
@@ -533,6 +545,7 @@ template <class T> struct ModelT : ModelBase {
                 J.val_bottom[n] = (T)cfg.bc_bottom_value[f]; J.val_top[n] = (T)cfg.bc_top_value[f];
             }
         }
+        add_pending_p(J);
         PhaseScope ps(this, OM_PHASE_HALOS);
         launch_halos(g, J, st);
     }
@@ -544,6 +557,8 @@ template <class T> struct ModelT : ModelBase {
             J.f[n] = p;
             J.kind_bottom[n] = OM_BC_DEFAULT; J.kind_top[n] = OM_BC_DEFAULT;
         }
+        add_pending_p(J);
+        if (J.nf == 0) return;
         PhaseScope ps(this, OM_PHASE_HALOS);
         launch_halos(g, J, st);
     }
@@ -624,8 +639,14 @@ template <class T> struct ModelT : ModelBase {
             launch_poisson(P, ctx(), (const T *)nullptr, s.pNHS, 0, dtp, st);
             if (tune_refine) refine(s.pNHS, g.sy, g.sz, nullptr, (double)dtp);
         }
-        halos_aux({s.pNHS});
-        { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, 0, st); }
+        // periodic x and y without an immersed bottom or a refinement pass: the correction reads
+        // p's neighbours across the edges by wrapped index, and p's halo fill joins the next
+        // halo launch (update_state's), one launch fewer per substage
+        const bool wrap = !tune_refine && !g.kbot && g.tx == OM_PERIODIC && g.ty == OM_PERIODIC &&
+                          (g.tz == OM_BOUNDED || g.tz == OM_PERIODIC);
+        if (!wrap) halos_aux({s.pNHS});
+        { PhaseScope ps(this, OM_PHASE_CORRECTION); launch_correct(ctx(), dtp, 0, st, wrap ? 1 : 0); }
+        p_halo_pending = wrap;
     }
 
     // ---- conjugate-gradient solve of V∇²x = b (conjugate_gradient_solver.jl:177-284) with the
@@ -1264,6 +1285,7 @@ int om_rk3_substep(om_model_t *m, double dt, double gamma, double zeta, int32_t 
         using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
         M->cg_unconverged = 0;
         M->rk3_substep(dt, (TT)gamma, (TT)zeta, first_stage != 0);
+        M->flush_p_halo();
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
         M->report_unconverged();
@@ -1277,6 +1299,7 @@ int om_pressure_correction(om_model_t *m, double dt) {
         using TT = std::remove_pointer_t<decltype(M->rhs_dense)>;
         M->cg_unconverged = 0;
         M->pressure_correction((TT)dt);
+        M->flush_p_halo();
         HIPCHECK(hipStreamSynchronize(M->st));
         HIPCHECK(hipGetLastError());
         M->report_unconverged();

@@ -692,7 +692,8 @@ template <class T> void launch_closure_tiled(const Ctx<T> &C, int kchunk, hipStr
 template <class T> void launch_hydrostatic(const Ctx<T> &C, hipStream_t st);
 template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int first, int nfields, hipStream_t st);
 // U −= dtp·∇p (phi = 0: p is the pressure) or U −= ∇ϕ (phi = 1: ϕ = p·Δt, as the reference stores it)
-template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st);
+// wrap = 1: periodic x and y without an immersed bottom; p's halos are not read
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st, int wrap = 0);
 // mask_immersed_field! of a field at location loc (0 ccc, 1 fcc, 2 cfc, 3 ccf) on an immersed grid
 template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st);
 // conjugate-gradient Poisson solver kernels (om_cg.hip)

@@ -737,6 +737,7 @@ template <class T> struct CorrectArgs {
     Ctx<T> C;
     T dtp;
     int phi;
+    int wrap;   // periodic x and y, no immersed bottom: neighbours across the edges by wrapped index
 };
 template <class T> struct KCorrect {
     static constexpr int kThreads = 256, kMinBlocks = 1;
@@ -754,15 +755,21 @@ template <class T> __device__ __forceinline__ void KCorrect<T>::run(const Correc
     const T *p = C.s.pNHS;
     const IB<T> ib(C.g);
     T pc[kCorrectLevels], pw[kCorrectLevels], ps[kCorrectLevels], U0[kCorrectLevels], U1[kCorrectLevels], U2[kCorrectLevels];
-    const T pb = p[C.at(i, j, k0 - 1)];   // the level below the chunk (a halo at the bottom wall)
+    // with wrap, the x, y and z neighbours across the domain edges are read where the halo fill of
+    // p would copy them from (the periodic image; at a Bounded bottom the zero-gradient copy of
+    // level 0), so p's halos need not be filled first; the values are the same
+    const int im = A.wrap && i == 0 ? C.g.Nx - 1 : i - 1;
+    const int jm = A.wrap && j == 0 ? C.g.Ny - 1 : j - 1;
+    const int kb = A.wrap && k0 == 0 ? (C.g.tz == OM_PERIODIC ? C.g.Nz - 1 : 0) : k0 - 1;
+    const T pb = p[C.at(i, j, kb)];   // the level below the chunk (a halo at the bottom wall)
 #pragma unroll
     for (int q = 0; q < kCorrectLevels; ++q) {
         if (q >= n) break;
         const int k = k0 + q;
         const int64_t o = C.at(i, j, k);
         pc[q] = p[o];
-        pw[q] = p[C.at(i - 1, j, k)];
-        ps[q] = p[C.at(i, j - 1, k)];
+        pw[q] = p[C.at(im, j, k)];
+        ps[q] = p[C.at(i, jm, k)];
         U0[q] = C.s.U[0][o];
         U1[q] = C.s.U[1][o];
         U2[q] = C.s.U[2][o];
@@ -872,10 +879,10 @@ template <class T> void launch_rk3(const Ctx<T> &C, T dt, T gamma, T zeta, int f
         default: launch_k<KRk3<T>>(grid, blk, 0, st, A);
     }
 }
-template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st) {
+template <class T> void launch_correct(const Ctx<T> &C, T dtp, int phi, hipStream_t st, int wrap) {
     dim3 blk(64, 4, 1);
     launch_k<KCorrect<T>>(cell_grid(C.g.Nx, C.g.Ny, (C.g.Nz + kCorrectLevels - 1) / kCorrectLevels, blk), blk, 0, st,
-                          CorrectArgs<T>{C, dtp, phi});
+                          CorrectArgs<T>{C, dtp, phi, wrap});
 }
 template <class T> void launch_mask(const DevGrid<T> &g, T *f, int loc, hipStream_t st) {
     if (!g.kbot) return;
@@ -918,7 +925,7 @@ template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, h
     template void launch_closure<T>(const Ctx<T> &, hipStream_t, bool);                            \
     template void launch_hydrostatic<T>(const Ctx<T> &, hipStream_t);                              \
     template void launch_rk3<T>(const Ctx<T> &, T, T, T, int, int, hipStream_t);                    \
-    template void launch_correct<T>(const Ctx<T> &, T, int, hipStream_t);                               \
+    template void launch_correct<T>(const Ctx<T> &, T, int, hipStream_t, int);                          \
     template void launch_mask<T>(const DevGrid<T> &, T *, int, hipStream_t);                           \
     template void launch_divergence<T>(const Ctx<T> &, T *, hipStream_t);                          \
     template void launch_halos<T>(const DevGrid<T> &, const HaloJob<T> &, hipStream_t);

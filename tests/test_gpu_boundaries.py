@@ -102,3 +102,40 @@ def test_near_wall_tendencies_read_one_halo_layer(case, kernel, float_type):
     for n in names:
         assert np.array_equal(model.Gn[n].interior(), G0[n]), f"{case} {kernel}: G{n} changed"
     model.close()
+
+
+def _check_pressure_halos(model, label):
+    """pNHS's halos are the periodic images (x, y) and the zero-gradient copies (the first layer
+    below and above a Bounded z) of its interior, bit for bit."""
+    p = model.pressures.pNHS
+    info = p.info()
+    hx, hy, hz = (1 - int(info.offset[d]) for d in range(3))   # offset: first parent index, 1 − H
+    inner = p.interior()
+    par = p.parent()
+    nx, ny, nz = inner.shape
+    zs = slice(hz, hz + nz)
+    for h in range(hx):
+        assert np.array_equal(par[h, hy:hy + ny, zs], inner[nx - hx + h]), (label, "west", h)
+        assert np.array_equal(par[hx + nx + h, hy:hy + ny, zs], inner[h]), (label, "east", h)
+    for h in range(hy):
+        assert np.array_equal(par[hx:hx + nx, h, zs], inner[:, ny - hy + h]), (label, "south", h)
+        assert np.array_equal(par[hx:hx + nx, hy + ny + h, zs], inner[:, h]), (label, "north", h)
+    assert np.array_equal(par[hx:hx + nx, hy:hy + ny, hz - 1], inner[:, :, 0]), (label, "bottom")
+    assert np.array_equal(par[hx:hx + nx, hy:hy + ny, hz + nz], inner[:, :, nz - 1]), (label, "top")
+
+
+@pytest.mark.parametrize("float_type", [np.float64, np.float32])
+def test_pressure_halos_filled_after_the_correction_reads_wrapped_neighbours(float_type):
+    """With periodic x and y the pressure correction reads p's neighbours across the edges by
+    wrapped index and p's halo fill joins the next halo launch.  The halos must still hold the
+    filled values when a call returns: after time_step! (the fill joins update_state!'s) and after
+    a bare pressure correction (filled at the end of the call), as the reference's
+    fill_halo_regions!(pNHS) after the solve leaves them (pressure_correction.jl:6-20)."""
+    grid, kw, init = convection(N=(16, 16, 16), noise_velocity=1e-2, float_type=float_type)
+    model = product_for(grid, kw, init)
+    om.time_step_(model, 1.0)
+    assert np.abs(model.pressures.pNHS.interior()).max() > 0
+    _check_pressure_halos(model, "time_step")
+    model.compute_pressure_correction(0.5)
+    _check_pressure_halos(model, "pressure_correction")
+    model.close()
