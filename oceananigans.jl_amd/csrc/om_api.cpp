@@ -532,7 +532,12 @@ template <class T> struct ModelT : ModelBase {
         p_halo_pending = false;
     }
     void flush_p_halo() { if (p_halo_pending) halos_aux({}); }
+    // the velocities' halo fill before the Poisson solve, deferred inside a time step when the
+    // solve's source reads no halo: it joins the next prognostic fill (update_state's), with the
+    // impenetrable w faces set there; nothing reads the velocity halos in between
+    bool u_halo_pending = false;
     void halos_prognostic(uint32_t mask, bool fill_normal) {
+        if (u_halo_pending) { mask |= 0x7u; fill_normal = true; u_halo_pending = false; }
         HaloJob<T> J;
         memset(&J, 0, sizeof(J));
         for (int f = 0; f < nf; ++f) {
@@ -607,12 +612,16 @@ template <class T> struct ModelT : ModelBase {
         if (J.nf > 0) { PhaseScope ps(this, OM_PHASE_HALOS); launch_halos(g, J, st); }
         compute_tendencies();
     }
-    void pressure_correction(T dtau) {
+    // defer: called inside a time step, whose update_state fills the velocity halos next
+    void pressure_correction(T dtau, bool defer = false) {
         if (g.kbot) {   // mask immersed velocities (pressure_correction.jl:7)
             PhaseScope ps(this, OM_PHASE_HALOS);
             for (int f = 0; f < 3; ++f) launch_mask(g, s.U[f], f + 1, st);
         }
-        halos_prognostic(0x7u, true);
+        if (defer && !cg && !tune_refine && g.tz == OM_BOUNDED && poisson_source_reads_no_halo(P, ctx()))
+            u_halo_pending = true;
+        else
+            halos_prognostic(0x7u, true);
         const T eps = std::numeric_limits<T>::epsilon();
         const T dtp = dtau > eps ? dtau : eps;
         if (cg) {
@@ -867,11 +876,11 @@ template <class T> struct ModelT : ModelBase {
         cg_gauged_norm(g, cg_q, emean, cg_partial, cg_out, st);
         cg_true_res_last = std::sqrt(cg_read(1));
     }
-    void rk3_substep(double dt, T gamma, T zeta, bool first) {
+    void rk3_substep(double dt, T gamma, T zeta, bool first, bool defer = false) {
         const T kdt = (T)dt;
         const T dtau = first ? (T)(dt * (double)gamma) : (T)(dt * (double)(gamma + zeta));
         { PhaseScope ps(this, OM_PHASE_RK_UPDATE); launch_rk3(ctx(), kdt, gamma, zeta, first ? 1 : 0, nf, st); }
-        pressure_correction(dtau);
+        pressure_correction(dtau, defer);
     }
     void set_refinement(int on) {
         if (on && (cg || g.kbot)) throw ArgError{"iterative refinement applies to the FFT and Fourier-tridiagonal solvers on a grid without an immersed bottom"};
@@ -898,13 +907,13 @@ template <class T> struct ModelT : ModelBase {
     // the device work of one time step: every launch depends only on Δt and the buffer assignment
     void step_launches(double dt) {
         const T g1 = T(kG1), g2 = T(kG2), g3 = T(kG3), z2 = T(kZ2), z3 = T(kZ3);
-        rk3_substep(dt, g1, T(0), true);
+        rk3_substep(dt, g1, T(0), true, true);
         cache_previous_tendencies();
         update_state();
-        rk3_substep(dt, g2, z2, false);
+        rk3_substep(dt, g2, z2, false, true);
         cache_previous_tendencies();
         update_state();
-        rk3_substep(dt, g3, z3, false);
+        rk3_substep(dt, g3, z3, false, true);
         cache_previous_tendencies();
         update_state();
     }

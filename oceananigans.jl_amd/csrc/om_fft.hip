@@ -211,8 +211,11 @@ template <class T> __device__ __forceinline__ void KXFwd<T>::run(const XFwdArgs<
     T *ap = reinterpret_cast<T *>(a);
     // without an immersed bottom the source is ∇·U of three non-Flat axes on a whole row: each
     // thread forms its (at most kFastX) elements from loads all issued before any arithmetic,
-    // with no per-element branch (the general path below tests the topology per element)
-    constexpr int kFastX = 8;
+    // with no per-element branch (the general path below tests the topology per element).  Its
+    // neighbours across the x and y edges are read by wrapped index, and a Bounded z's boundary
+    // faces of w as the zero the impenetrable fill writes there, so it reads no halo
+    // (poisson_source_reads_no_halo is the same test on the host)
+    constexpr int kFastX = kXFwdFastElems;
     const bool fast = !A.src && !C.g.kbot && !C.flx() && !C.fly() && !C.flz() && half && !A.scale_dzc &&
                       Nx <= kFastX * (int)(blockDim.x >> A.L.lgR);
     if (fast) {
@@ -226,6 +229,7 @@ template <class T> __device__ __forceinline__ void KXFwd<T>::run(const XFwdArgs<
         const int64_t o = C.at(0, j, k);
         const T *u = C.s.U[0] + o, *v = C.s.U[1] + o, *w = C.s.U[2] + o;
         const T *vp = C.s.U[1] + C.at(0, jp, k), *wp = w + C.g.sz;
+        const bool wb = C.g.tz == OM_BOUNDED && k == 0, wt = C.g.tz == OM_BOUNDED && k == C.g.Nz - 1;
         T su[kFastX], sq[kFastX], sv[kFastX], svp[kFastX], sw[kFastX], swp[kFastX];
 #pragma unroll
         for (int e = 0; e < kFastX; ++e) {
@@ -237,8 +241,8 @@ template <class T> __device__ __forceinline__ void KXFwd<T>::run(const XFwdArgs<
             sq[e] = in ? u[ip] : T(0);
             sv[e] = in ? v[ii] : T(0);
             svp[e] = in ? vp[ii] : T(0);
-            sw[e] = in ? w[ii] : T(0);
-            swp[e] = in ? wp[ii] : T(0);
+            sw[e] = in && !wb ? w[ii] : T(0);
+            swp[e] = in && !wt ? wp[ii] : T(0);
         }
         const T Ax = C.Axc(k), Ay = C.Ayc(k), Az = C.g.Az, iV = C.iVc(k);
 #pragma unroll
@@ -786,7 +790,16 @@ template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C,
     }
 }
 
+template <class T> bool poisson_source_reads_no_halo(const PoissonPlan<T> &P, const Ctx<T> &C) {
+    const DevGrid<T> &g = C.g;
+    if (g.kbot || g.tx == OM_FLAT || g.ty == OM_FLAT || g.tz == OM_FLAT || (g.Nx & 1) || P.tridiagonal) return false;
+    const int M = g.Nx / 2;
+    const int R = fft_lines_per_block(M, M, sizeof(C2<T>));
+    return R >= 1 && g.Nx <= kXFwdFastElems * (kFftThreads / R);
+}
+
 #ifndef OM_ONLY_DOUBLE
+template bool poisson_source_reads_no_halo<float>(const PoissonPlan<float> &, const Ctx<float> &);
 template void launch_poisson_residual<float>(const Ctx<float> &, const RefineOp &, const float *, int64_t, int64_t,
                                              const double *, double, float *, hipStream_t);
 template void launch_add_correction<float>(const DevGrid<float> &, float *, int64_t, int64_t, const float *, hipStream_t);
@@ -794,6 +807,7 @@ template void launch_poisson<float>(const PoissonPlan<float> &, const Ctx<float>
                                     hipStream_t);
 #endif
 #ifndef OM_ONLY_FLOAT
+template bool poisson_source_reads_no_halo<double>(const PoissonPlan<double> &, const Ctx<double> &);
 template void launch_poisson_residual<double>(const Ctx<double> &, const RefineOp &, const double *, int64_t, int64_t,
                                               const double *, double, double *, hipStream_t);
 template void launch_add_correction<double>(const DevGrid<double> &, double *, int64_t, int64_t, const double *, hipStream_t);

@@ -775,6 +775,10 @@ template <class T> void launch_divergence(const Ctx<T> &C, T *out, hipStream_t s
 template <class T> void launch_halos(const DevGrid<T> &g, const HaloJob<T> &J, hipStream_t st);
 // Solve ∇²ϕ = source; writes ϕ / divisor into `out` (the nonhydrostatic pressure p/Δt when
 // called from the pressure correction, divisor = 1 otherwise).
+// elements per thread of the X-forward pass's halo-free source path
+constexpr int kXFwdFastElems = 8;
+// whether the X-forward pass forms ∇·U of the state without reading a halo (its fast path)
+template <class T> bool poisson_source_reads_no_halo(const PoissonPlan<T> &P, const Ctx<T> &C);
 template <class T> void launch_poisson(const PoissonPlan<T> &P, const Ctx<T> &C, const T *src, T *out, int dense_out,
                                        T divisor, hipStream_t st);
 
