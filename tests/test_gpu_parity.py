@@ -307,6 +307,31 @@ def test_tendencies_identical_inputs_fp64(case):
         assert rel_err(model.pressures.pHY.interior(), orc.get(om._abi.OM_FIELD_PHY)) <= FP64_TOL
 
 
+
+def test_one_step_from_an_unprojected_state_fp64():
+    """set! without the projection, then one time step: the first substage's pressure solve starts
+    from a divergent state, with the velocities' halo fill before the solve joined to update_state's
+    (the X-forward pass reads no halo: wrapped x and y neighbours, the Bounded faces of w as the
+    zero the impenetrable fill writes there; set! already leaves them zero).  The state after the
+    step equals the oracle's (FP64 ≤ 1e-12; pressure_correction.jl:6-20)."""
+    grid, kw, init = convection(N=(16, 16, 16), noise_velocity=1e-2)
+    model = om.NonhydrostaticModel(grid, **kw)
+    om.set_(model, enforce_incompressibility=False, **init)
+    assert np.abs(model.velocities.w.interior()[:, :, 0]).max() == 0
+    from oracle.oracle import OracleModel
+    cfg, keep = om.build_config(grid, **kw)
+    orc = OracleModel(cfg, grid)
+    orc._keep = keep
+    names = prognostic_names(kw)
+    for name, val in init.items():
+        orc.set(names.index(name), om.models._evaluate(grid, name, val))
+    orc.set_finalize(False)
+    om.time_step_(model, 1.0)
+    orc.time_step(1.0, 1)
+    assert np.abs(model.velocities.w.interior()[:, :, 0]).max() == 0
+    compare_state(model, orc, kw, fields=("state",), tol=FP64_TOL, label="one step from an unprojected state")
+    model.close()
+
 TILED_CASES = {
     "convection_noisy_fp64": (lambda: convection(N=(16, 16, 16), noise_velocity=1e-2), 1e-14),
     "wind_convection_ragged_fp64": (lambda: wind_convection(N=(20, 17, 13)), 1e-14),
